@@ -1,0 +1,104 @@
+/*
+ * qzbsc — MI355X (gfx950) block-sorting engine for qz's BSC stream path.
+ *
+ * Part 1 (libbsc drop-in): the four symbols qz-lib binds through FFI, with
+ * libbsc's signatures, error codes and buffer contract. A build of this
+ * library named liblibbsc.a / liblibbsc.so links in place of libbsc.
+ *
+ * Part 2 (BWT stages): libbsc's bwt.h entry points, plus batch and
+ * device-resident forms used by the batched and serving paths.
+ *
+ * Error codes (crates/qz-lib/src/compression/bsc.rs:18-24):
+ *   0 no error, -1 bad parameter, -2 not enough memory, -3 not compressible,
+ *   -4 not supported, -5 unexpected end of block, -6 data corrupt;
+ *   -7 GPU runtime error (this library only).
+ */
+#ifndef QZBSC_H
+#define QZBSC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QZBSC_API __attribute__((visibility("default")))
+
+#define LIBBSC_NO_ERROR 0
+#define LIBBSC_BAD_PARAMETER -1
+#define LIBBSC_NOT_ENOUGH_MEMORY -2
+#define LIBBSC_NOT_COMPRESSIBLE -3
+#define LIBBSC_NOT_SUPPORTED -4
+#define LIBBSC_UNEXPECTED_EOB -5
+#define LIBBSC_DATA_CORRUPT -6
+#define QZBSC_GPU_ERROR -7
+
+#define LIBBSC_BLOCKSORTER_BWT 1
+#define LIBBSC_CODER_QLFC_STATIC 1
+#define LIBBSC_CODER_QLFC_ADAPTIVE 2
+#define LIBBSC_FEATURE_FASTMODE 1
+#define LIBBSC_FEATURE_MULTITHREADING 2
+#define LIBBSC_FEATURE_CUDA 8 /* "use the GPU" — selects the HIP path here */
+#define LIBBSC_HEADER_SIZE 28
+
+/* ---- Part 1: libbsc drop-in ------------------------------------------- */
+
+/* Replaces libbsc bsc_init; bound at crates/qz-lib/src/compression/bsc.rs:94,
+ * called once through std::sync::Once (bsc.rs:128-141). */
+QZBSC_API int bsc_init(int features);
+
+/* Replaces libbsc bsc_compress; bound at bsc.rs:97-106, called at
+ * bsc.rs:172-183 and bsc.rs:234-247. `output` holds n + 28 + 1024 bytes
+ * (bsc.rs:170). Returns the block size or a negative error. */
+QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize,
+                           int lzpMinLen, int blockSorter, int coder, int features);
+
+/* Replaces libbsc bsc_block_info; bound at bsc.rs:109-115, called at
+ * bsc.rs:401-409 with the whole block as the header. */
+QZBSC_API int bsc_block_info(const unsigned char* blockHeader, int headerSize, int* pBlockSize, int* pDataSize,
+                             int features);
+
+/* Replaces libbsc bsc_decompress; bound at bsc.rs:118-124, called at
+ * bsc.rs:421-429 with outputSize == dataSize. Returns 0 on success. */
+QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
+                             int features);
+
+/* ---- Part 2: BWT stages ------------------------------------------------ */
+
+/* libbsc bwt.h bsc_bwt_encode (called inside libbsc's bsc_compress, which
+ * qz reaches at bsc.rs:172-183): in-place forward BWT; returns the primary
+ * index and writes (n-1)/r trailer indexes, r the largest power of two
+ * <= n/8. */
+QZBSC_API int qzbsc_bwt_encode(unsigned char* T, int n, unsigned char* num_indexes, int* indexes, int features);
+
+/* libbsc bwt.h bsc_bwt_decode (inside bsc_decompress, bsc.rs:421-429):
+ * in-place inverse BWT. */
+QZBSC_API int qzbsc_bwt_decode(unsigned char* T, int n, int index, unsigned char num_indexes, int* indexes,
+                               int features);
+
+/* Batch forms of the two calls above: nblocks blocks of sizes[b] bytes laid
+ * end to end in `in`/`out` (host memory), sorted in one device pass.
+ * primary[b] receives the block's primary index; aux (nblocks x 256 ints,
+ * may be NULL) receives I[k] = 1 + rank of suffix k*r, k = 0..(n-1)/r.
+ * No counterpart in the reference: they replace qz's rayon fan-out of
+ * per-block calls (bsc.rs:282-288, decompress_impl.rs:117-120). */
+QZBSC_API int qzbsc_bwt_encode_batch(const unsigned char* in, unsigned char* out, const int* sizes, int nblocks,
+                                     int* primary, int* aux);
+QZBSC_API int qzbsc_bwt_decode_batch(const unsigned char* in, unsigned char* out, const int* sizes, int nblocks,
+                                     const int* primary);
+
+/* Device-resident batch forms: d_in / d_out are device pointers. gpu_ms
+ * (may be NULL) receives the device time of the transform measured with
+ * HIP events on the engine's stream. No counterpart in the reference. */
+QZBSC_API int qzbsc_dev_bwt_encode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
+                                         int* primary, int* aux, float* gpu_ms);
+QZBSC_API int qzbsc_dev_bwt_decode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
+                                         const int* primary, float* gpu_ms);
+
+/* Pool control: at most n engines (streams + resident workspaces). */
+QZBSC_API int qzbsc_set_max_engines(int n);
+/* 1 if a HIP device is usable, else 0. */
+QZBSC_API int qzbsc_gpu_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+// C-ABI for the BWT stages: per-block calls with libbsc's bwt.h signatures,
+// batch calls over host buffers, and device-resident batch calls for
+// callers whose blocks already sit in HBM.
+//
+// Engines (each with its own stream and resident workspace) live in a pool,
+// so concurrent callers (qz drives libbsc from many rayon workers,
+// crates/qz-lib/src/compression/mod.rs:402-404) each get one.
+#include "engine.h"
+#include "pool.h"
+#include "../../include/qzbsc.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace qz {
+
+static int sample_rate(int n)
+{
+    int m = n / 8;
+    m |= m >> 1; m |= m >> 2; m |= m >> 4; m |= m >> 8; m |= m >> 16;
+    m >>= 1;
+    return m + 1;
+}
+
+static int check_sizes(const int* sizes, int nblocks, std::vector<uint32_t>& offs)
+{
+    if (nblocks <= 0 || !sizes) return QZ_BAD_PARAMETER;
+    offs.assign(nblocks + 1, 0);
+    uint64_t tot = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        if (sizes[b] < 1) return QZ_BAD_PARAMETER;
+        offs[b] = (uint32_t)tot;
+        tot += (uint64_t)sizes[b];
+        if (tot >= 0x7fffffffull) return QZ_BAD_PARAMETER;
+    }
+    offs[nblocks] = (uint32_t)tot;
+    return 0;
+}
+
+// Forward BWT of a batch whose text already sits in device memory.
+static int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
+                            int* primary, int* aux, float* gpu_ms)
+{
+    const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
+    hipStream_t s = sl.stream;
+    QZ_TRY(sl.fwd.reserve(n, nblk));
+    QZ_HIP(hipMemcpyAsync(sl.fwd.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipEventRecord(sl.ev0, s));
+    QZ_TRY(sl.fwd.encode_device(d_in, d_out, n, nblk, s));
+    QZ_HIP(hipEventRecord(sl.ev1, s));
+    std::vector<int32_t> idx((size_t)nblk * 256);
+    QZ_HIP(hipMemcpyAsync(idx.data(), sl.fwd.idx_, idx.size() * 4, hipMemcpyDeviceToHost, s));
+    QZ_HIP(hipStreamSynchronize(s));
+    if (gpu_ms) QZ_HIP(hipEventElapsedTime(gpu_ms, sl.ev0, sl.ev1));
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const int nb = (int)(offs[b + 1] - offs[b]);
+        const int cnt = (nb - 1) / sample_rate(nb);
+        primary[b] = idx[(size_t)b * 256];
+        if (aux)
+            for (int k = 0; k <= cnt && k < 256; ++k) aux[(size_t)b * 256 + k] = idx[(size_t)b * 256 + k];
+    }
+    return 0;
+}
+
+static int inv_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
+                            const int* primary, float* gpu_ms)
+{
+    const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const int nb = (int)(offs[b + 1] - offs[b]);
+        if (primary[b] < 1 || primary[b] > nb) return QZ_BAD_PARAMETER;
+    }
+    hipStream_t s = sl.stream;
+    QZ_TRY(sl.inv.reserve(n, nblk));
+    QZ_HIP(hipMemcpyAsync(sl.inv.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipMemcpyAsync(sl.inv.prim_, primary, (size_t)nblk * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipEventRecord(sl.ev0, s));
+    QZ_TRY(sl.inv.decode_device(d_in, d_out, n, nblk, offs, s));
+    QZ_HIP(hipEventRecord(sl.ev1, s));
+    QZ_HIP(hipStreamSynchronize(s));
+    if (gpu_ms) QZ_HIP(hipEventElapsedTime(gpu_ms, sl.ev0, sl.ev1));
+    return 0;
+}
+
+}  // namespace qz
+
+using namespace qz;
+
+extern "C" {
+
+QZBSC_API int qzbsc_bwt_encode_batch(const unsigned char* in, unsigned char* out, const int* sizes, int nblocks,
+                                     int* primary, int* aux)
+{
+    std::vector<uint32_t> offs;
+    QZ_TRY(check_sizes(sizes, nblocks, offs));
+    if (!in || !out || !primary) return QZ_BAD_PARAMETER;
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    Slot& sl = g.slot();
+    const uint32_t n = offs.back();
+    QZ_TRY(sl.fwd.reserve(n, (uint32_t)nblocks));
+    QZ_HIP(hipMemcpyAsync(sl.fwd.text_, in, n, hipMemcpyHostToDevice, sl.stream));
+    QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, primary, aux, nullptr));
+    QZ_HIP(hipMemcpyAsync(out, sl.fwd.out_, n, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipStreamSynchronize(sl.stream));
+    return 0;
+}
+
+QZBSC_API int qzbsc_bwt_decode_batch(const unsigned char* in, unsigned char* out, const int* sizes, int nblocks,
+                                     const int* primary)
+{
+    std::vector<uint32_t> offs;
+    QZ_TRY(check_sizes(sizes, nblocks, offs));
+    if (!in || !out || !primary) return QZ_BAD_PARAMETER;
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    Slot& sl = g.slot();
+    const uint32_t n = offs.back();
+    QZ_TRY(sl.inv.reserve(n, (uint32_t)nblocks));
+    QZ_HIP(hipMemcpyAsync(sl.inv.U_, in, n, hipMemcpyHostToDevice, sl.stream));
+    QZ_TRY(inv_batch_device(sl, sl.inv.U_, sl.inv.T_, offs, primary, nullptr));
+    QZ_HIP(hipMemcpyAsync(out, sl.inv.T_, n, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipStreamSynchronize(sl.stream));
+    return 0;
+}
+
+QZBSC_API int qzbsc_dev_bwt_encode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks, int* primary,
+                                         int* aux, float* gpu_ms)
+{
+    std::vector<uint32_t> offs;
+    QZ_TRY(check_sizes(sizes, nblocks, offs));
+    if (!d_in || !d_out || !primary) return QZ_BAD_PARAMETER;
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    return fwd_batch_device(g.slot(), (const uint8_t*)d_in, (uint8_t*)d_out, offs, primary, aux, gpu_ms);
+}
+
+QZBSC_API int qzbsc_dev_bwt_decode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
+                                         const int* primary, float* gpu_ms)
+{
+    std::vector<uint32_t> offs;
+    QZ_TRY(check_sizes(sizes, nblocks, offs));
+    if (!d_in || !d_out || !primary) return QZ_BAD_PARAMETER;
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    return inv_batch_device(g.slot(), (const uint8_t*)d_in, (uint8_t*)d_out, offs, primary, gpu_ms);
+}
+
+// libbsc bwt.h: int bsc_bwt_encode(unsigned char* T, int n, unsigned char*
+// num_indexes, int* indexes, int features). In place; returns the primary
+// index; writes (n-1)/r trailer indexes I[k+1] - 1.
+QZBSC_API int qzbsc_bwt_encode(unsigned char* T, int n, unsigned char* num_indexes, int* indexes, int features)
+{
+    (void)features;
+    if (!T || n < 1 || !num_indexes || !indexes) return QZ_BAD_PARAMETER;
+    int primary = 0;
+    std::vector<int> aux(256);
+    QZ_TRY(qzbsc_bwt_encode_batch(T, T, &n, 1, &primary, aux.data()));
+    const int cnt = (n - 1) / sample_rate(n);
+    *num_indexes = (unsigned char)cnt;
+    for (int t = 0; t < cnt; ++t) indexes[t] = aux[t + 1] - 1;
+    return primary;
+}
+
+// libbsc bwt.h: int bsc_bwt_decode(unsigned char* T, int n, int index,
+// unsigned char num_indexes, int* indexes, int features). In place.
+QZBSC_API int qzbsc_bwt_decode(unsigned char* T, int n, int index, unsigned char num_indexes, int* indexes,
+                               int features)
+{
+    (void)num_indexes;
+    (void)indexes;
+    (void)features;
+    if (!T || n < 0 || index <= 0 || index > n) return QZ_BAD_PARAMETER;
+    if (n <= 1) return QZ_NO_ERROR;
+    return qzbsc_bwt_decode_batch(T, T, &n, 1, &index);
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Shared helpers for the gfx950 block-sorting kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#define QZ_WAVE 64
+
+// Error codes follow libbsc's (crates/qz-lib/src/compression/bsc.rs:18-24);
+// GPU runtime failures map to -7 (below libbsc's range).
+#define QZ_NO_ERROR 0
+#define QZ_BAD_PARAMETER -1
+#define QZ_NOT_ENOUGH_MEMORY -2
+#define QZ_NOT_COMPRESSIBLE -3
+#define QZ_NOT_SUPPORTED -4
+#define QZ_UNEXPECTED_EOB -5
+#define QZ_DATA_CORRUPT -6
+#define QZ_GPU_ERROR -7
+
+#define QZ_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            fprintf(stderr, "qzbsc: HIP error %s at %s:%d\n", hipGetErrorString(_e), __FILE__, \
+                    __LINE__);                                                                \
+            return QZ_GPU_ERROR;                                                              \
+        }                                                                                     \
+    } while (0)
+
+#define QZ_TRY(expr)              \
+    do {                          \
+        int _rc = (expr);         \
+        if (_rc != 0) return _rc; \
+    } while (0)
+
+static inline uint32_t qz_div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Number of set bits of `mask` in lanes strictly below this lane.
+__device__ __forceinline__ uint32_t popc_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}

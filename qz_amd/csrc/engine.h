@@ -1,0 +1,78 @@
+// Device engines for the forward and inverse BWT. Each engine owns its HBM
+// workspace, grown on demand and kept resident between calls.
+#pragma once
+#include "common.h"
+#include <vector>
+
+namespace qz {
+
+class BwtEngine {
+public:
+    ~BwtEngine() { release(); }
+    int reserve(uint64_t n, uint32_t nblk);
+    void release();
+    // off_[0..nblk] must already hold the batch offsets; text/out are device
+    // buffers of n bytes (text_/out_ serve as staging for host callers).
+    int encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint32_t nblk, hipStream_t s);
+
+    uint8_t* text_ = nullptr;    // input symbols
+    uint8_t* out_ = nullptr;     // BWT output
+    uint32_t* off_ = nullptr;    // block offsets (nblk + 1)
+    int32_t* idx_ = nullptr;     // per block: [0] primary, [k] = 1 + rank(k*r)
+    uint32_t* sa_ = nullptr;
+    uint32_t* rank_ = nullptr;
+    int rounds_ = 0;
+    uint32_t k0_ = 0;
+    uint64_t cap_ = 0;
+    uint32_t cap_blk_ = 0;
+
+private:
+    int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
+    uint32_t* act_ = nullptr;
+    uint32_t* bufA_ = nullptr;
+    uint32_t* bufB_ = nullptr;
+    uint64_t* keys0_ = nullptr;
+    uint64_t* keys1_ = nullptr;
+    uint32_t* vals0_ = nullptr;
+    uint32_t* vals1_ = nullptr;
+    uint32_t* tmp_ = nullptr;
+    uint32_t* hist_ = nullptr;
+    void* pinned_ = nullptr;
+};
+
+class IbwtEngine {
+public:
+    ~IbwtEngine() { release(); }
+    int reserve(uint64_t n, uint32_t nblk);
+    void release();
+    // off_[0..nblk] and prim_[0..nblk) must hold the batch; U/T are device
+    // buffers of n bytes (U_/T_ serve as staging for host callers).
+    int decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t nblk, const std::vector<uint32_t>& offs,
+                      hipStream_t s);
+
+    uint8_t* U_ = nullptr;
+    uint8_t* T_ = nullptr;
+    uint32_t* off_ = nullptr;
+    uint32_t* prim_ = nullptr;
+    uint32_t split_ = 32;
+    uint64_t cap_ = 0;
+    uint32_t cap_blk_ = 0;
+
+private:
+    uint64_t* E_ = nullptr;
+    uint32_t* hist_ = nullptr;
+    uint32_t* tmp_ = nullptr;
+    void* tiles_ = nullptr;
+    uint32_t* sbase_ = nullptr;
+    uint32_t* nxt0_ = nullptr;
+    uint32_t* nxt1_ = nullptr;
+    uint32_t* len_ = nullptr;
+    uint32_t* d0_ = nullptr;
+    uint32_t* d1_ = nullptr;
+    uint32_t* flag_ = nullptr;
+    void* pinned_ = nullptr;
+    uint64_t cap_tiles_ = 0;
+    uint64_t cap_split_ = 0;
+};
+
+}  // namespace qz

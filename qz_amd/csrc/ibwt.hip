@@ -1,0 +1,316 @@
+// Inverse BWT on gfx950 by LF-mapping over a batch of blocks.
+//
+// Replaces libbsc's bsc_bwt_decode (libsais unbwt; third_party, linked via
+// crates/qz-lib/build.rs:19-37) on the decode path of
+// crates/qz-lib/src/compression/bsc.rs:384-436.
+//
+// Block b (n bytes, primary p) has n+1 rows; row p holds the sentinel. For a
+// real symbol at BWT index u (row = u < p ? u : u+1),
+//     LF(row) = 1 + #(symbols < c in the block) + #(c in U[0..u)),
+// i.e. the row's position in a stable counting sort of U, shifted by one for
+// the sentinel row; LF(p) = 0. Walking LF from row 0 emits T backwards.
+//
+// Phases:
+//   1. block-aligned tiles of 4096 symbols: per-tile symbol histograms,
+//      one exclusive scan laid out [block][symbol][tile], then a stable
+//      wave-ballot ranking pass that writes one 64-bit entry per row:
+//      LF (bits 0..31) | symbol (32..39) | splitter flag (40).
+//   2. every S-th row is a splitter; each splitter walks LF until the next
+//      splitter, recording the segment length and successor.
+//   3. list ranking of the splitter lists by pointer jumping gives each
+//      segment its distance from the chain start.
+//   4. each splitter walks its segment again and writes its symbols.
+#include "prims.h"
+#include "engine.h"
+
+#include <vector>
+#include <algorithm>
+
+namespace qz {
+
+constexpr int kIThreads = 256;
+constexpr uint64_t kSplitFlag = 1ull << 40;
+constexpr uint32_t kNil = 0xffffffffu;
+
+struct TileDesc {
+    uint32_t start;     // first BWT index (global)
+    uint32_t len;       // symbols in the tile (<= kSortTile)
+    uint32_t block;
+    uint32_t hbase;     // base of this block's [symbol][tile] histogram region
+    uint32_t ntiles;    // tiles of this block
+    uint32_t tlocal;    // tile index within the block
+};
+
+__global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict__ U,
+                                                        const TileDesc* __restrict__ tiles,
+                                                        uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t cnt[4][256];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+    for (int w = 0; w < 4; ++w) cnt[w][tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < td.len; i += kIThreads) atomicAdd(&cnt[wave][U[td.start + i]], 1u);
+    __syncthreads();
+    const uint32_t t = cnt[0][tid] + cnt[1][tid] + cnt[2][tid] + cnt[3][tid];
+    hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] = t;
+}
+
+// hist has been exclusive-scanned: entry = off[b] + #(< c in block) + #(c in
+// earlier tiles of the block).
+__global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict__ U,
+                                                        const TileDesc* __restrict__ tiles,
+                                                        const uint32_t* __restrict__ hist,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ prim,
+                                                        uint32_t split, uint64_t* __restrict__ E)
+{
+    constexpr int W = kIThreads / QZ_WAVE;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wh[W][256];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+    const uint32_t ob = off[td.block];
+    const uint32_t p = prim[td.block];
+    const uint32_t rb = ob + td.block;  // row base of this block
+    run[tid] = hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] - ob;
+    for (uint32_t it = 0; it < td.len; it += kIThreads) {
+        const uint32_t i = it + tid;
+        const bool valid = i < td.len;
+        const uint32_t c = valid ? U[td.start + i] : 0u;
+#pragma unroll
+        for (int w = 0; w < W; ++w) wh[w][tid] = 0;
+        __syncthreads();
+        uint64_t peer = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (c >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peer &= bit ? m : ~m;
+        }
+        const uint32_t below = popc_below(peer);
+        if (valid && below == 0) wh[wave][c] = (uint32_t)__popcll(peer);
+        __syncthreads();
+        {
+            uint32_t r = run[tid];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint32_t t = wh[w][tid];
+                wh[w][tid] = r;
+                r += t;
+            }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t u = td.start - ob + i;          // local BWT index
+            const uint32_t row = u < p ? u : u + 1u;
+            const uint32_t lf = 1u + wh[wave][c] + below;  // local row
+            uint64_t e = (uint64_t)lf | ((uint64_t)c << 32);
+            if (row % split == 0) e |= kSplitFlag;
+            E[rb + row] = e;
+        }
+        __syncthreads();
+    }
+}
+
+// Sentinel rows: LF(p) = 0.
+__global__ void k_ib_sentinel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ prim, uint32_t nblk,
+                              uint32_t split, uint64_t* __restrict__ E)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const uint32_t p = prim[b];
+    E[off[b] + b + p] = (p % split == 0) ? kSplitFlag : 0ull;
+}
+
+__device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ base, uint32_t nblk, uint32_t i)
+{
+    uint32_t lo = 0, hi = nblk;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Phase 2: walk from each splitter to the next one.
+__global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restrict__ E,
+                                                         const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ sbase, uint32_t nblk,
+                                                         uint32_t nsplit, uint32_t split,
+                                                         uint32_t* __restrict__ nxt, uint32_t* __restrict__ len)
+{
+    const uint32_t sid = blockIdx.x * kIThreads + threadIdx.x;
+    if (sid >= nsplit) return;
+    const uint32_t b = find_block_u32(sbase, nblk, sid);
+    const uint32_t rb = off[b] + b;
+    const uint32_t lr0 = (sid - sbase[b]) * split;
+    uint64_t e = E[rb + lr0];
+    uint32_t l = 0;
+    uint32_t x;
+    for (;;) {
+        ++l;
+        x = (uint32_t)e;
+        const uint64_t en = E[rb + x];
+        if (en & kSplitFlag) break;
+        e = en;
+    }
+    len[sid] = l;
+    nxt[sid] = (x == 0) ? kNil : sbase[b] + x / split;
+}
+
+// Phase 3: pointer jumping; dist[s] ends as the row count from s to the end
+// of its chain (inclusive).
+__global__ __launch_bounds__(kIThreads) void k_ib_jump(const uint32_t* __restrict__ nin,
+                                                        const uint32_t* __restrict__ din, uint32_t* __restrict__ nout,
+                                                        uint32_t* __restrict__ dout, uint32_t nsplit,
+                                                        uint32_t* __restrict__ any)
+{
+    const uint32_t s = blockIdx.x * kIThreads + threadIdx.x;
+    if (s >= nsplit) return;
+    const uint32_t n1 = nin[s];
+    if (n1 == kNil) {
+        nout[s] = kNil;
+        dout[s] = din[s];
+        return;
+    }
+    dout[s] = din[s] + din[n1];
+    const uint32_t n2 = nin[n1];
+    nout[s] = n2;
+    if (n2 != kNil) *any = 1u;
+}
+
+// Phase 4: emit symbols. Chain step t (0-based from row 0) writes T[n-1-t].
+__global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restrict__ E,
+                                                         const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ sbase, uint32_t nblk,
+                                                         uint32_t nsplit, uint32_t split,
+                                                         const uint32_t* __restrict__ len,
+                                                         const uint32_t* __restrict__ dist,
+                                                         uint8_t* __restrict__ T)
+{
+    const uint32_t sid = blockIdx.x * kIThreads + threadIdx.x;
+    if (sid >= nsplit) return;
+    const uint32_t b = find_block_u32(sbase, nblk, sid);
+    const uint32_t ob = off[b];
+    const uint32_t n = off[b + 1] - ob;
+    const uint32_t rb = ob + b;
+    const uint32_t lr0 = (sid - sbase[b]) * split;
+    const uint32_t l = len[sid];
+    uint32_t t = n + 1u - dist[sid];
+    uint64_t e = E[rb + lr0];
+    for (uint32_t k = 0; k < l; ++k, ++t) {
+        if (t < n) T[ob + n - 1u - t] = (uint8_t)(e >> 32);
+        e = E[rb + (uint32_t)e];
+    }
+}
+
+int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
+{
+    if (n <= cap_ && nblk <= cap_blk_) return 0;
+    release();
+    const uint64_t c = std::max<uint64_t>(n, 1024);
+    const uint32_t cb = std::max<uint32_t>(nblk, 1);
+    const uint64_t rows = c + cb;
+    const uint64_t tiles = c / kSortTile + cb + 1;
+    const uint64_t nsplit = rows / 16 + cb + 1;  // smallest split step supported is 16
+    QZ_HIP(hipMalloc(&U_, c + 64));
+    QZ_HIP(hipMalloc(&T_, c + 64));
+    QZ_HIP(hipMalloc(&E_, rows * 8));
+    QZ_HIP(hipMalloc(&hist_, tiles * 256 * 4));
+    QZ_HIP(hipMalloc(&tmp_, (scan_tmp_words(tiles * 256) + 64) * 4));
+    QZ_HIP(hipMalloc(&tiles_, tiles * sizeof(TileDesc)));
+    QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
+    QZ_HIP(hipMalloc(&sbase_, (cb + 1) * 4));
+    QZ_HIP(hipMalloc(&prim_, cb * 4));
+    QZ_HIP(hipMalloc(&nxt0_, nsplit * 4));
+    QZ_HIP(hipMalloc(&nxt1_, nsplit * 4));
+    QZ_HIP(hipMalloc(&len_, nsplit * 4));
+    QZ_HIP(hipMalloc(&d0_, nsplit * 4));
+    QZ_HIP(hipMalloc(&d1_, nsplit * 4));
+    QZ_HIP(hipMalloc(&flag_, 64));
+    QZ_HIP(hipHostMalloc(&pinned_, 64));
+    cap_ = c;
+    cap_blk_ = cb;
+    cap_tiles_ = tiles;
+    cap_split_ = nsplit;
+    return 0;
+}
+
+void IbwtEngine::release()
+{
+    void* ps[] = {U_, T_, E_, hist_, tmp_, tiles_, off_, sbase_, prim_, nxt0_, nxt1_, len_, d0_, d1_, flag_};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (pinned_) (void)hipHostFree(pinned_);
+    U_ = T_ = nullptr;
+    E_ = nullptr;
+    hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = nullptr;
+    tiles_ = nullptr;
+    pinned_ = nullptr;
+    cap_ = 0;
+    cap_blk_ = 0;
+}
+
+// off_ and prim_ are staged; reads U, writes T (device).
+int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t nblk, const std::vector<uint32_t>& offs,
+                              hipStream_t s)
+{
+    const uint32_t split = split_;
+    // block-aligned tiles
+    std::vector<TileDesc> td;
+    td.reserve(n / kSortTile + nblk + 1);
+    uint32_t hbase = 0;
+    std::vector<uint32_t> sb(nblk + 1);
+    uint32_t nsplit = 0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t o = offs[b], nb = offs[b + 1] - o;
+        const uint32_t nt = qz_div_up(nb, kSortTile);
+        for (uint32_t t = 0; t < nt; ++t) {
+            TileDesc d;
+            d.start = o + t * kSortTile;
+            d.len = std::min<uint32_t>(kSortTile, nb - t * kSortTile);
+            d.block = b;
+            d.hbase = hbase;
+            d.ntiles = nt;
+            d.tlocal = t;
+            td.push_back(d);
+        }
+        hbase += nt * 256;
+        sb[b] = nsplit;
+        nsplit += nb / split + 1;
+    }
+    sb[nblk] = nsplit;
+    if (td.size() > cap_tiles_ || nsplit > cap_split_) return QZ_BAD_PARAMETER;
+    QZ_HIP(hipMemcpyAsync(tiles_, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+    QZ_HIP(hipMemcpyAsync(sbase_, sb.data(), sb.size() * 4, hipMemcpyHostToDevice, s));
+    const uint32_t ntiles = (uint32_t)td.size();
+    hipLaunchKernelGGL(k_ib_hist, dim3(ntiles), dim3(kIThreads), 0, s, U, (const TileDesc*)tiles_, hist_);
+    QZ_HIP(hipGetLastError());
+    QZ_TRY((device_scan<OpAdd, false>(hist_, hist_, hbase, tmp_, s)));
+    hipLaunchKernelGGL(k_ib_rank, dim3(ntiles), dim3(kIThreads), 0, s, U, (const TileDesc*)tiles_, hist_, off_, prim_, split, E_);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, s, off_, prim_, nblk, split, E_);
+    QZ_HIP(hipGetLastError());
+    const uint32_t gs = qz_div_up(nsplit, kIThreads);
+    hipLaunchKernelGGL(k_ib_walk1, dim3(gs), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_);
+    QZ_HIP(hipGetLastError());
+    QZ_HIP(hipMemcpyAsync(d0_, len_, (size_t)nsplit * 4, hipMemcpyDeviceToDevice, s));
+    uint32_t *na = nxt0_, *nb_ = nxt1_, *da = d0_, *db = d1_;
+    // ceil(log2(nsplit)) + 1 rounds always suffice
+    int rounds = 1;
+    while ((1u << rounds) < nsplit && rounds < 31) ++rounds;
+    for (int r = 0; r <= rounds; ++r) {
+        hipLaunchKernelGGL(k_ib_jump, dim3(gs), dim3(kIThreads), 0, s, na, da, nb_, db, nsplit, flag_);
+        QZ_HIP(hipGetLastError());
+        std::swap(na, nb_);
+        std::swap(da, db);
+    }
+    hipLaunchKernelGGL(k_ib_walk2, dim3(gs), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, len_, da,
+                       T);
+    QZ_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace qz

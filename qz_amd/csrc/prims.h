@@ -1,0 +1,242 @@
+// Device-wide primitives used by the block-sorting kernels: multi-level scans
+// and an LSD radix sort with wave-ballot stable ranking. Everything here is
+// written for 64-lane wavefronts and 256-thread workgroups (4 waves).
+#pragma once
+#include "common.h"
+
+namespace qz {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 1024
+
+struct OpAdd {
+    static __device__ __forceinline__ uint32_t apply(uint32_t a, uint32_t b) { return a + b; }
+    static constexpr uint32_t identity = 0u;
+};
+struct OpMax {
+    static __device__ __forceinline__ uint32_t apply(uint32_t a, uint32_t b) { return a > b ? a : b; }
+    static constexpr uint32_t identity = 0u;
+};
+
+// Inclusive scan of one value per lane across the wave.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, QZ_WAVE);
+        if (lane >= (uint32_t)d) v = Op::apply(o, v);
+    }
+    return v;
+}
+
+// Block-level scan of kScanTile items; writes local scan and the block total.
+template <class Op, bool kInclusive>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, uint32_t n,
+                                                             uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t wsum[kScanThreads / QZ_WAVE];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)tid * kScanItems;
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) v[i] = (base + i < n) ? in[base + i] : Op::identity;
+    uint32_t acc = Op::identity;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) acc = Op::apply(acc, v[i]);
+    uint32_t incl = wave_inclusive<Op>(acc);
+    if (lane == QZ_WAVE - 1) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wpre = Op::identity;
+    for (uint32_t w = 0; w < wave; ++w) wpre = Op::apply(wpre, wsum[w]);
+    uint32_t excl_thread = __shfl_up(incl, 1, QZ_WAVE);
+    if (lane == 0) excl_thread = Op::identity;
+    uint32_t run = Op::apply(wpre, excl_thread);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        uint32_t nr = Op::apply(run, v[i]);
+        if (base + i < n) out[base + i] = kInclusive ? nr : run;
+        run = nr;
+    }
+    if (sums && tid == kScanThreads - 1) {
+        uint32_t tot = Op::identity;
+        for (uint32_t w = 0; w < kScanThreads / QZ_WAVE; ++w) tot = Op::apply(tot, wsum[w]);
+        sums[blockIdx.x] = tot;
+    }
+}
+
+template <class Op>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restrict__ out, uint32_t n,
+                                                             const uint32_t* __restrict__ pre)
+{
+    const uint32_t p = pre[blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    for (uint32_t i = threadIdx.x; i < kScanTile; i += kScanThreads)
+        if (base + i < n) out[base + i] = Op::apply(p, out[base + i]);
+}
+
+// Scratch words a scan of n elements needs.
+static inline size_t scan_tmp_words(uint64_t n)
+{
+    size_t words = 0;
+    while (n > (uint64_t)kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        words += n + 1;
+    }
+    return words + 1;
+}
+
+// out may alias in. If total != nullptr the grand total (sum or max) is
+// written to device memory *total.
+template <class Op, bool kInclusive>
+static int device_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp, hipStream_t s)
+{
+    if (n == 0) return 0;
+    const uint32_t tiles = qz_div_up(n, kScanTile);
+    if (tiles == 1) {
+        hipLaunchKernelGGL((k_scan_tiles<Op, kInclusive>), dim3(1), dim3(kScanThreads), 0, s, in, out,
+                           (uint32_t)n, nullptr);
+        QZ_HIP(hipGetLastError());
+        return 0;
+    }
+    uint32_t* sums = tmp;
+    hipLaunchKernelGGL((k_scan_tiles<Op, kInclusive>), dim3(tiles), dim3(kScanThreads), 0, s, in, out,
+                       (uint32_t)n, sums);
+    QZ_HIP(hipGetLastError());
+    // exclusive scan of tile totals, then apply
+    QZ_TRY((device_scan<Op, false>(sums, sums, tiles, tmp + tiles + 1, s)));
+    hipLaunchKernelGGL((k_scan_apply<Op>), dim3(tiles), dim3(kScanThreads), 0, s, out, (uint32_t)n, sums);
+    QZ_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (key, value) pairs, 8-bit digits.
+// Tile = 256 threads x 16 items, striped so that item order inside a tile is
+// index order; ranking inside a wave uses ballots (peer masks), across waves
+// a per-digit prefix in LDS. Stable.
+// ---------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kRadix = 256;
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const K* __restrict__ keys, uint32_t n,
+                                                             int shift, uint32_t* __restrict__ hist,
+                                                             uint32_t tiles)
+{
+    __shared__ uint32_t cnt[kSortThreads / QZ_WAVE][kRadix];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) cnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int it = 0; it < kSortItems; ++it) {
+        uint64_t idx = base + (uint64_t)it * kSortThreads + tid;
+        if (idx < n) atomicAdd(&cnt[wave][(uint32_t)(keys[idx] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) t += cnt[w][tid];
+    hist[(uint64_t)tid * tiles + blockIdx.x] = t;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const K* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                K* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout, uint32_t n,
+                                                                int shift,
+                                                                const uint32_t* __restrict__ offs,
+                                                                uint32_t tiles)
+{
+    constexpr int W = kSortThreads / QZ_WAVE;
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wh[W][kRadix];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+    run[tid] = offs[(uint64_t)tid * tiles + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int it = 0; it < kSortItems; ++it) {
+        const uint64_t idx = base + (uint64_t)it * kSortThreads + tid;
+        const bool valid = idx < n;
+        K k = valid ? kin[idx] : (K)0;
+        uint32_t v = valid ? vin[idx] : 0u;
+        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+#pragma unroll
+        for (int w = 0; w < W; ++w) wh[w][tid] = 0;
+        __syncthreads();
+        uint64_t peer = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peer &= bit ? m : ~m;
+        }
+        const uint32_t below = popc_below(peer);
+        if (valid && below == 0) wh[wave][d] = (uint32_t)__popcll(peer);
+        __syncthreads();
+        {
+            uint32_t r = run[tid];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint32_t c = wh[w][tid];
+                wh[w][tid] = r;
+                r += c;
+            }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = wh[wave][d] + below;
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Scratch words a radix sort of n elements needs (histograms + their scan).
+static inline size_t radix_tmp_words(uint64_t n)
+{
+    const size_t tiles = qz_div_up(n ? n : 1, kSortTile);
+    return tiles * kRadix + scan_tmp_words(tiles * kRadix) + 64;
+}
+
+// Sorts by the digits at the given shifts (LSD order: first shift first).
+// Result lands in (k0,v0) if the pass count is even, else (k1,v1); *in_alt
+// reports which.
+template <typename K>
+static int radix_sort_pairs(K* k0, uint32_t* v0, K* k1, uint32_t* v1, uint64_t n, const int* shifts,
+                            int passes, uint32_t* tmp, hipStream_t s, bool* in_alt)
+{
+    *in_alt = false;
+    if (n <= 1 || passes == 0) return 0;
+    const uint32_t tiles = qz_div_up(n, kSortTile);
+    uint32_t* hist = tmp;
+    uint32_t* stmp = tmp + (size_t)tiles * kRadix + 32;
+    K* ka = k0;
+    K* kb = k1;
+    uint32_t* va = v0;
+    uint32_t* vb = v1;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(kSortThreads), 0, s, ka, (uint32_t)n, shifts[p],
+                           hist, tiles);
+        QZ_HIP(hipGetLastError());
+        QZ_TRY((device_scan<OpAdd, false>(hist, hist, (uint64_t)tiles * kRadix, stmp, s)));
+        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(kSortThreads), 0, s, ka, va, kb, vb,
+                           (uint32_t)n, shifts[p], hist, tiles);
+        QZ_HIP(hipGetLastError());
+        K* tk = ka; ka = kb; kb = tk;
+        uint32_t* tv = va; va = vb; vb = tv;
+    }
+    *in_alt = (passes & 1) != 0;
+    return 0;
+}
+
+}  // namespace qz

@@ -1,0 +1,73 @@
+"""GPU forward/inverse BWT parity against the oracle (bit-exact)."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from qz_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    rng = random.Random(42)
+    cases = [b"ab" * 20, b"A" * 1000, b"\x00" * 64, bytes(range(256)) * 3, b"banana" * 11, b"mississippi" * 9]
+    for _ in range(20):
+        n = rng.randint(29, 5000)
+        sigma = rng.choice([1, 2, 4, 5, 40, 256])
+        cases.append(bytes(rng.randrange(sigma) for _ in range(n)))
+    for p in (1, 2, 3, 5, 8):
+        base = bytes(rng.randrange(4) + 65 for _ in range(p)) * (6000 // p)
+        b = bytearray(base)
+        b[rng.randrange(len(b))] = 90
+        cases.append(bytes(b))
+    return cases
+
+
+def _check(blocks, gpu):
+    out, prim, aux = gpu.bwt.encode_batch(blocks)
+    for b, blk in enumerate(blocks):
+        u, p, a = O.bwt_aux(blk)
+        assert np.array_equal(out[b], u), f"block {b} (n={len(blk)}) BWT differs"
+        assert prim[b] == p, f"block {b} primary {prim[b]} != {p}"
+        assert np.array_equal(aux[b], a), f"block {b} aux differs"
+    back = gpu.bwt.decode_batch(out, prim)
+    for b, blk in enumerate(blocks):
+        assert np.array_equal(back[b], np.frombuffer(bytes(blk), np.uint8)), f"block {b} round trip"
+
+
+def test_bwt_small_cases_one_by_one(gpu):
+    for blk in _cases():
+        print(f"case n={len(blk)}", flush=True)
+        _check([blk], gpu)
+
+
+def test_bwt_small_cases_batched(gpu):
+    _check(_cases(), gpu)
+
+
+def test_bwt_libbsc_wrappers(gpu):
+    blk = synth.sequence_stream(20_000, 2_000, 150, 3, 0.001).tobytes()
+    u, p, idx = gpu.bwt.bwt_encode(blk)
+    ou, op, oidx = O.bsc_bwt_encode(blk)
+    assert np.array_equal(u, ou) and p == op and np.array_equal(idx, oidx)
+    assert gpu.bwt.bwt_decode(u, p).tobytes() == blk
+
+
+def test_bwt_mixed_streams(gpu):
+    rng = np.random.default_rng(7)
+    blocks = [
+        synth.sequence_stream(200_000, 4_000, 150, 5, 0.002, 0.0001),
+        synth.header_stream(5_000, 6),
+        synth.quality_stream(3_000, 150, 7),
+        rng.integers(0, 256, size=100_000, dtype=np.uint8),
+    ]
+    _check(blocks, gpu)
+
+
+def test_bwt_config1_block(gpu):
+    """One 15,000,000-byte sequence block (1 Mbp genome, 100K x 150 bp)."""
+    blk = synth.config1_block(seed=1)
+    assert blk.size == 15_000_000
+    _check([blk], gpu)
