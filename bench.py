@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput of the BSC block-sorting hot path on one MI355X.
+
+Workload (default): the full-size sequence stream — 50 Mbp synthetic genome,
+10M x 150 bp reads, 0.2% substitutions, 0.01% N, 1.5e9 bytes — cut into
+qz's 25 MiB BSC blocks (58 blocks, crates/qz-lib/src/compression/mod.rs:
+388-413). One step = forward BWT + inverse BWT of every block, inputs
+already resident in HBM.
+
+Two forms are timed:
+  * batched  — all blocks in one device pass per direction (the value);
+  * per-block — one call per block, as qz's rayon map issues them; its
+    per-block latency gives p50/p99.
+
+The CPU baseline runs the oracle (SA-IS + LF-walk restatement of libbsc's
+BWT stages, not upstream libsais) one block per thread on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_FWD = 10  # per input byte: text 1 + SA write 4 + SA read 4 + BWT write 1
+ALG_BYTES_INV = 10  # BWT 1 + LF write 4 + LF read 4 + text 1
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(blocks, threads: int):
+    """Oracle BWT + inverse BWT, one block per thread (qz's rayon model)."""
+    from oracle import oracle as O
+    O.lib()
+    sample = blocks[:threads]
+    done = []
+
+    def work(b):
+        u, p, _ = O.bwt_aux(b)
+        back = O.unbwt(u, p)
+        done.append(np.array_equal(back, b))
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(b,)) for b in sample]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    nbytes = sum(int(b.size) for b in sample)
+    assert all(done) and len(done) == len(sample)
+    return {
+        "value": round(nbytes / dt / 1e6, 3),
+        "unit": "MB/s",
+        "cores": len(sample),
+        "kind": "port",
+        "sample": f"{len(sample)} x {sample[0].size} B sequence blocks of the same stream, forward BWT "
+                  f"(SA-IS) + inverse BWT (LF walk), one block per thread, {dt:.1f} s; restatement, "
+                  "not upstream libsais",
+        "per_core_mb_s": round(nbytes / dt / 1e6 / len(sample), 3),
+        "nproc": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-per-block", action="store_true", help="skip the per-block form")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; blocks are independent and would be "
+              "dealt to GPUs round-robin with no collective (see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import qz_amd
+    from qz_amd import synth, bwt
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    lib = qz_amd.lib()
+
+    t = time.perf_counter()
+    stream = synth.config2_sequences(args.reads, seed=2)
+    blocks = synth.cut_blocks(stream)
+    sizes = np.array([b.size for b in blocks], dtype=np.int32)
+    total = int(sizes.sum())
+    log(f"generated {total} B in {len(blocks)} blocks ({time.perf_counter() - t:.1f} s)")
+
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(stream).to(dev)
+    d_bwt = torch.empty_like(d_in)
+    d_back = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+
+    def step_batched():
+        prim, _, fms = bwt.dev_encode_batch(d_in.data_ptr(), d_bwt.data_ptr(), sizes)
+        ims = bwt.dev_decode_batch(d_bwt.data_ptr(), d_back.data_ptr(), sizes, prim)
+        return fms, ims
+
+    for _ in range(args.warmup):
+        step_batched()
+    torch.cuda.synchronize()
+    fwd_ms, inv_ms, lat = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        f, i = step_batched()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        fwd_ms.append(f)
+        inv_ms.append(i)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ok = torch.equal(d_back, d_in)
+    log(f"batched: {args.steps} steps in {wall:.3f} s, round trip ok={ok}")
+    if not ok:
+        print("bench.py: inverse BWT did not restore the input", file=sys.stderr)
+        sys.exit(1)
+    value = total * args.steps / wall / 1e6
+
+    per_block = None
+    if not args.no_per_block:
+        blat = []
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        for b in range(len(blocks)):  # warm
+            pass
+        t1 = time.perf_counter()
+        for b in range(len(blocks)):
+            ts = time.perf_counter()
+            o = int(offs[b])
+            sz = sizes[b:b + 1]
+            prim, _, _ = bwt.dev_encode_batch(d_in.data_ptr() + o, d_bwt.data_ptr() + o, sz)
+            bwt.dev_decode_batch(d_bwt.data_ptr() + o, d_back.data_ptr() + o, sz, prim)
+            blat.append((time.perf_counter() - ts) * 1e3)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t1
+        per_block = {
+            "value": round(total / pwall / 1e6, 2),
+            "unit": "MB/s",
+            "jobs_per_s": round(len(blocks) / pwall, 3),
+            "p50_ms": round(float(np.percentile(blat, 50)), 3),
+            "p99_ms": round(float(np.percentile(blat, 99)), 3),
+        }
+        log(f"per-block: {per_block}")
+
+    fwd = float(np.mean(fwd_ms)) / 1e3
+    inv = float(np.mean(inv_ms)) / 1e3
+    ach_fwd = ALG_BYTES_FWD * total / fwd / 1e9
+    ach_inv = ALG_BYTES_INV * total / inv / 1e9
+    roof = {
+        "bound": "hbm",
+        "kernel": "forward BWT (prefix-doubling suffix sort + BWT gather), one batched launch sequence",
+        "achieved": round(ach_fwd, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(ach_fwd / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "launch_ms": round(fwd * 1e3, 3),
+        "inverse": {"achieved": round(ach_inv, 2), "frac": round(ach_inv / HBM_PEAK_GBS, 5),
+                    "launch_ms": round(inv * 1e3, 3)},
+        "bwt_ibwt_frac": round((ALG_BYTES_FWD + ALG_BYTES_INV) * total / (fwd + inv) / 1e9 / HBM_PEAK_GBS, 5),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1, len(blocks)))
+        cpu = cpu_baseline(blocks, thr)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "BWT+iBWT MB/s per GPU (batched vs per-block) + p50/p99 job latency, 25MB blocks",
+        "value": round(value, 2),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": f"full-size sequence stream: 50 Mbp genome, {args.reads} x 150 bp reads, 0.2% subs, "
+                        f"0.01% N = {total} B in {len(blocks)} x 25 MiB blocks; forward BWT + inverse BWT",
+            "blocks": len(blocks),
+            "block_bytes": int(sizes.max()),
+            "seed": 2,
+        },
+        "batched": {
+            "value": round(value, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        },
+        "per_block": per_block,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if per_block:
+        out["batched_vs_per_block"] = round(value / per_block["value"], 3)
+    if cpu:
+        out["gpu_vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

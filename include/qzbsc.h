@@ -93,6 +93,19 @@ QZBSC_API int qzbsc_dev_bwt_encode_batch(const void* d_in, void* d_out, const in
 QZBSC_API int qzbsc_dev_bwt_decode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
                                          const int* primary, float* gpu_ms);
 
+/* Host-only QLFC coder stage of bsc_compress / bsc_decompress (the
+ * "[nb][segment table][segments]" payload of one block from its BWT, and
+ * back). No device needed. Counterpart: libbsc's coder.cpp
+ * bsc_coder_compress / bsc_coder_decompress, reached through bsc.rs:172-183
+ * and bsc.rs:421-429. */
+QZBSC_API int qzbsc_coder_compress(const unsigned char* bwt, int n, unsigned char* out, int cap, int features);
+QZBSC_API int qzbsc_coder_decompress(const unsigned char* in, int size, unsigned char* out, int n, int features);
+
+/* Reads the size word and symbol table opening one coded segment. Test
+ * hook with no reference counterpart: checks the segment framing against
+ * libbsc-written blocks. */
+QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, unsigned char* table, int* nsym);
+
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
 /* 1 if a HIP device is usable, else 0. */

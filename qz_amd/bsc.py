@@ -1,0 +1,166 @@
+"""Python mirror of qz-lib's BSC wrapper (crates/qz-lib/src/compression/bsc.rs)
+over this library's libbsc-compatible C-ABI.
+
+Same names, argument meaning and error behaviour: empty input gives empty
+output (bsc.rs:162-164), inputs above i32::MAX are rejected (bsc.rs:165-167),
+the output buffer is n + 28 + 1024 bytes (bsc.rs:170), decompression asks
+bsc_block_info for the data size first (bsc.rs:401-419), and any negative
+libbsc code raises.
+"""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+
+from ._native import lib, u8p, i32p
+from .bwt import BscError
+
+LIBBSC_NO_ERROR = 0
+LIBBSC_BAD_PARAMETER = -1
+LIBBSC_NOT_ENOUGH_MEMORY = -2
+LIBBSC_NOT_COMPRESSIBLE = -3
+LIBBSC_NOT_SUPPORTED = -4
+LIBBSC_UNEXPECTED_EOB = -5
+LIBBSC_DATA_CORRUPT = -6
+QZBSC_GPU_ERROR = -7
+
+LIBBSC_BLOCKSORTER_BWT = 1
+LIBBSC_CODER_QLFC_STATIC = 1
+LIBBSC_CODER_QLFC_ADAPTIVE = 2
+LIBBSC_FEATURE_FASTMODE = 1
+LIBBSC_FEATURE_MULTITHREADING = 2
+LIBBSC_FEATURE_CUDA = 8
+LIBBSC_HEADER_SIZE = 28
+BSC_BLOCK_SIZE = 25 * 1024 * 1024  # bsc.rs:89
+
+_I32_MAX = 2**31 - 1
+_inited = False
+
+
+def _ensure_initialized():
+    """bsc.rs:128-141: one bsc_init, a nonzero result only warns."""
+    global _inited
+    if not _inited:
+        rc = lib().bsc_init(LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_MULTITHREADING | LIBBSC_FEATURE_CUDA)
+        if rc != LIBBSC_NO_ERROR:
+            import sys
+            print(f"Warning: BSC initialization failed with code {rc}", file=sys.stderr)
+        _inited = True
+
+
+def _arr(data) -> np.ndarray:
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data, dtype=np.uint8)
+    return np.frombuffer(bytes(data), dtype=np.uint8)
+
+
+def compress_with_params(data, lzp_hash_size: int, lzp_min_len: int, block_sorter: int, coder: int,
+                         features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> bytes:
+    """bsc.rs:153-191."""
+    _ensure_initialized()
+    a = _arr(data)
+    if a.size == 0:
+        return b""
+    if a.size > _I32_MAX:
+        raise BscError(LIBBSC_BAD_PARAMETER, f"BSC input too large: {a.size} bytes (max {_I32_MAX})")
+    out = np.zeros(a.size + LIBBSC_HEADER_SIZE + 1024, dtype=np.uint8)
+    r = lib().bsc_compress(a.ctypes.data_as(u8p), out.ctypes.data_as(u8p), a.size, lzp_hash_size, lzp_min_len,
+                           block_sorter, coder, features)
+    if r < 0:
+        raise BscError(r, "BSC compression")
+    return out[:r].tobytes()
+
+
+def compress(data) -> bytes:
+    """bsc.rs:146-150: static QLFC (not supported by this engine yet)."""
+    return compress_with_params(data, 0, 0, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_STATIC)
+
+
+def compress_adaptive(data) -> bytes:
+    """bsc.rs:194-196: LZP 16/128 + adaptive QLFC (LZP not supported yet)."""
+    return compress_with_params(data, 16, 128, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_ADAPTIVE)
+
+
+def compress_adaptive_no_lzp(data) -> bytes:
+    """bsc.rs:201-203: the default sequence/quality/header block coder."""
+    return compress_with_params(data, 0, 0, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_ADAPTIVE)
+
+
+def compress_adaptive_mt_no_lzp(data) -> bytes:
+    """bsc.rs:218-220,224-255 without LZP: segment coding on several threads."""
+    return compress_with_params(data, 0, 0, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_ADAPTIVE,
+                                LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_MULTITHREADING | LIBBSC_FEATURE_CUDA)
+
+
+def _compress_parallel_with(data, block_compressor) -> bytes:
+    """bsc.rs:257-301: [num_blocks u32][len u32, block]... even for one block."""
+    a = _arr(data)
+    if a.size == 0:
+        return b""
+    blocks = [a[i:i + BSC_BLOCK_SIZE] for i in range(0, a.size, BSC_BLOCK_SIZE)]
+    parts = [struct.pack("<I", len(blocks))]
+    for b in blocks:
+        c = block_compressor(b)
+        parts.append(struct.pack("<I", len(c)))
+        parts.append(c)
+    return b"".join(parts)
+
+
+def compress_parallel_adaptive_no_lzp(data) -> bytes:
+    return _compress_parallel_with(data, compress_adaptive_no_lzp)
+
+
+def decompress_with_features(data, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> bytes:
+    """bsc.rs:384-436."""
+    _ensure_initialized()
+    a = _arr(data)
+    if a.size == 0:
+        return b""
+    if a.size < LIBBSC_HEADER_SIZE:
+        raise BscError(LIBBSC_UNEXPECTED_EOB,
+                       f"BSC: compressed data too small (need at least {LIBBSC_HEADER_SIZE} bytes for header)")
+    bs = np.zeros(1, np.int32)
+    ds = np.zeros(1, np.int32)
+    rc = lib().bsc_block_info(a.ctypes.data_as(u8p), a.size, bs.ctypes.data_as(i32p), ds.ctypes.data_as(i32p),
+                              features)
+    if rc != LIBBSC_NO_ERROR:
+        raise BscError(rc, "BSC block_info")
+    if ds[0] <= 0:
+        raise BscError(LIBBSC_DATA_CORRUPT, f"BSC: invalid decompressed size: {int(ds[0])}")
+    out = np.zeros(int(ds[0]), dtype=np.uint8)
+    rc = lib().bsc_decompress(a.ctypes.data_as(u8p), a.size, out.ctypes.data_as(u8p), int(ds[0]), features)
+    if rc != LIBBSC_NO_ERROR:
+        raise BscError(rc, "BSC decompression")
+    return out.tobytes()
+
+
+def decompress(data) -> bytes:
+    return decompress_with_features(data)
+
+
+def decompress_mt(data) -> bytes:
+    return decompress_with_features(data, LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_MULTITHREADING
+                                    | LIBBSC_FEATURE_CUDA)
+
+
+def decompress_parallel(data) -> bytes:
+    """bsc.rs:324-371."""
+    a = bytes(data)
+    if not a:
+        return b""
+    if len(a) < 4:
+        raise BscError(LIBBSC_UNEXPECTED_EOB, "BSC parallel: data too small for header")
+    (nb,) = struct.unpack_from("<I", a, 0)
+    off = 4
+    blocks = []
+    for _ in range(nb):
+        if off + 4 > len(a):
+            raise BscError(LIBBSC_UNEXPECTED_EOB, "BSC parallel: truncated block length")
+        (ln,) = struct.unpack_from("<I", a, off)
+        off += 4
+        if off + ln > len(a):
+            raise BscError(LIBBSC_UNEXPECTED_EOB, "BSC parallel: truncated block data")
+        blocks.append(a[off:off + ln])
+        off += ln
+    return b"".join(decompress(b) for b in blocks)

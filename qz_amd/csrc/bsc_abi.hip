@@ -1,0 +1,331 @@
+// libbsc drop-in: bsc_init / bsc_compress / bsc_block_info / bsc_decompress
+// with the signatures, error codes and buffer contract qz-lib binds
+// (crates/qz-lib/src/compression/bsc.rs:91-125).
+//
+// Block layout (libbsc's, pinned by the libbsc archive under tests/golden):
+//   +0  i32 block size (header included)   +4  i32 n
+//   +8  i32 mode = sorter | coder << 5 | lzpMinLen << 8 | lzpHashSize << 16
+//   +12 i32 primary index                  +16 adler32(data)
+//   +20 adler32(bytes 28..end)             +24 adler32(bytes 0..23)
+//   28.. coder payload: [nb u8][nb x (in i32, out i32)][segments]
+//        (a single segment: [1][segment]), then num_indexes x i32 sampled
+//        BWT indexes and the count byte num_indexes.
+// Blocks of <= 28 bytes, or that do not shrink, are stored (mode 0).
+//
+// The forward and inverse BWT always run on the GPU (there is no CPU
+// transform in this library); the QLFC coder runs on the host.
+#include "bsc_block.h"
+#include "engine.h"
+#include "pool.h"
+#include "qlfc_coder.h"
+#include "../../include/qzbsc.h"
+
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace qz {
+
+uint32_t adler32(const uint8_t* p, size_t n)
+{
+    uint32_t a = 1, b = 0;
+    while (n > 0) {
+        size_t k = n < 5552 ? n : 5552;
+        n -= k;
+        while (k >= 8) {
+            a += p[0]; b += a; a += p[1]; b += a; a += p[2]; b += a; a += p[3]; b += a;
+            a += p[4]; b += a; a += p[5]; b += a; a += p[6]; b += a; a += p[7]; b += a;
+            p += 8;
+            k -= 8;
+        }
+        while (k--) { a += *p++; b += a; }
+        a %= 65521u;
+        b %= 65521u;
+    }
+    return a | (b << 16);
+}
+
+static inline void put32(uint8_t* p, uint32_t v) { memcpy(p, &v, 4); }
+static inline uint32_t get32(const uint8_t* p)
+{
+    uint32_t v;
+    memcpy(&v, p, 4);
+    return v;
+}
+
+int bwt_sample_rate(int n)
+{
+    int m = n / 8;
+    m |= m >> 1; m |= m >> 2; m |= m >> 4; m |= m >> 8; m |= m >> 16;
+    m >>= 1;
+    return m + 1;
+}
+
+int store_block(const uint8_t* in, uint8_t* out, int n)
+{
+    const uint32_t a = adler32(in, (size_t)n);
+    memmove(out + kHeader, in, (size_t)n);
+    put32(out + 0, (uint32_t)(n + kHeader));
+    put32(out + 4, (uint32_t)n);
+    put32(out + 8, 0);
+    put32(out + 12, 0);
+    put32(out + 16, a);
+    put32(out + 20, a);
+    put32(out + 24, adler32(out, 24));
+    return n + kHeader;
+}
+
+// Coder payload for one block from its front-end runs. cap: bytes available.
+int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parallel)
+{
+    const int nb = qlfc::num_segments(n);
+    int start[qlfc::kMaxSegments], size[qlfc::kMaxSegments];
+    if (nb == 1) {
+        start[0] = 0;
+        size[0] = n;
+    } else {
+        qlfc::split_segments(bwt, n, nb, start, size);
+    }
+    std::vector<std::vector<uint8_t>> segout(nb);
+    std::vector<int> res(nb, -1);
+    auto work = [&](int s) {
+        const int sn = size[s];
+        std::vector<uint8_t> sym(sn), rank(sn);
+        std::vector<uint32_t> len(sn);
+        qlfc::SegRuns R;
+        R.count = qlfc::front_end(bwt + start[s], sn, sym.data(), rank.data(), len.data(), R.table, &R.nsym);
+        R.sym = sym.data();
+        R.rank = rank.data();
+        R.len = len.data();
+        segout[s].resize((size_t)sn + 64);
+        res[s] = qlfc::encode_segment(R, sn, segout[s].data(), sn);
+    };
+    if (parallel && nb > 1) {
+        std::vector<std::thread> th;
+        for (int s = 0; s < nb; ++s) th.emplace_back(work, s);
+        for (auto& t : th) t.join();
+    } else {
+        for (int s = 0; s < nb; ++s) work(s);
+    }
+    if (nb == 1) {
+        if (res[0] < 0 || 1 + res[0] > cap) return QZ_NOT_COMPRESSIBLE;
+        out[0] = 1;
+        memcpy(out + 1, segout[0].data(), (size_t)res[0]);
+        return 1 + res[0];
+    }
+    int ptr = 1 + 8 * nb;
+    if (ptr > cap) return QZ_NOT_COMPRESSIBLE;
+    out[0] = (uint8_t)nb;
+    for (int s = 0; s < nb; ++s) {
+        const uint8_t* src = segout[s].data();
+        int r = res[s];
+        if (r < 0 || r >= size[s]) {
+            r = size[s];
+            src = bwt + start[s];
+        }
+        if (ptr + r >= n || ptr + r > cap) return QZ_NOT_COMPRESSIBLE;
+        put32(out + 1 + 8 * s, (uint32_t)size[s]);
+        put32(out + 5 + 8 * s, (uint32_t)r);
+        memcpy(out + ptr, src, (size_t)r);
+        ptr += r;
+    }
+    return ptr;
+}
+
+int coder_decompress(const uint8_t* in, int in_size, uint8_t* out, int n, bool parallel)
+{
+    if (in_size < 1) return QZ_UNEXPECTED_EOB;
+    const int nb = in[0];
+    if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_DATA_CORRUPT;
+    if (nb == 1) return qlfc::decode_segment(in + 1, in_size - 1, out, n) ? QZ_DATA_CORRUPT : 0;
+    if (1 + 8 * nb > in_size) return QZ_UNEXPECTED_EOB;
+    int istart[qlfc::kMaxSegments], isz[qlfc::kMaxSegments], ostart[qlfc::kMaxSegments], osz[qlfc::kMaxSegments];
+    int64_t ip = 1 + 8 * nb, op = 0;
+    for (int s = 0; s < nb; ++s) {
+        osz[s] = (int)get32(in + 1 + 8 * s);
+        isz[s] = (int)get32(in + 5 + 8 * s);
+        if (osz[s] < 1 || isz[s] < 1) return QZ_DATA_CORRUPT;
+        istart[s] = (int)ip;
+        ostart[s] = (int)op;
+        ip += isz[s];
+        op += osz[s];
+        if (ip > in_size || op > n) return QZ_DATA_CORRUPT;
+    }
+    if (op != n) return QZ_DATA_CORRUPT;
+    std::vector<int> rc(nb, 0);
+    auto work = [&](int s) {
+        if (isz[s] == osz[s]) memcpy(out + ostart[s], in + istart[s], (size_t)isz[s]);
+        else rc[s] = qlfc::decode_segment(in + istart[s], isz[s], out + ostart[s], osz[s]);
+    };
+    if (parallel) {
+        std::vector<std::thread> th;
+        for (int s = 0; s < nb; ++s) th.emplace_back(work, s);
+        for (auto& t : th) t.join();
+    } else {
+        for (int s = 0; s < nb; ++s) work(s);
+    }
+    for (int s = 0; s < nb; ++s)
+        if (rc[s]) return QZ_DATA_CORRUPT;
+    return 0;
+}
+
+int check_params(int n, int lzpHashSize, int lzpMinLen, int blockSorter, int coder, int* mode)
+{
+    if (blockSorter != LIBBSC_BLOCKSORTER_BWT) return (blockSorter >= 3 && blockSorter <= 8) ? QZ_NOT_SUPPORTED
+                                                                                              : QZ_BAD_PARAMETER;
+    if (coder != LIBBSC_CODER_QLFC_ADAPTIVE) return (coder == 1 || coder == 3) ? QZ_NOT_SUPPORTED : QZ_BAD_PARAMETER;
+    if (lzpMinLen != 0 || lzpHashSize != 0) {
+        if (lzpMinLen < 4 || lzpMinLen > 255 || lzpHashSize < 10 || lzpHashSize > 28) return QZ_BAD_PARAMETER;
+        return QZ_NOT_SUPPORTED;
+    }
+    if (n < 0 || n > (1 << 30)) return QZ_BAD_PARAMETER;
+    *mode = blockSorter | (coder << 5);
+    return 0;
+}
+
+// Finishes a block whose BWT (in `bwt`, n bytes), primary index and sampled
+// indexes are known: codes the payload into out+28 and writes the header.
+int finish_block(const uint8_t* input, const uint8_t* bwt, int n, int mode, int primary, const int* aux,
+                 uint8_t* out, bool parallel)
+{
+    const uint32_t adler_data = adler32(input, (size_t)n);
+    int num_indexes = (n - 1) / bwt_sample_rate(n);
+    if (n < 64 * 1024) num_indexes = 0;
+    std::vector<uint8_t> payload((size_t)n + 4096);
+    const int result = coder_compress(bwt, n, payload.data(), (int)payload.size(), parallel);
+    if (result < 0 || result + 1 + 4 * num_indexes >= n) return store_block(input, out, n);
+    memcpy(out + kHeader, payload.data(), (size_t)result);
+    for (int t = 0; t < num_indexes; ++t) put32(out + kHeader + result + 4 * t, (uint32_t)(aux[t + 1] - 1));
+    out[kHeader + result + 4 * num_indexes] = (uint8_t)num_indexes;
+    const int body = result + 1 + 4 * num_indexes;
+    put32(out + 0, (uint32_t)(body + kHeader));
+    put32(out + 4, (uint32_t)n);
+    put32(out + 8, (uint32_t)mode);
+    put32(out + 12, (uint32_t)primary);
+    put32(out + 16, adler_data);
+    put32(out + 20, adler32(out + kHeader, (size_t)body));
+    put32(out + 24, adler32(out, 24));
+    return body + kHeader;
+}
+
+int parse_header(const uint8_t* h, int size, BlockHeader* bh)
+{
+    if (size < kHeader) return QZ_UNEXPECTED_EOB;
+    if (adler32(h, 24) != get32(h + 24)) return QZ_DATA_CORRUPT;
+    bh->block_size = (int)get32(h);
+    bh->data_size = (int)get32(h + 4);
+    bh->mode = (int)get32(h + 8);
+    bh->index = (int)get32(h + 12);
+    bh->adler_data = get32(h + 16);
+    bh->adler_body = get32(h + 20);
+    const int sorter = bh->mode & 0x1f, coder = (bh->mode >> 5) & 0x7;
+    const int lzpMin = (bh->mode >> 8) & 0xff, lzpHash = (bh->mode >> 16) & 0xff;
+    bool ok = bh->block_size >= kHeader && bh->data_size >= 0 && bh->index >= 0 && bh->index <= bh->data_size;
+    if (bh->mode != 0) {
+        ok = ok && sorter >= 1 && sorter <= 8 && coder >= 1 && coder <= 3 && (bh->mode >> 24) == 0;
+        ok = ok && ((lzpMin == 0 && lzpHash == 0) || (lzpMin >= 4 && lzpHash >= 10 && lzpHash <= 28));
+    } else {
+        ok = ok && bh->block_size == bh->data_size + kHeader;
+    }
+    return ok ? 0 : QZ_DATA_CORRUPT;
+}
+
+// Checks the body checksum and decodes the coder payload into `bwt`
+// (n = data size). For a stored block copies the data and returns 1.
+int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, uint8_t* bwt, bool parallel)
+{
+    if (inputSize < bh.block_size) return QZ_UNEXPECTED_EOB;
+    if (adler32(input + kHeader, (size_t)(bh.block_size - kHeader)) != bh.adler_body) return QZ_DATA_CORRUPT;
+    if (bh.mode == 0) {
+        memmove(bwt, input + kHeader, (size_t)bh.data_size);
+        return 1;
+    }
+    const int sorter = bh.mode & 0x1f, coder = (bh.mode >> 5) & 0x7;
+    if (sorter != LIBBSC_BLOCKSORTER_BWT || coder != LIBBSC_CODER_QLFC_ADAPTIVE || (bh.mode >> 8) != 0)
+        return QZ_NOT_SUPPORTED;
+    if (bh.data_size < 1 || bh.index < 1) return QZ_DATA_CORRUPT;
+    const int num_indexes = input[bh.block_size - 1];
+    const int payload = bh.block_size - kHeader - 1 - 4 * num_indexes;
+    if (payload < 1) return QZ_DATA_CORRUPT;
+    return coder_decompress(input + kHeader, payload, bwt, bh.data_size, parallel);
+}
+
+}  // namespace qz
+
+using namespace qz;
+
+extern "C" {
+
+QZBSC_API int bsc_init(int features)
+{
+    (void)features;
+    qlfc::num_segments(1);
+    return pool_device_ok() ? LIBBSC_NO_ERROR : QZBSC_GPU_ERROR;
+}
+
+QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize, int lzpMinLen,
+                           int blockSorter, int coder, int features)
+{
+    int mode = 0;
+    QZ_TRY(check_params(n, lzpHashSize, lzpMinLen, blockSorter, coder, &mode));
+    if (!input || !output) return QZ_BAD_PARAMETER;
+    if (n <= kHeader) return store_block(input, output, n);
+    std::vector<uint8_t> bwt((size_t)n);
+    int primary = 0;
+    std::vector<int> aux(256);
+    QZ_TRY(qzbsc_bwt_encode_batch(input, bwt.data(), &n, 1, &primary, aux.data()));
+    return finish_block(input, bwt.data(), n, mode, primary, aux.data(), output,
+                        (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
+}
+
+QZBSC_API int bsc_block_info(const unsigned char* blockHeader, int headerSize, int* pBlockSize, int* pDataSize,
+                             int features)
+{
+    (void)features;
+    if (!blockHeader) return QZ_BAD_PARAMETER;
+    BlockHeader bh;
+    QZ_TRY(parse_header(blockHeader, headerSize, &bh));
+    if (pBlockSize) *pBlockSize = bh.block_size;
+    if (pDataSize) *pDataSize = bh.data_size;
+    return LIBBSC_NO_ERROR;
+}
+
+QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
+                             int features)
+{
+    if (!input || !output) return QZ_BAD_PARAMETER;
+    BlockHeader bh;
+    QZ_TRY(parse_header(input, inputSize, &bh));
+    if (inputSize < bh.block_size || outputSize < bh.data_size) return QZ_UNEXPECTED_EOB;
+    const int rc = decode_payload(input, inputSize, bh, output, (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
+    if (rc < 0) return rc;
+    if (rc == 0 && bh.data_size > 1) {
+        int n = bh.data_size;
+        QZ_TRY(qzbsc_bwt_decode_batch(output, output, &n, 1, &bh.index));
+    }
+    if (adler32(output, (size_t)bh.data_size) != bh.adler_data) return QZ_DATA_CORRUPT;
+    return LIBBSC_NO_ERROR;
+}
+
+// Test hook: size word and symbol table of one coded segment.
+QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, unsigned char* table, int* nsym)
+{
+    if (!seg || !n || !table || !nsym) return QZ_BAD_PARAMETER;
+    return qlfc::read_segment_header(seg, size, n, table, nsym);
+}
+
+// Host-only coder round trip of one BWT-domain block (no device needed).
+QZBSC_API int qzbsc_coder_compress(const unsigned char* bwt, int n, unsigned char* out, int cap, int features)
+{
+    if (!bwt || !out || n < 1) return QZ_BAD_PARAMETER;
+    return coder_compress(bwt, n, out, cap, (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
+}
+
+QZBSC_API int qzbsc_coder_decompress(const unsigned char* in, int size, unsigned char* out, int n, int features)
+{
+    if (!in || !out || n < 1) return QZ_BAD_PARAMETER;
+    return coder_decompress(in, size, out, n, (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
+}
+
+}  // extern "C"

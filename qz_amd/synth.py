@@ -34,22 +34,23 @@ def reads(g: np.ndarray, n_reads: int, read_len: int, seed: int, sub_rate: float
     """Concatenated reads from uniform positions on both strands."""
     rng = np.random.default_rng(seed)
     out = np.empty(n_reads * read_len, dtype=np.uint8)
-    offs = np.arange(read_len, dtype=np.int64)
+    win = np.lib.stride_tricks.sliding_window_view(g, read_len)
+    code_of = np.zeros(256, dtype=np.uint8)
+    code_of[_ACGT] = np.arange(4, dtype=np.uint8)
     for s in range(0, n_reads, chunk):
         k = min(chunk, n_reads - s)
         starts = rng.integers(0, g.size - read_len + 1, size=k)
-        r = g[starts[:, None] + offs[None, :]]
-        rc = rng.random(k) < 0.5
-        r[rc] = _COMP[r[rc][:, ::-1]]
+        r = win[starts]
+        rc = np.flatnonzero(rng.random(k) < 0.5)
+        r[rc] = _COMP[r[rc, ::-1]]
+        flat = r.reshape(-1)
         if sub_rate > 0:
-            m = rng.random(r.shape) < sub_rate
-            cur = r[m]
-            code = np.searchsorted(_ACGT, cur)
-            code = (code + rng.integers(1, 4, size=cur.size)) % 4
-            r[m] = _ACGT[code]
+            pos = rng.integers(0, flat.size, size=rng.binomial(flat.size, sub_rate))
+            code = (code_of[flat[pos]] + rng.integers(1, 4, size=pos.size)) % 4
+            flat[pos] = _ACGT[code]
         if n_rate > 0:
-            r[rng.random(r.shape) < n_rate] = ord("N")
-        out[s * read_len:(s + k) * read_len] = r.reshape(-1)
+            flat[rng.integers(0, flat.size, size=rng.binomial(flat.size, n_rate))] = ord("N")
+        out[s * read_len:(s + k) * read_len] = flat
     return out
 
 

@@ -1,0 +1,91 @@
+"""libbsc drop-in ABI on the GPU: round trips, block fields against the
+oracle, error behaviour on corrupt / truncated blocks (the reference's
+integration tests require an Err on both, crates/qz-lib/tests/
+integration_test.rs:470-549), determinism."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from qz_amd import bsc, synth
+from qz_amd.bwt import BscError
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs():
+    rng = np.random.default_rng(11)
+    return [
+        b"ACGTACGTACGTACGTACGTACGTACGTACGT",            # bsc.rs:446 test_roundtrip_simple
+        b"ACGTACGTNNNACGTACGTACGTACGTNNACGTACGTACGTACGTACGT"
+        b"ACGTACGTACGTACGTACGTACGTACGTACGTACGTACGTACGTACGTACGT",  # bsc.rs:454
+        b"I" * 53 + b"HHHHHHHHHHGGGGGGGGFFFFFFFFFFEEEEEEEEDDDDDDDDDCCCCCCC",  # bsc.rs:463
+        bytes(29), b"A" * 70_000, rng.integers(0, 256, 70_000).astype(np.uint8).tobytes(),
+        synth.sequence_stream(50_000, 3_000, 150, 12, 0.002, 0.0001).tobytes(),
+        synth.header_stream(30_000, 13).tobytes(),
+        synth.quality_stream(8_000, 150, 14).tobytes(),
+    ]
+
+
+def test_roundtrip_and_fields():
+    for data in _inputs():
+        print(f"n={len(data)}", flush=True)
+        c = bsc.compress_adaptive_no_lzp(data)
+        assert bsc.decompress(c) == data
+        size, n, mode, prim, a_data, a_body, a_hdr = struct.unpack_from("<iiiiIII", c, 0)
+        assert size == len(c) and n == len(data)
+        assert a_data == zlib.adler32(data) and a_body == zlib.adler32(c[28:]) and a_hdr == zlib.adler32(c[:24])
+        if mode == 0:
+            assert c[28:] == data
+            continue
+        assert mode == 0x41
+        _, p, idx = O.bsc_bwt_encode(data)
+        assert prim == p
+        nidx = c[-1]
+        want = idx.tolist() if len(data) >= 64 * 1024 else []
+        assert nidx == len(want)
+        got = list(struct.unpack_from(f"<{nidx}i", c, len(c) - 1 - 4 * nidx)) if nidx else []
+        assert got == want
+
+
+def test_ratio_and_mt_identity():
+    data = synth.config1_block(3)[:3_000_000].tobytes()
+    a = bsc.compress_adaptive_no_lzp(data)
+    b = bsc.compress_adaptive_mt_no_lzp(data)
+    assert a == b
+    assert len(a) < len(data) // 3
+    assert bsc.decompress_mt(a) == data
+
+
+def test_corrupt_and_truncated_blocks_fail():
+    data = synth.quality_stream(5_000, 150, 21).tobytes()
+    c = bytearray(bsc.compress_adaptive_no_lzp(data))
+    for pos in (10, 40, len(c) // 2, len(c) - 2):
+        bad = bytearray(c)
+        bad[pos] ^= 0xFF
+        with pytest.raises(BscError):
+            bsc.decompress(bytes(bad))
+    with pytest.raises(BscError) as e:
+        bsc.decompress(bytes(c[:-10]))
+    assert e.value.code == -5
+    with pytest.raises(BscError):
+        bsc.decompress(bytes(c[:20]))
+
+
+def test_parallel_framing_roundtrip():
+    data = synth.quality_stream(2_000, 150, 31).tobytes()
+    c = bsc.compress_parallel_adaptive_no_lzp(data)
+    assert struct.unpack_from("<I", c, 0)[0] == 1
+    assert bsc.decompress_parallel(c) == data
+    assert bsc.compress_parallel_adaptive_no_lzp(b"") == b"" and bsc.decompress_parallel(b"") == b""
+
+
+def test_unsupported_modes_raise():
+    with pytest.raises(BscError) as e:
+        bsc.compress(b"ACGT" * 100)
+    assert e.value.code == -4
+    with pytest.raises(BscError) as e:
+        bsc.compress_adaptive(b"ACGT" * 100)
+    assert e.value.code == -4

@@ -1,0 +1,183 @@
+"""Host-side checks of libqzbsc.so that need no GPU.
+
+* the library loads and exports every symbol include/qzbsc.h declares;
+* the segment framing (range coder, size word, table coding) against the
+  22 coded segments of the libbsc archive under tests/golden;
+* libbsc block header fields of that archive through bsc_block_info;
+* the host QLFC coder round trip on BWT-domain data;
+* the store path (blocks <= 28 bytes) and error codes of the drop-in ABI.
+"""
+import ctypes as C
+import struct
+import zlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import qz_amd
+from qz_amd._native import header_symbols, u8p, i32p
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden" / "ref_ne50k.qz"
+# (archive offset, block length) of the headers, sequences and qualities
+# streams: one libbsc block each (qz archive v2, 50,000 reads).
+BLOCKS = [(76, 203_684), (203_768, 1_199_082), (1_402_858, 1_222_070)]
+
+
+def gold_blocks():
+    d = GOLD.read_bytes()
+    return [d[o:o + n] for o, n in BLOCKS]
+
+
+def test_library_exports_header_symbols():
+    L = qz_amd.lib()
+    names = header_symbols()
+    assert len(names) >= 15
+    missing = [s for s in names if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_golden_archive_sha():
+    import hashlib
+    assert hashlib.sha256(GOLD.read_bytes()).hexdigest() == \
+        "4278c85ea958d4e6c70b777341d740f6617f8bcd53cc90c267ef0b1022fd8a86"
+
+
+def test_golden_block_headers():
+    expect = [(2_904_234, 1_215_920, 6, 11), (5_050_000, 559_887, 8, 9), (4_450_000, 3_324_900, 8, 8)]
+    L = qz_amd.lib()
+    for blk, (n, prim, nb, nidx) in zip(gold_blocks(), expect):
+        size, data, mode, index, a_data, a_body, a_hdr = struct.unpack_from("<iiiiIII", blk, 0)
+        assert size == len(blk) and data == n and mode == 0x41 and index == prim
+        assert a_body == zlib.adler32(blk[28:]) and a_hdr == zlib.adler32(blk[:24])
+        assert blk[-1] == nidx == (n - 1) // O.sample_rate(n)
+        assert blk[28] == nb
+        bs, ds = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        a = np.frombuffer(blk, np.uint8)
+        assert L.bsc_block_info(a.ctypes.data_as(u8p), a.size, bs.ctypes.data_as(i32p), ds.ctypes.data_as(i32p), 1) == 0
+        assert (int(bs[0]), int(ds[0])) == (len(blk), n)
+        bad = a.copy()
+        bad[5] ^= 1
+        assert L.bsc_block_info(bad.ctypes.data_as(u8p), bad.size, bs.ctypes.data_as(i32p),
+                                ds.ctypes.data_as(i32p), 1) == -6
+        assert L.bsc_block_info(a.ctypes.data_as(u8p), 27, bs.ctypes.data_as(i32p), ds.ctypes.data_as(i32p), 1) == -5
+
+
+SEQ_TABLES = [b"CATGN", b"CTGAN", b"AGCTN", b"CTGAN", b"CGTAN", b"ATGCN", b"AGTCN", b"GCTAN"]
+TABLE_SIZES = [[13, 12, 11, 11, 17, 21], None, [206, 199, 203, 196, 192, 193, 170, 175]]
+
+
+def test_golden_segment_framing():
+    """Every coded segment opens with its input size (as in the segment
+    table) and a symbol table: pins range coder, size word, table coding."""
+    L = qz_amd.lib()
+    for k, blk in enumerate(gold_blocks()):
+        p = blk[28:]
+        nb = p[0]
+        segs = [struct.unpack_from("<ii", p, 1 + 8 * i) for i in range(nb)]
+        pos = 1 + 8 * nb
+        assert pos + sum(o for _, o in segs) + 1 + 4 * blk[-1] == len(p)
+        assert sum(i for i, _ in segs) == struct.unpack_from("<i", blk, 4)[0]
+        for s, (isz, osz) in enumerate(segs):
+            # libbsc cuts segments at 1 + 32k (rank-change sampling)
+            seg = np.frombuffer(p[pos:pos + osz], np.uint8)
+            n = np.zeros(1, np.int32)
+            table = np.zeros(256, np.uint8)
+            nsym = np.zeros(1, np.int32)
+            rc = L.qzbsc_segment_header(seg.ctypes.data_as(u8p), seg.size, n.ctypes.data_as(i32p),
+                                        table.ctypes.data_as(u8p), nsym.ctypes.data_as(i32p))
+            assert rc == 0 and int(n[0]) == isz
+            tab = table[: int(nsym[0])].tobytes()
+            if k == 1:
+                assert tab == SEQ_TABLES[s]
+            else:
+                assert len(tab) == TABLE_SIZES[k][s]
+            assert len(set(tab)) == len(tab)
+            pos += osz
+        starts = np.cumsum([0] + [i for i, _ in segs])[1:-1]
+        assert all(x % 32 == 1 for x in starts)
+
+
+def _coder_rt(bwt, features=0):
+    L = qz_amd.lib()
+    b = np.ascontiguousarray(bwt, np.uint8)
+    out = np.zeros(b.size + 4096, np.uint8)
+    r = L.qzbsc_coder_compress(b.ctypes.data_as(u8p), b.size, out.ctypes.data_as(u8p), out.size, features)
+    if r < 0:
+        return r, None
+    back = np.zeros(b.size, np.uint8)
+    rc = L.qzbsc_coder_decompress(out.ctypes.data_as(u8p), r, back.ctypes.data_as(u8p), b.size, features)
+    assert rc == 0
+    return r, back
+
+
+def test_coder_roundtrip_streams():
+    from qz_amd import synth
+    rng = np.random.default_rng(3)
+    for data in [synth.config1_block(1)[:2_000_000], synth.header_stream(20_000, 4),
+                 synth.quality_stream(10_000, 150, 5), np.full(700_000, 65, np.uint8),
+                 rng.integers(0, 3, 100_000).astype(np.uint8)]:
+        u, p, _ = O.bwt_aux(data)
+        for feat in (0, 2):
+            r, back = _coder_rt(u, feat)
+            assert r > 0 and np.array_equal(back, u)
+
+
+def test_coder_deterministic_across_threads():
+    from qz_amd import synth
+    u, _, _ = O.bwt_aux(synth.quality_stream(20_000, 150, 9))
+    L = qz_amd.lib()
+    outs = []
+    for feat in (0, 2):
+        out = np.zeros(u.size + 4096, np.uint8)
+        r = L.qzbsc_coder_compress(u.ctypes.data_as(u8p), u.size, out.ctypes.data_as(u8p), out.size, feat)
+        outs.append(out[:r].tobytes())
+    assert outs[0] == outs[1]
+
+
+def test_coder_incompressible_is_rejected():
+    rng = np.random.default_rng(5)
+    u = rng.integers(0, 256, 600_000).astype(np.uint8)
+    r, back = _coder_rt(u)
+    assert r == -3 or (back is not None and np.array_equal(back, u))
+
+
+def test_coder_corrupt_segment_fails():
+    from qz_amd import synth
+    u, _, _ = O.bwt_aux(synth.config1_block(2)[:300_000])
+    L = qz_amd.lib()
+    out = np.zeros(u.size + 4096, np.uint8)
+    r = L.qzbsc_coder_compress(u.ctypes.data_as(u8p), u.size, out.ctypes.data_as(u8p), out.size, 0)
+    bad = out[:r].copy()
+    bad[r // 2] ^= 0x40
+    back = np.zeros(u.size, np.uint8)
+    rc = L.qzbsc_coder_decompress(bad.ctypes.data_as(u8p), r, back.ctypes.data_as(u8p), u.size, 0)
+    assert rc != 0 or not np.array_equal(back, u)
+
+
+def test_store_path_without_gpu():
+    L = qz_amd.lib()
+    for n in (1, 5, 28):
+        a = np.frombuffer(bytes(range(n)), np.uint8).copy()
+        out = np.zeros(n + 28 + 1024, np.uint8)
+        r = L.bsc_compress(a.ctypes.data_as(u8p), out.ctypes.data_as(u8p), n, 0, 0, 1, 2, 1)
+        assert r == n + 28
+        hdr = struct.unpack_from("<iiiiIII", out.tobytes(), 0)
+        assert hdr[:4] == (n + 28, n, 0, 0) and hdr[4] == hdr[5] == zlib.adler32(a.tobytes())
+        back = np.zeros(n, np.uint8)
+        assert L.bsc_decompress(out.ctypes.data_as(u8p), r, back.ctypes.data_as(u8p), n, 1) == 0
+        assert back.tobytes() == a.tobytes()
+
+
+def test_bad_parameters():
+    L = qz_amd.lib()
+    a = np.zeros(100, np.uint8)
+    out = np.zeros(2000, np.uint8)
+    ap, op = a.ctypes.data_as(u8p), out.ctypes.data_as(u8p)
+    assert L.bsc_compress(ap, op, 100, 0, 0, 2, 2, 1) == -1      # unknown sorter
+    assert L.bsc_compress(ap, op, 100, 0, 0, 1, 9, 1) == -1      # unknown coder
+    assert L.bsc_compress(ap, op, 100, 0, 0, 1, 1, 1) == -4      # static QLFC not supported yet
+    assert L.bsc_compress(ap, op, 100, 16, 128, 1, 2, 1) == -4   # LZP not supported yet
+    assert L.bsc_compress(ap, op, 100, 3, 128, 1, 2, 1) == -1    # LZP hash size out of range
+    assert L.bsc_compress(ap, op, -1, 0, 0, 1, 2, 1) == -1
