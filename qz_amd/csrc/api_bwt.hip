@@ -45,6 +45,7 @@ static int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
     hipStream_t s = sl.stream;
     QZ_TRY(sl.fwd.reserve(n, nblk));
+    sl.fwd.offs_host_ = offs;
     QZ_HIP(hipMemcpyAsync(sl.fwd.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
     QZ_HIP(hipEventRecord(sl.ev0, s));
     QZ_TRY(sl.fwd.encode_device(d_in, d_out, n, nblk, s));

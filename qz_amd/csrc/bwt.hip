@@ -7,28 +7,50 @@
 //
 // Layout in HBM (positions are u32; a batch holds < 2^31 bytes):
 //   text[N]      input bytes of all blocks, block b at [off[b], off[b+1])
-//   sa[N]        suffix array, global positions, block b's suffixes at
+//   sa[N]        suffix array, global positions; block b's suffixes occupy
 //                SA slots [off[b], off[b+1])
-//   rank[N]      for each text position, the SA slot where its current group
+//   rank[N]      per text position, the SA slot where its current group
 //                starts (Larsson-Sadakane ranks); the inverse SA at the end
-//   act[m]       SA slots of suffixes still in groups of size >= 2
-//   keys/vals    sort pairs for the active suffixes (ping-pong)
+//   key2[N]      per SA slot, the doubling key gathered for this round
+//   groups       unsorted groups (start slot, size) in eight size classes:
+//                2, 3-4, 5-8, 9-16, 17-32, 33-64, 65-4096, > 4096
 //
-// Round 0 sorts all suffixes by (block id, first K0 symbols) with the
-// symbols packed in mixed radix (sigma+1), 0 standing for "past the block
-// end", so suffixes shorter than K0 are already distinguished. Round r >= 1
-// sorts the active suffixes by (group start, rank[i+h] + 1), 0 when i+h hits
-// the block end, h = K0 * 2^(r-1). Groups that become singletons leave the
-// active list; the loop ends when it is empty.
+// Round 0 radix-sorts all suffixes by (block id, first K0 symbols), the
+// symbols packed in mixed radix (sigma+1) with 0 for "past the block end",
+// so suffixes shorter than K0 are already distinguished. Every later round
+// (h = K0 * 2^(r-1)) first gathers key2 = rank[s+h] + 1 (0 at the block
+// end) for every unsorted suffix, then sorts each group by key2:
+//   * groups of <= 64 inside one wavefront (bitonic network in registers,
+//     64 / 2^c groups per wave),
+//   * groups of 65..4096 inside one workgroup (bitonic network in LDS),
+//   * larger groups with the device radix sort on (group id, key2).
+// Sorting a group splits it into subgroups; subgroups of size >= 2 go to
+// the next round's lists. The loop ends when every list is empty.
 #include "prims.h"
 #include "engine.h"
 
-#include <vector>
 #include <algorithm>
+#include <cstdlib>
+#include <vector>
 
 namespace qz {
 
 constexpr int kThreads = 256;
+constexpr int kClasses = 8;
+constexpr int kSmallClasses = 6;  // sizes 2 .. 64
+constexpr int kMedMax = 4096;
+constexpr uint32_t kPadKey = 0xffffffffu;
+
+struct Grp {
+    uint32_t start, size;
+};
+
+struct GroupLists {
+    Grp* list[kClasses];
+    uint32_t* cnt;        // kClasses counters
+    uint32_t cap[kClasses];
+    uint32_t* overflow;   // set if a list would overflow (cannot happen by construction)
+};
 
 __device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ off, uint32_t nblk, uint32_t i)
 {
@@ -38,6 +60,62 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ off,
         if (off[mid] <= i) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+__device__ __forceinline__ int grp_class(uint32_t size)
+{
+    if (size <= 64) return 31 - __builtin_clz(size - 1);  // 2 -> 0, 3..4 -> 1, ..., 33..64 -> 5
+    return size <= (uint32_t)kMedMax ? 6 : 7;
+}
+
+// New unsorted groups are staged in LDS per workgroup and flushed with one
+// device-scope atomic per size class, so the list counters see a few
+// thousand atomics per round rather than one per wave.
+constexpr int kStage = 2048;  // a workgroup touches <= 4096 suffixes -> <= 2048 groups
+struct Stage {
+    uint32_t cnt[kClasses];
+    uint32_t base[kClasses];
+    uint32_t n;
+    uint4 e[kStage];  // (class, index in class, start, size)
+};
+
+__device__ __forceinline__ void stage_init(Stage& S)
+{
+    if (threadIdx.x < kClasses) S.cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) S.n = 0;
+}
+
+__device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t size)
+{
+    const uint32_t c = (uint32_t)grp_class(size);
+    const uint32_t i = atomicAdd(&S.cnt[c], 1u);
+    const uint32_t k = atomicAdd(&S.n, 1u);
+    if (k < kStage) S.e[k] = make_uint4(c, i, start, size);
+}
+
+// Whole workgroup; ends with the staged groups in the device lists.
+__device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl)
+{
+    __syncthreads();
+    if (threadIdx.x < kClasses && S.cnt[threadIdx.x])
+        S.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S.cnt[threadIdx.x]);
+    if (threadIdx.x == 0 && S.n > kStage) *gl.overflow = 1u;
+    __syncthreads();
+    const uint32_t n = S.n < kStage ? S.n : kStage;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const uint4 e = S.e[k];
+        const uint32_t at = S.base[e.x] + e.y;
+        if (at < gl.cap[e.x]) gl.list[e.x][at] = Grp{e.z, e.w};
+        else *gl.overflow = 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
+                                              uint32_t nblk, uint32_t s, uint32_t h)
+{
+    const uint32_t end = off[find_block(off, nblk, s) + 1];
+    const uint64_t p = (uint64_t)s + h;
+    return (p < end) ? rank[(uint32_t)p] + 1u : 0u;
 }
 
 __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restrict__ text, uint32_t n,
@@ -56,94 +134,325 @@ __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restric
 
 struct PackParams {
     uint16_t code[256];  // symbol -> 1..sigma (sigma may be 256)
-    uint64_t radix;      // sigma + 1
-    uint32_t k0;         // symbols per key
-    uint32_t cbits;      // key bits reserved for symbols; block id above
+    uint32_t radix;      // sigma + 1
+    uint32_t k0;         // symbols per 32-bit key
 };
 
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           const uint32_t* __restrict__ off, uint32_t nblk,
-                                                          PackParams pp, uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals,
-                                                          uint32_t* __restrict__ act)
+                                                          PackParams pp, uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals)
 {
     __shared__ uint16_t code[256];
     code[threadIdx.x] = pp.code[threadIdx.x];
     __syncthreads();
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const uint32_t b = find_block(off, nblk, i);
-    const uint32_t end = off[b + 1];
-    uint64_t key = 0;
+    const uint32_t end = off[find_block(off, nblk, i) + 1];
+    uint32_t key = 0;
     for (uint32_t j = 0; j < pp.k0; ++j) {
         const uint32_t p = i + j;
         const uint32_t c = (p < end) ? code[text[p]] : 0u;
         key = key * pp.radix + c;
     }
-    if (nblk > 1) key |= (uint64_t)b << pp.cbits;
     keys[i] = key;
     vals[i] = i;
-    act[i] = i;
 }
 
-// hv[j] = act[j] at group heads, 0 elsewhere (for the max-scan).
-__global__ __launch_bounds__(kThreads) void k_heads(const uint64_t* __restrict__ keys, uint32_t m,
-                                                    const uint32_t* __restrict__ act, uint32_t* __restrict__ hv)
+// hv[j] = j at group heads (key change or block start), 0 elsewhere.
+__global__ __launch_bounds__(kThreads) void k_heads0(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     const uint32_t* __restrict__ off, uint32_t nblk,
+                                                     uint32_t* __restrict__ hv)
 {
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= m) return;
-    const bool head = (j == 0) || keys[j] != keys[j - 1];
-    hv[j] = head ? act[j] : 0u;
+    if (j >= n) return;
+    const bool head = (j == 0) || keys[j] != keys[j - 1] || off[find_block(off, nblk, j)] == j;
+    hv[j] = head ? j : 0u;
 }
 
-// Writes back the sorted slice, updates ranks, flags suffixes still unsorted.
-__global__ __launch_bounds__(kThreads) void k_apply(const uint64_t* __restrict__ keys,
-                                                    const uint32_t* __restrict__ vals, uint32_t m,
-                                                    const uint32_t* __restrict__ act,
-                                                    const uint32_t* __restrict__ gstart,
-                                                    uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
-                                                    uint32_t* __restrict__ flag)
+constexpr int kFinItems = 16;  // suffixes per thread in the finishing kernels
+
+// Round-0 finish: SA, ranks, and the unsorted groups.
+__global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ vals, uint32_t n,
+                                                     const uint32_t* __restrict__ off, uint32_t nblk,
+                                                     const uint32_t* __restrict__ gstart, uint32_t* __restrict__ sa,
+                                                     uint32_t* __restrict__ rank, GroupLists gl)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= m) return;
-    const uint64_t k = keys[j];
-    const bool head = (j == 0) || k != keys[j - 1];
-    const bool next_head = (j + 1 == m) || keys[j + 1] != k;
-    const uint32_t s = vals[j];
-    sa[act[j]] = s;
-    rank[s] = gstart[j];
-    flag[j] = (head && next_head) ? 0u : 1u;
+    __shared__ Stage S;
+    stage_init(S);
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (kThreads * kFinItems);
+    for (int it = 0; it < kFinItems; ++it) {
+        const uint32_t j = base + it * kThreads + threadIdx.x;
+        if (j >= n) break;
+        const uint32_t k = keys[j];
+        const uint32_t s = vals[j];
+        const uint32_t g = gstart[j];
+        sa[j] = s;
+        rank[s] = g;
+        const bool end = (j + 1 == n) || keys[j + 1] != k || off[find_block(off, nblk, j) + 1] == j + 1;
+        if (end && j > g) stage_push(S, g, j - g + 1);
+    }
+    stage_flush(S, gl);
 }
 
-// Compacts still-active SA slots; incl = inclusive scan of flags.
-__global__ __launch_bounds__(kThreads) void k_compact(const uint32_t* __restrict__ incl,
-                                                      const uint32_t* __restrict__ act_in, uint32_t m,
-                                                      uint32_t* __restrict__ act_out)
+// ---- small groups (<= 64), one wave holds 64 / 2^(c+1) groups of class c --
+struct SmallPlan {
+    uint32_t wave_base[kSmallClasses + 1];  // prefix of waves per class
+};
+
+__device__ __forceinline__ bool small_lane(const Grp* const* cur, const uint32_t* __restrict__ cnt, const SmallPlan& plan, uint32_t gwave,
+                                           int* cls, uint32_t* start, uint32_t* size, uint32_t* e, uint32_t* segw)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= m) return;
-    const uint32_t p = incl[j];
-    const uint32_t prev = j ? incl[j - 1] : 0u;
-    if (p != prev) act_out[p - 1] = act_in[j];
+    int c = 0;
+    while (c < kSmallClasses - 1 && gwave >= plan.wave_base[c + 1]) ++c;
+    const uint32_t seg = 2u << c;  // lanes per group
+    const uint32_t lane = lane_id();
+    const uint32_t gi = (gwave - plan.wave_base[c]) * (QZ_WAVE / seg) + lane / seg;
+    *cls = c;
+    *segw = seg;
+    *e = lane & (seg - 1);
+    if (gi >= cnt[c]) return false;
+    const Grp g = cur[c][gi];
+    *start = g.start;
+    *size = g.size;
+    return *e < g.size;
 }
 
-// Keys of round h: (group start, rank[s+h] + 1 or 0 at the block end).
-__global__ __launch_bounds__(kThreads) void k_round_keys(const uint32_t* __restrict__ act, uint32_t m,
+struct CurLists {
+    const Grp* list[kClasses];
+    const uint32_t* cnt;
+};
+
+__global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPlan plan,
+                                                           const uint32_t* __restrict__ sa,
+                                                           const uint32_t* __restrict__ rank,
+                                                           const uint32_t* __restrict__ off, uint32_t nblk,
+                                                           uint32_t h, uint32_t* __restrict__ key2)
+{
+    const uint32_t gwave = blockIdx.x * (kThreads / QZ_WAVE) + threadIdx.x / QZ_WAVE;
+    if (gwave >= plan.wave_base[kSmallClasses]) return;  // gather has no barriers
+    int c;
+    uint32_t start = 0, size = 0, e, seg;
+    const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
+    if (valid) {
+        const uint32_t slot = start + e;
+        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+    }
+}
+
+constexpr int kSmallTasks = 16;  // wave tasks per wave in k_sort_small (<= 4096 suffixes per workgroup)
+
+__global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan plan, const uint32_t* __restrict__ key2,
+                                                         uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
+                                                         GroupLists gl)
+{
+    __shared__ Stage S;
+    stage_init(S);
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t wave_g = blockIdx.x * (kThreads / QZ_WAVE) + threadIdx.x / QZ_WAVE;
+    for (int t = 0; t < kSmallTasks; ++t) {
+        const uint32_t gwave = wave_g * kSmallTasks + t;
+        if (gwave >= plan.wave_base[kSmallClasses]) break;  // wave-uniform
+        int c;
+        uint32_t start = 0, size = 0, e, seg;
+        const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
+        uint32_t key = kPadKey, val = 0;
+        if (valid) {
+            key = key2[start + e];
+            val = sa[start + e];
+        }
+        // bitonic sort inside aligned segments of `seg` lanes, ascending
+        for (uint32_t k = 2; k <= seg; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t ok = __shfl_xor(key, (int)j, QZ_WAVE);
+                const uint32_t ov = __shfl_xor(val, (int)j, QZ_WAVE);
+                const bool up = (k == seg) || ((lane & k) == 0);
+                const bool lower = (lane & j) == 0;
+                const bool take = (lower == up) ? (ok < key) : (ok > key);
+                if (take) {
+                    key = ok;
+                    val = ov;
+                }
+            }
+        }
+        // subgroup heads and their positions within the segment
+        const uint32_t kprev = __shfl_up(key, 1, QZ_WAVE);
+        const uint32_t knext = __shfl_down(key, 1, QZ_WAVE);
+        const bool head = (e == 0) || kprev != key;
+        uint32_t hp = head ? e : 0u;
+        for (uint32_t d = 1; d < seg; d <<= 1) {
+            const uint32_t o = __shfl_up(hp, d, QZ_WAVE);
+            if (e >= d && o > hp) hp = o;
+        }
+        if (valid) {
+            sa[start + e] = val;
+            rank[val] = start + hp;
+            const bool end = (e + 1 == size) || knext != key;
+            if (end && e > hp) stage_push(S, start + hp, e - hp + 1);
+        }
+        (void)c;
+    }
+    stage_flush(S, gl);
+}
+
+// ---- medium groups (65 .. 4096), one workgroup each, bitonic in LDS -------
+__global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         const uint32_t* __restrict__ off, uint32_t nblk,
-                                                         uint32_t h, uint64_t* __restrict__ keys,
-                                                         uint32_t* __restrict__ vals)
+                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         uint32_t* __restrict__ key2)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= m) return;
-    const uint32_t s = sa[act[j]];
-    const uint32_t end = off[find_block(off, nblk, s) + 1];
-    const uint32_t g = rank[s];
-    const uint64_t p = (uint64_t)s + h;
-    const uint32_t k2 = (p < end) ? rank[(uint32_t)p] + 1u : 0u;
-    keys[j] = ((uint64_t)g << 32) | k2;
-    vals[j] = s;
+    if (blockIdx.x >= *cnt) return;
+    const Grp g = list[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < g.size; i += kThreads) {
+        const uint32_t slot = g.start + i;
+        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ key2, uint32_t* __restrict__ sa,
+                                                       uint32_t* __restrict__ rank, GroupLists gl)
+{
+    __shared__ uint32_t K[kMedMax];
+    __shared__ uint32_t V[kMedMax];
+    __shared__ uint32_t H[kMedMax];
+    __shared__ uint32_t wtot[kThreads / QZ_WAVE];
+    __shared__ Stage S;
+    if (blockIdx.x >= *cnt) return;  // block-uniform
+    stage_init(S);
+    const Grp g = list[blockIdx.x];
+    const uint32_t tid = threadIdx.x;
+    uint32_t P = 128;
+    while (P < g.size) P <<= 1;
+    for (uint32_t i = tid; i < P; i += kThreads) {
+        const bool v = i < g.size;
+        K[i] = v ? key2[g.start + i] : kPadKey;
+        V[i] = v ? sa[g.start + i] : 0u;
+    }
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            __syncthreads();
+            for (uint32_t i = tid; i < P; i += kThreads) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const bool up = (i & k) == 0;
+                    const uint32_t a = K[i], b = K[p];
+                    if ((a > b) == up) {
+                        K[i] = b;
+                        K[p] = a;
+                        const uint32_t t = V[i];
+                        V[i] = V[p];
+                        V[p] = t;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // head positions: per-thread chunk of E elements, then a block max-scan
+    const uint32_t E = P / kThreads > 0 ? P / kThreads : 1;
+    const uint32_t b0 = tid * E;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = b0 + q;
+        if (i < P) {
+            const bool head = (i == 0) || K[i] != K[i - 1];
+            if (head) run = i;
+            H[i] = run;
+        }
+    }
+    // exclusive max across threads
+    uint32_t incl = run;
+    const uint32_t lane = lane_id(), wave = tid / QZ_WAVE;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
+        if (lane >= (uint32_t)d && o > incl) incl = o;
+    }
+    if (lane == QZ_WAVE - 1) wtot[wave] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre = wtot[w] > pre ? wtot[w] : pre;
+    uint32_t excl = __shfl_up(incl, 1, QZ_WAVE);
+    if (lane == 0) excl = 0;
+    if (excl < pre) excl = pre;
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = b0 + q;
+        if (i < P && H[i] < excl) H[i] = excl;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < g.size; i += kThreads) {
+        const uint32_t hp = H[i];
+        sa[g.start + i] = V[i];
+        rank[V[i]] = g.start + hp;
+        const bool end = (i + 1 == g.size) || K[i + 1] != K[i];
+        if (end && i > hp) stage_push(S, g.start + hp, i - hp + 1);
+    }
+    stage_flush(S, gl);
+}
+
+// ---- large groups (> 4096): block-segmented radix sort of key2, one -----
+// segment per group (tiles never span two groups), flat index j in
+// [lbase[g], lbase[g+1]) standing for slot list[g].start + j - lbase[g].
+__global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restrict__ tiles,
+                                                         const Grp* __restrict__ list,
+                                                         const uint32_t* __restrict__ lbase,
+                                                         const uint32_t* __restrict__ sa,
+                                                         const uint32_t* __restrict__ rank,
+                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
+    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
+        const uint32_t j = td.start + i;
+        const uint32_t s = sa[g0 + (j - b0)];
+        keys[j] = round_key(rank, off, nblk, s, h);
+        vals[j] = s;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_heads_large(const TileDesc* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ lbase,
+                                                          const uint32_t* __restrict__ keys, uint32_t* __restrict__ hv)
+{
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t b0 = lbase[td.block];
+    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
+        const uint32_t j = td.start + i;
+        const bool head = (j == b0) || keys[j] != keys[j - 1];
+        hv[j] = head ? j : 0u;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __restrict__ tiles,
+                                                          const Grp* __restrict__ list,
+                                                          const uint32_t* __restrict__ lbase,
+                                                          const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals,
+                                                          const uint32_t* __restrict__ gstart, uint32_t* __restrict__ sa,
+                                                          uint32_t* __restrict__ rank, GroupLists gl)
+{
+    __shared__ Stage S;
+    stage_init(S);
+    __syncthreads();
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block], b1 = lbase[td.block + 1];
+    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
+        const uint32_t j = td.start + i;
+        const uint32_t k = keys[j];
+        const uint32_t slot = g0 + (j - b0);
+        const uint32_t s = vals[j];
+        const uint32_t hs = g0 + (gstart[j] - b0);  // slot of the subgroup head
+        sa[slot] = s;
+        rank[s] = hs;
+        const bool end = (j + 1 == b1) || keys[j + 1] != k;
+        if (end && slot > hs) stage_push(S, hs, slot - hs + 1);
+    }
+    stage_flush(S, gl);
 }
 
 // libsais-convention BWT of every block: U[0] = T[n-1]; U[j] = T[SA[j-1]-1]
@@ -188,20 +497,25 @@ static int bitlen(uint64_t v)
     return b;
 }
 
-static PackParams make_pack(const uint32_t* hist, uint32_t nblk)
+static PackParams make_pack(const uint32_t* hist)
 {
     PackParams pp;
     int sigma = 0;
     for (int c = 0; c < 256; ++c) pp.code[c] = hist[c] ? (uint16_t)(++sigma) : 0;
-    pp.radix = (uint64_t)sigma + 1;
-    const int bb = nblk > 1 ? bitlen(nblk - 1) : 0;
-    pp.cbits = 64 - bb;
-    // largest k with radix^k <= 2^cbits
-    unsigned __int128 lim = (unsigned __int128)1 << pp.cbits, v = 1;
+    pp.radix = (uint32_t)sigma + 1;
+    // largest k with radix^k <= 2^32
+    uint64_t v = 1;
     uint32_t k = 0;
-    while (k < 64 && v * pp.radix <= lim) { v *= pp.radix; ++k; }
+    while (k < 32 && v * pp.radix <= (1ull << 32)) { v *= pp.radix; ++k; }
     pp.k0 = k ? k : 1;
     return pp;
+}
+
+static uint32_t class_cap(uint64_t n, int c)
+{
+    // a class-c group has more than min_size - 1 members
+    static const uint64_t min_size[kClasses] = {2, 3, 5, 9, 17, 33, 65, (uint64_t)kMedMax + 1};
+    return (uint32_t)(n / min_size[c] + 1);
 }
 
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -210,12 +524,13 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     release();
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    size_t tmpw = std::max(radix_tmp_words(c), scan_tmp_words(c)) + 1024;
+    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * kRadix;
+    size_t tmpw = std::max(std::max(radix_tmp_words(c), scan_tmp_words(c)), seg_words + scan_tmp_words(seg_words) + 64) +
+                  1024;
     QZ_HIP(hipMalloc(&text_, c + 64));
     QZ_HIP(hipMalloc(&out_, c + 64));
     QZ_HIP(hipMalloc(&sa_, c * 4));
     QZ_HIP(hipMalloc(&rank_, c * 4));
-    QZ_HIP(hipMalloc(&act_, c * 4));
     QZ_HIP(hipMalloc(&bufA_, c * 4));
     QZ_HIP(hipMalloc(&bufB_, c * 4));
     QZ_HIP(hipMalloc(&keys0_, c * 8));
@@ -226,7 +541,17 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
     QZ_HIP(hipMalloc(&hist_, 256 * 4));
-    QZ_HIP(hipHostMalloc(&pinned_, 64));
+    QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
+    for (int set = 0; set < 2; ++set) {
+        size_t words = 0;
+        for (int k = 0; k < kClasses; ++k) words += class_cap(c, k);
+        QZ_HIP(hipMalloc(&groups_[set], words * sizeof(Grp)));
+    }
+    QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
+    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;  // round 0 or large groups
+    QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
+    tiles_host_.resize(tile_cap);
+    QZ_HIP(hipHostMalloc(&pinned_, 4096));
     cap_ = c;
     cap_blk_ = cb;
     return 0;
@@ -234,94 +559,203 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 
 void BwtEngine::release()
 {
-    void* ps[] = {text_, out_, sa_, rank_, act_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, tmp_, off_, idx_, hist_};
+    void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, tmp_, off_, idx_, hist_,
+                  cnt_, groups_[0], groups_[1], lbase_, tiles_};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
     text_ = out_ = nullptr;
-    sa_ = rank_ = act_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = nullptr;
+    sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = cnt_ = lbase_ = nullptr;
     keys0_ = keys1_ = nullptr;
+    groups_[0] = groups_[1] = nullptr;
+    tiles_ = nullptr;
     idx_ = nullptr;
     pinned_ = nullptr;
     cap_ = 0;
     cap_blk_ = 0;
 }
 
+// Builds the per-class list pointers of group set `set` (device).
+static GroupLists make_lists(void* base, uint32_t* cnt, uint32_t* overflow, uint64_t cap)
+{
+    GroupLists gl;
+    Grp* p = (Grp*)base;
+    for (int k = 0; k < kClasses; ++k) {
+        gl.list[k] = p;
+        gl.cap[k] = class_cap(cap, k);
+        p += gl.cap[k];
+    }
+    gl.cnt = cnt;
+    gl.overflow = overflow;
+    return gl;
+}
+
 // Sorts the suffixes of the batch in text (device); off_ holds the offsets.
 int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s)
 {
     rounds_ = 0;
-    // alphabet
+    trace_ = getenv("QZBSC_TRACE") != nullptr;
+    if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
     QZ_HIP(hipMemsetAsync(hist_, 0, 256 * 4, s));
+    QZ_HIP(hipMemsetAsync(cnt_, 0, 2 * 32 * 4, s));
     hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
                        text, n, hist_);
     QZ_HIP(hipGetLastError());
     uint32_t hh[256];
     QZ_HIP(hipMemcpyAsync(hh, hist_, sizeof(hh), hipMemcpyDeviceToHost, s));
     QZ_HIP(hipStreamSynchronize(s));
-    const PackParams pp = make_pack(hh, nblk);
+    const PackParams pp = make_pack(hh);
     k0_ = pp.k0;
 
+    uint32_t* overflow = cnt_ + 2 * kClasses;
+    GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_),
+                         make_lists(groups_[1], cnt_ + 32, overflow, cap_)};
+
+    // ---- round 0: block-segmented radix sort on the packed prefix --------
+    uint32_t ntiles = 0;
+    const uint32_t hist_words = build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &ntiles);
+    QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+    uint32_t* k32a = (uint32_t*)keys0_;
+    uint32_t* k32b = (uint32_t*)keys1_;
     const uint32_t g = qz_div_up(n, kThreads);
-    hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, pp, keys0_, vals0_, act_);
+    hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, pp, k32a, vals0_);
     QZ_HIP(hipGetLastError());
-
-    // round-0 digit passes: symbol bits, then block id bits
     int shifts[16], np = 0;
-    unsigned __int128 maxsym = 1;
-    for (uint32_t k = 0; k < pp.k0; ++k) maxsym *= pp.radix;
-    int symbits = 0;
     {
-        unsigned __int128 v = maxsym - 1;
+        uint64_t v = 1;
+        for (uint32_t k = 0; k < pp.k0; ++k) v *= pp.radix;
+        v -= 1;
+        int symbits = 0;
         while (v) { ++symbits; v >>= 1; }
+        for (int sh = 0; sh < symbits; sh += 8) shifts[np++] = sh;
     }
-    for (int sh = 0; sh < symbits; sh += 8) shifts[np++] = sh;
-    if (nblk > 1)
-        for (int sh = (int)pp.cbits; sh < 64; sh += 8) shifts[np++] = sh;
-
-    uint32_t m = n;
-    uint32_t h = pp.k0;
-    uint32_t* act = act_;
-    uint32_t* fA = bufA_;
-    uint32_t* fB = bufB_;
-    for (int round = 0;; ++round) {
-        if (round > 0) {
-            hipLaunchKernelGGL(k_round_keys, dim3(qz_div_up(m, kThreads)), dim3(kThreads), 0, s, act, m, sa_, rank_,
-                               off_, nblk, h, keys0_, vals0_);
-            QZ_HIP(hipGetLastError());
-            np = 0;
-            const int b2 = bitlen(n), bg = bitlen(n - 1);
-            for (int sh = 0; sh < b2; sh += 8) shifts[np++] = sh;
-            for (int sh = 32; sh < 32 + bg; sh += 8) shifts[np++] = sh;
-            h = (h > 0x40000000u) ? 0x80000000u : h * 2;
-        }
-        bool alt = false;
-        QZ_TRY(radix_sort_pairs<uint64_t>(keys0_, vals0_, keys1_, vals1_, m, shifts, np, tmp_, s, &alt));
-        uint64_t* keys = alt ? keys1_ : keys0_;
+    bool alt = false;
+    QZ_TRY(seg_radix_sort<uint32_t>(k32a, vals0_, k32b, vals1_, (const TileDesc*)tiles_, ntiles, hist_words, shifts, np,
+                              tmp_, s, &alt));
+    {
+        uint32_t* keys = alt ? k32b : k32a;
         uint32_t* vals = alt ? vals1_ : vals0_;
-        const uint32_t gm = qz_div_up(m, kThreads);
-        hipLaunchKernelGGL(k_heads, dim3(gm), dim3(kThreads), 0, s, keys, m, act, fA);
+        hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, off_, nblk, bufA_);
         QZ_HIP(hipGetLastError());
-        QZ_TRY((device_scan<OpMax, true>(fA, fA, m, tmp_, s)));
-        hipLaunchKernelGGL(k_apply, dim3(gm), dim3(kThreads), 0, s, keys, vals, m, act, fA, sa_, rank_, fB);
+        QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, n, tmp_, s)));
+        hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n,
+                           off_, nblk, bufA_, sa_, rank_, gls[0]);
         QZ_HIP(hipGetLastError());
-        QZ_TRY((device_scan<OpAdd, true>(fB, fB, m, tmp_, s)));
-        hipLaunchKernelGGL(k_compact, dim3(gm), dim3(kThreads), 0, s, fB, act, m, fA);
-        QZ_HIP(hipGetLastError());
-        QZ_HIP(hipMemcpyAsync(pinned_, fB + (m - 1), 4, hipMemcpyDeviceToHost, s));
+    }
+    ++rounds_;
+
+    // ---- doubling rounds ---------------------------------------------------
+    uint32_t h = pp.k0;
+    int cur = 0;
+    uint32_t* hc = (uint32_t*)pinned_;
+    for (;;) {
+        QZ_HIP(hipMemcpyAsync(hc, cnt_ + 32 * cur, kClasses * 4, hipMemcpyDeviceToHost, s));
+        QZ_HIP(hipMemcpyAsync(hc + 16, overflow, 4, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipStreamSynchronize(s));
-        const uint32_t m2 = *(volatile uint32_t*)pinned_;
-        ++rounds_;
-        // rotate: fA holds the new active list
-        uint32_t* t = act;
-        act = fA;
-        fA = t;
-        m = m2;
-        if (m == 0) break;
+        uint32_t counts[kClasses];
+        uint64_t total = 0;
+        for (int k = 0; k < kClasses; ++k) {
+            counts[k] = ((volatile uint32_t*)hc)[k];
+            total += counts[k];
+        }
+        if (((volatile uint32_t*)hc)[16]) {
+            fprintf(stderr, "qzbsc: group list overflow\n");
+            return QZ_GPU_ERROR;
+        }
+        if (trace_) {
+            fprintf(stderr, "qzbsc: round %d h=%u groups", rounds_, h);
+            for (int k = 0; k < kClasses; ++k) fprintf(stderr, " %u", counts[k]);
+            fprintf(stderr, "\n");
+        }
+        if (total == 0) break;
         if (rounds_ > 40) {
             fprintf(stderr, "qzbsc: suffix sort did not converge\n");
             return QZ_GPU_ERROR;
         }
+        const GroupLists& C = gls[cur];
+        const GroupLists& N = gls[cur ^ 1];
+        QZ_HIP(hipMemsetAsync(N.cnt, 0, kClasses * 4, s));
+        CurLists cl;
+        for (int k = 0; k < kClasses; ++k) cl.list[k] = C.list[k];
+        cl.cnt = C.cnt;
+
+        SmallPlan plan;
+        plan.wave_base[0] = 0;
+        for (int k = 0; k < kSmallClasses; ++k) {
+            const uint32_t per_wave = QZ_WAVE / (2u << k);
+            plan.wave_base[k + 1] = plan.wave_base[k] + qz_div_up(counts[k], per_wave);
+        }
+        const uint32_t small_waves = plan.wave_base[kSmallClasses];
+        const uint32_t small_blocks = qz_div_up(small_waves, kThreads / QZ_WAVE);
+        const uint32_t small_sort_blocks = qz_div_up(small_waves, (kThreads / QZ_WAVE) * kSmallTasks);
+        const uint32_t nmed = counts[6];
+        const uint32_t nlarge = counts[7];
+
+        // large groups: host-side prefix of sizes and per-group tiles
+        uint32_t M = 0, lt = 0, lhist = 0;
+        std::vector<Grp> lg;
+        std::vector<uint32_t> lb;
+        if (nlarge) {
+            lg.resize(nlarge);
+            QZ_HIP(hipMemcpyAsync(lg.data(), C.list[7], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipStreamSynchronize(s));
+            lb.resize(nlarge + 1);
+            for (uint32_t i = 0; i < nlarge; ++i) {
+                lb[i] = M;
+                M += lg[i].size;
+            }
+            lb[nlarge] = M;
+            lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt);
+            QZ_HIP(hipMemcpyAsync(lbase_, lb.data(), (nlarge + 1) * 4, hipMemcpyHostToDevice, s));
+            QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+        }
+        const TileDesc* ltiles = (const TileDesc*)tiles_;
+        uint32_t* lk0 = (uint32_t*)keys0_;
+        uint32_t* lk1 = (uint32_t*)keys1_;
+
+        // phase A: gather every key of the round before any rank changes
+        if (small_waves) {
+            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), 0, s, cl, plan, sa_, rank_, off_,
+                               nblk, h, bufB_);
+            QZ_HIP(hipGetLastError());
+        }
+        if (nmed) {
+            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, C.list[6], C.cnt + 6, sa_, rank_, off_,
+                               nblk, h, bufB_);
+            QZ_HIP(hipGetLastError());
+        }
+        if (M) {
+            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[7], lbase_, sa_, rank_,
+                               off_, nblk, h, lk0, vals0_);
+            QZ_HIP(hipGetLastError());
+        }
+        // phase B: sort groups, update SA / ranks, emit subgroups
+        if (small_waves) {
+            hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), 0, s, cl, plan, bufB_, sa_, rank_, N);
+            QZ_HIP(hipGetLastError());
+        }
+        if (nmed) {
+            hipLaunchKernelGGL(k_sort_med, dim3(nmed), dim3(kThreads), 0, s, C.list[6], C.cnt + 6, bufB_, sa_, rank_,
+                               N);
+            QZ_HIP(hipGetLastError());
+        }
+        if (M) {
+            np = 0;
+            const int b2 = bitlen(n);
+            for (int sh = 0; sh < b2; sh += 8) shifts[np++] = sh;
+            QZ_TRY(seg_radix_sort<uint32_t>(lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, shifts, np, tmp_, s, &alt));
+            uint32_t* keys = alt ? lk1 : lk0;
+            uint32_t* vals = alt ? vals1_ : vals0_;
+            hipLaunchKernelGGL(k_heads_large, dim3(lt), dim3(kThreads), 0, s, ltiles, lbase_, keys, bufA_);
+            QZ_HIP(hipGetLastError());
+            QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, M, tmp_, s)));
+            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[7], lbase_, keys, vals,
+                               bufA_, sa_, rank_, N);
+            QZ_HIP(hipGetLastError());
+        }
+        ++rounds_;
+        h = (h > 0x40000000u) ? 0x80000000u : h * 2;
+        cur ^= 1;
     }
     return 0;
 }

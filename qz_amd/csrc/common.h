@@ -33,6 +33,18 @@
         if (_rc != 0) return _rc; \
     } while (0)
 
+namespace qz {
+// A radix/ranking tile that never spans two blocks of a batch.
+struct TileDesc {
+    uint32_t start;   // first element (global)
+    uint32_t len;     // elements in the tile
+    uint32_t block;
+    uint32_t hbase;   // base of this block's [digit][tile] histogram region
+    uint32_t ntiles;  // tiles of this block
+    uint32_t tlocal;  // tile index within the block
+};
+}  // namespace qz
+
 static inline uint32_t qz_div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }

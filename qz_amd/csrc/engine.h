@@ -22,13 +22,13 @@ public:
     uint32_t* sa_ = nullptr;
     uint32_t* rank_ = nullptr;
     int rounds_ = 0;
+    bool trace_ = false;
     uint32_t k0_ = 0;
     uint64_t cap_ = 0;
     uint32_t cap_blk_ = 0;
 
 private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
-    uint32_t* act_ = nullptr;
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
     uint64_t* keys0_ = nullptr;
@@ -37,6 +37,14 @@ private:
     uint32_t* vals1_ = nullptr;
     uint32_t* tmp_ = nullptr;
     uint32_t* hist_ = nullptr;
+    uint32_t* cnt_ = nullptr;      // group-list counters (two sets) + overflow flag
+    void* groups_[2] = {nullptr, nullptr};
+    uint32_t* lbase_ = nullptr;    // prefix of large-group sizes
+    void* tiles_ = nullptr;        // block-aligned radix tiles (device)
+    std::vector<TileDesc> tiles_host_;
+  public:
+    std::vector<uint32_t> offs_host_;  // block offsets of the current batch (host copy)
+  private:
     void* pinned_ = nullptr;
 };
 

@@ -23,23 +23,15 @@
 #include "prims.h"
 #include "engine.h"
 
-#include <vector>
 #include <algorithm>
+#include <cstdlib>
+#include <vector>
 
 namespace qz {
 
 constexpr int kIThreads = 256;
 constexpr uint64_t kSplitFlag = 1ull << 40;
 constexpr uint32_t kNil = 0xffffffffu;
-
-struct TileDesc {
-    uint32_t start;     // first BWT index (global)
-    uint32_t len;       // symbols in the tile (<= kSortTile)
-    uint32_t block;
-    uint32_t hbase;     // base of this block's [symbol][tile] histogram region
-    uint32_t ntiles;    // tiles of this block
-    uint32_t tlocal;    // tile index within the block
-};
 
 __global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict__ U,
                                                         const TileDesc* __restrict__ tiles,
@@ -134,30 +126,55 @@ __device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ 
     return lo;
 }
 
-// Phase 2: walk from each splitter to the next one.
+// Phase 2: walk from each splitter to the next one. Segment lengths are
+// geometric, so lanes pull splitters from a per-workgroup pool in LDS as
+// soon as their current segment ends instead of idling until the longest
+// lane of the wave is done.
+constexpr uint32_t kPool = 1024;  // splitters per workgroup
+
 __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
                                                          uint32_t nsplit, uint32_t split,
                                                          uint32_t* __restrict__ nxt, uint32_t* __restrict__ len)
 {
-    const uint32_t sid = blockIdx.x * kIThreads + threadIdx.x;
-    if (sid >= nsplit) return;
-    const uint32_t b = find_block_u32(sbase, nblk, sid);
-    const uint32_t rb = off[b] + b;
-    const uint32_t lr0 = (sid - sbase[b]) * split;
-    uint64_t e = E[rb + lr0];
-    uint32_t l = 0;
-    uint32_t x;
-    for (;;) {
-        ++l;
-        x = (uint32_t)e;
-        const uint64_t en = E[rb + x];
-        if (en & kSplitFlag) break;
-        e = en;
+    __shared__ uint32_t next;
+    const uint32_t pbeg = blockIdx.x * kPool;
+    const uint32_t pend = min(pbeg + kPool, nsplit);
+    if (threadIdx.x == 0) next = pbeg;
+    __syncthreads();
+    uint32_t sid = atomicAdd(&next, 1u);
+    bool active = sid < pend;
+    uint32_t rb = 0, sb = 0, x = 0, l = 0;
+    uint64_t e = 0;
+    if (active) {
+        const uint32_t b = find_block_u32(sbase, nblk, sid);
+        rb = off[b] + b;
+        sb = sbase[b];
+        e = E[rb + (sid - sb) * split];
     }
-    len[sid] = l;
-    nxt[sid] = (x == 0) ? kNil : sbase[b] + x / split;
+    while (__ballot(active)) {
+        if (active) {
+            ++l;
+            x = (uint32_t)e;
+            const uint64_t en = E[rb + x];
+            if (en & kSplitFlag) {
+                len[sid] = l;
+                nxt[sid] = (x == 0) ? kNil : sb + x / split;
+                sid = atomicAdd(&next, 1u);
+                active = sid < pend;
+                l = 0;
+                if (active) {
+                    const uint32_t b = find_block_u32(sbase, nblk, sid);
+                    rb = off[b] + b;
+                    sb = sbase[b];
+                    e = E[rb + (sid - sb) * split];
+                }
+            } else {
+                e = en;
+            }
+        }
+    }
 }
 
 // Phase 3: pointer jumping; dist[s] ends as the row count from s to the end
@@ -190,19 +207,39 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
                                                          const uint32_t* __restrict__ dist,
                                                          uint8_t* __restrict__ T)
 {
-    const uint32_t sid = blockIdx.x * kIThreads + threadIdx.x;
-    if (sid >= nsplit) return;
-    const uint32_t b = find_block_u32(sbase, nblk, sid);
-    const uint32_t ob = off[b];
-    const uint32_t n = off[b + 1] - ob;
-    const uint32_t rb = ob + b;
-    const uint32_t lr0 = (sid - sbase[b]) * split;
-    const uint32_t l = len[sid];
-    uint32_t t = n + 1u - dist[sid];
-    uint64_t e = E[rb + lr0];
-    for (uint32_t k = 0; k < l; ++k, ++t) {
-        if (t < n) T[ob + n - 1u - t] = (uint8_t)(e >> 32);
-        e = E[rb + (uint32_t)e];
+    __shared__ uint32_t next;
+    const uint32_t pbeg = blockIdx.x * kPool;
+    const uint32_t pend = min(pbeg + kPool, nsplit);
+    if (threadIdx.x == 0) next = pbeg;
+    __syncthreads();
+    uint32_t sid = atomicAdd(&next, 1u);
+    bool active = sid < pend;
+    uint32_t rb = 0, l = 0, t = 0, last = 0;
+    uint8_t* dst = nullptr;
+    uint64_t e = 0;
+    auto setup = [&]() {
+        const uint32_t b = find_block_u32(sbase, nblk, sid);
+        const uint32_t ob = off[b], n = off[b + 1] - ob;
+        rb = ob + b;
+        l = len[sid];
+        t = n + 1u - dist[sid];
+        last = n;
+        dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
+        e = E[rb + (sid - sbase[b]) * split];
+    };
+    if (active) setup();
+    while (__ballot(active)) {
+        if (active) {
+            if (t < last) dst[-(int64_t)t] = (uint8_t)(e >> 32);
+            ++t;
+            if (--l == 0) {
+                sid = atomicAdd(&next, 1u);
+                active = sid < pend;
+                if (active) setup();
+            } else {
+                e = E[rb + (uint32_t)e];
+            }
+        }
     }
 }
 
@@ -214,7 +251,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb;
     const uint64_t tiles = c / kSortTile + cb + 1;
-    const uint64_t nsplit = rows / 16 + cb + 1;  // smallest split step supported is 16
+    const uint64_t nsplit = rows / 8 + cb + 1;  // smallest split step supported is 8
     QZ_HIP(hipMalloc(&U_, c + 64));
     QZ_HIP(hipMalloc(&T_, c + 64));
     QZ_HIP(hipMalloc(&E_, rows * 8));
@@ -257,7 +294,11 @@ void IbwtEngine::release()
 int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t nblk, const std::vector<uint32_t>& offs,
                               hipStream_t s)
 {
-    const uint32_t split = split_;
+    uint32_t split = split_;
+    if (const char* e = getenv("QZBSC_SPLIT")) {
+        const long v = strtol(e, nullptr, 10);
+        if (v >= 8 && v <= 4096) split = (uint32_t)v;
+    }
     // block-aligned tiles
     std::vector<TileDesc> td;
     td.reserve(n / kSortTile + nblk + 1);
@@ -294,7 +335,8 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, s, off_, prim_, nblk, split, E_);
     QZ_HIP(hipGetLastError());
     const uint32_t gs = qz_div_up(nsplit, kIThreads);
-    hipLaunchKernelGGL(k_ib_walk1, dim3(gs), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_);
+    const uint32_t gp = qz_div_up(nsplit, kPool);
+    hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_);
     QZ_HIP(hipGetLastError());
     QZ_HIP(hipMemcpyAsync(d0_, len_, (size_t)nsplit * 4, hipMemcpyDeviceToDevice, s));
     uint32_t *na = nxt0_, *nb_ = nxt1_, *da = d0_, *db = d1_;
@@ -307,7 +349,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         std::swap(na, nb_);
         std::swap(da, db);
     }
-    hipLaunchKernelGGL(k_ib_walk2, dim3(gs), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, len_, da,
+    hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, len_, da,
                        T);
     QZ_HIP(hipGetLastError());
     return 0;

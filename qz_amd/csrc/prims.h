@@ -239,4 +239,167 @@ static int radix_sort_pairs(K* k0, uint32_t* v0, K* k1, uint32_t* v1, uint64_t n
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Block-segmented radix sort of (u32 key, u32 value) pairs. Tiles never span
+// two blocks; histograms are laid out [block][digit][tile], so one exclusive
+// scan over them yields, for every (tile, digit), the global output slot:
+// block b's elements stay inside [off[b], off[b+1]) and no block id bits are
+// needed in the keys.
+// ---------------------------------------------------------------------------
+// Block-aligned tiles; returns the histogram words (256 per tile).
+static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc* out, uint32_t* count)
+{
+    uint32_t hbase = 0, k = 0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t o = offs[b], nb = offs[b + 1] - o;
+        const uint32_t nt = qz_div_up(nb, kSortTile);
+        for (uint32_t t = 0; t < nt; ++t) {
+            TileDesc d;
+            d.start = o + t * kSortTile;
+            d.len = (nb - t * kSortTile) < (uint32_t)kSortTile ? nb - t * kSortTile : (uint32_t)kSortTile;
+            d.block = b;
+            d.hbase = hbase;
+            d.ntiles = nt;
+            d.tlocal = t;
+            out[k++] = d;
+        }
+        hbase += nt * kRadix;
+    }
+    *count = k;
+    return hbase;
+}
+
+static inline uint32_t max_tiles(uint64_t n, uint32_t nblk) { return (uint32_t)(n / kSortTile + nblk + 1); }
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__ keys,
+                                                           const TileDesc* __restrict__ tiles, int shift,
+                                                           uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t cnt[kSortThreads / QZ_WAVE][kRadix];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) cnt[w][tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < td.len; i += kSortThreads)
+        atomicAdd(&cnt[wave][(uint32_t)(keys[td.start + i] >> shift) & (kRadix - 1)], 1u);
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) t += cnt[w][tid];
+    hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] = t;
+}
+
+// One LSD pass over a tile of <= 4096 pairs. Each wave ranks its own 1024
+// consecutive elements (16 iterations of 64) with ballot peer masks and
+// wave-private LDS counters; the tile is then permuted by digit in LDS and
+// written out so that consecutive lanes store consecutive slots of a digit
+// run. Stable: element order is wave-major, then iteration, then lane.
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin,
+                                                              K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                              const TileDesc* __restrict__ tiles, int shift,
+                                                              const uint32_t* __restrict__ offs)
+{
+    constexpr int W = kSortThreads / QZ_WAVE;
+    constexpr int IT = kSortTile / kSortThreads;  // 16
+    __shared__ K sk[kSortTile];
+    __shared__ uint32_t sv[kSortTile];
+    __shared__ uint32_t wcnt[W][kRadix];
+    __shared__ uint32_t lstart[kRadix];
+    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t wtot[W];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
+#pragma unroll
+    for (int w = 0; w < W; ++w) wcnt[w][tid] = 0;
+    gbase[tid] = offs[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal];
+    __syncthreads();
+    K kk[IT];
+    uint32_t vv[IT];
+    uint32_t rk[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t e = wave * (IT * QZ_WAVE) + it * QZ_WAVE + lane;
+        const bool valid = e < td.len;
+        const K k = valid ? kin[td.start + e] : (K)0;
+        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        uint64_t peer = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peer &= bit ? m : ~m;
+        }
+        const uint32_t below = popc_below(peer);
+        const uint32_t old = wcnt[wave][d];
+        if (valid && below == 0) wcnt[wave][d] = old + (uint32_t)__popcll(peer);
+        kk[it] = k;
+        vv[it] = valid ? vin[td.start + e] : 0u;
+        rk[it] = valid ? (old + below) : 0xffffffffu;
+    }
+    __syncthreads();
+    {
+        // digit tid: wave offsets and the tile-local start of the digit run
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = tot;
+            tot += c;
+        }
+        const uint32_t incl = wave_inclusive<OpAdd>(tot);
+        if (lane == QZ_WAVE - 1) wtot[wave] = incl;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (uint32_t w = 0; w < wave; ++w) pre += wtot[w];
+        lstart[tid] = pre + incl - tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        if (rk[it] != 0xffffffffu) {
+            const uint32_t d = (uint32_t)(kk[it] >> shift) & (kRadix - 1);
+            const uint32_t pos = lstart[d] + wcnt[wave][d] + rk[it];
+            sk[pos] = kk[it];
+            sv[pos] = vv[it];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < td.len; i += kSortThreads) {
+        const K k = sk[i];
+        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        const uint32_t o = gbase[d] + (i - lstart[d]);
+        kout[o] = k;
+        vout[o] = sv[i];
+    }
+}
+
+// tmp: hist_words + scan scratch. Result in (k0,v0) for even pass counts.
+template <typename K>
+static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
+                          uint32_t hist_words, const int* shifts, int passes, uint32_t* tmp, hipStream_t s,
+                          bool* in_alt)
+{
+    *in_alt = false;
+    uint32_t* hist = tmp;
+    uint32_t* stmp = tmp + hist_words + 32;
+    K *ka = k0, *kb = k1;
+    uint32_t *va = v0, *vb = v1;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_seg_hist<K>, dim3(ntiles), dim3(kSortThreads), 0, s, ka, tiles, shifts[p], hist);
+        QZ_HIP(hipGetLastError());
+        QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
+        hipLaunchKernelGGL(k_seg_scatter<K>, dim3(ntiles), dim3(kSortThreads), 0, s, ka, va, kb, vb, tiles,
+                           shifts[p], hist);
+        QZ_HIP(hipGetLastError());
+        K* t = ka; ka = kb; kb = t;
+        uint32_t* tv = va; va = vb; vb = tv;
+    }
+    *in_alt = (passes & 1) != 0;
+    return 0;
+}
+
 }  // namespace qz
