@@ -93,6 +93,15 @@ QZBSC_API int qzbsc_dev_bwt_encode_batch(const void* d_in, void* d_out, const in
 QZBSC_API int qzbsc_dev_bwt_decode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
                                          const int* primary, float* gpu_ms);
 
+/* Device QLFC front end of one BWT block (host buffers): the coder-segment
+ * split and, per segment, its symbol table and per-run symbol, rank and
+ * length (arrays of n entries; tables 8 x 256). Returns the run count.
+ * Counterpart: the front half of libbsc's QLFC coder (qlfc.cpp transform,
+ * coder.cpp segment split), reached through bsc.rs:172-183. */
+QZBSC_API int qzbsc_qlfc_front(const unsigned char* bwt, int n, int* nseg, int* seg_start, int* seg_size,
+                               int* seg_runs, unsigned char* tables, int* nsym, unsigned char* run_sym,
+                               unsigned char* run_rank, int* run_len);
+
 /* Host-only QLFC coder stage of bsc_compress / bsc_decompress (the
  * "[nb][segment table][segments]" payload of one block from its BWT, and
  * back). No device needed. Counterpart: libbsc's coder.cpp

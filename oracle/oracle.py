@@ -55,22 +55,10 @@ def _declare(L):
     L.orc_qlfc_transform.argtypes = [u8p, C.c_int32, u8p, u8p, i32p, u8p, i32p]
     L.orc_qlfc_inverse.restype = C.c_int32
     L.orc_qlfc_inverse.argtypes = [u8p, C.c_int32, u8p, i32p, C.c_int32, u8p, C.c_int32]
-    for name, res, args in _EXTRA:
-        if hasattr(L, name):
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-
-
-_EXTRA = [
-    ("orc_bsc_compress", C.c_int, [u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
-    ("orc_bsc_block_info", C.c_int, [u8p, C.c_int, i32p, i32p]),
-    ("orc_bsc_decompress", C.c_int, [u8p, C.c_int, u8p, C.c_int]),
-    ("orc_coder_decode_segment", C.c_int, [u8p, C.c_int, u8p, C.c_int]),
-    ("orc_coder_encode_segment", C.c_int, [u8p, C.c_int, u8p, C.c_int]),
-    ("orc_bsc_compress_many", C.c_int, [u8p, u8p, i32p, C.c_int, i32p, C.c_int]),
-    ("orc_bwt_many", C.c_int, [u8p, u8p, i32p, C.c_int, i32p, C.c_int]),
-]
+    L.orc_num_segments.restype = C.c_int32
+    L.orc_num_segments.argtypes = [C.c_int32]
+    L.orc_split_segments.restype = None
+    L.orc_split_segments.argtypes = [u8p, C.c_int32, C.c_int32, i32p, i32p]
 
 
 def _u8(a: np.ndarray):
@@ -155,3 +143,13 @@ def qlfc_inverse(table, ranks, lengths, n: int) -> np.ndarray:
     rc = lib().orc_qlfc_inverse(_u8(t), t.size, _u8(r), _i32(ln), r.size, _u8(out), n)
     assert rc == 0, rc
     return out[:n]
+
+
+def segments(bwt):
+    """libbsc coder segments of one block: list of (start, size)."""
+    a = as_u8(bwt)
+    nb = int(lib().orc_num_segments(a.size))
+    st = np.zeros(8, dtype=np.int32)
+    sz = np.zeros(8, dtype=np.int32)
+    lib().orc_split_segments(_u8(a), a.size, nb, _i32(st), _i32(sz))
+    return list(zip(st[:nb].tolist(), sz[:nb].tolist()))

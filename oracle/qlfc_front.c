@@ -77,3 +77,43 @@ int32_t orc_qlfc_inverse(const uint8_t *table, int32_t nsym, const uint8_t *run_
     }
     return pos == n ? 0 : -6;
 }
+
+/*
+ * Coder segments of one block (libbsc coder.cpp, absent from the reference
+ * tree). Count: one segment up to 512 KiB, then one per started 512 KiB, at
+ * most 8 — fitted to the three blocks of the libbsc archive under
+ * tests/golden (2,904,234 B -> 6; 4,450,000 and 5,050,000 B -> 8) and
+ * "parity unpinned" elsewhere. Boundaries: `changes` = number of
+ * i = 1, 33, 65, ... (< n) with in[i] != in[i-1]; if changes > nb, boundary
+ * m (1..nb-1) sits at the sample of the (m * (changes / nb))-th change;
+ * otherwise the block splits evenly. Pinned only by the archive's segment
+ * sizes being consistent with it (every boundary is 1 mod 32).
+ */
+int32_t orc_num_segments(int32_t n)
+{
+    int64_t nb = ((int64_t)n + 524287) >> 19;
+    return nb < 1 ? 1 : (nb > 8 ? 8 : (int32_t)nb);
+}
+
+void orc_split_segments(const uint8_t *in, int32_t n, int32_t nb, int32_t *start, int32_t *size)
+{
+    int32_t changes = 0;
+    for (int32_t i = 1; i < n; i += 32)
+        if (in[i] != in[i - 1]) ++changes;
+    if (nb > 1 && changes > nb) {
+        int32_t per = changes / nb, seen = 0, id = 0;
+        start[0] = 0;
+        for (int32_t i = 1; i < n && id < nb - 1; i += 32) {
+            if (in[i] != in[i - 1] && ++seen % per == 0) {
+                size[id] = i - start[id];
+                start[++id] = i;
+            }
+        }
+        size[nb - 1] = n - start[nb - 1];
+    } else {
+        for (int32_t p = 0; p < nb; ++p) {
+            start[p] = (n / nb) * p;
+            size[p] = (p != nb - 1) ? n / nb : n - (n / nb) * (nb - 1);
+        }
+    }
+}

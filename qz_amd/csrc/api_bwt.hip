@@ -7,6 +7,7 @@
 // crates/qz-lib/src/compression/mod.rs:402-404) each get one.
 #include "engine.h"
 #include "pool.h"
+#include "api_internal.h"
 #include "../../include/qzbsc.h"
 
 #include <algorithm>
@@ -39,7 +40,7 @@ static int check_sizes(const int* sizes, int nblocks, std::vector<uint32_t>& off
 }
 
 // Forward BWT of a batch whose text already sits in device memory.
-static int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
+int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
                             int* primary, int* aux, float* gpu_ms)
 {
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
@@ -64,7 +65,7 @@ static int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const
     return 0;
 }
 
-static int inv_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
+int inv_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
                             const int* primary, float* gpu_ms)
 {
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();

@@ -15,6 +15,7 @@
 // The forward and inverse BWT always run on the GPU (there is no CPU
 // transform in this library); the QLFC coder runs on the host.
 #include "bsc_block.h"
+#include "api_internal.h"
 #include "engine.h"
 #include "pool.h"
 #include "qlfc_coder.h"
@@ -76,7 +77,52 @@ int store_block(const uint8_t* in, uint8_t* out, int n)
     return n + kHeader;
 }
 
-// Coder payload for one block from its front-end runs. cap: bytes available.
+// Assembles the coder payload from per-segment coded bytes; a segment that
+// did not code smaller than its input is stored raw (fetched via raw_seg).
+template <class RawSeg>
+static int assemble_payload(int n, int nb, const int* size, std::vector<std::vector<uint8_t>>& segout,
+                            const std::vector<int>& res, RawSeg raw_seg, uint8_t* out, int cap)
+{
+    if (nb == 1) {
+        if (res[0] < 0 || 1 + res[0] > cap) return QZ_NOT_COMPRESSIBLE;
+        out[0] = 1;
+        memcpy(out + 1, segout[0].data(), (size_t)res[0]);
+        return 1 + res[0];
+    }
+    int ptr = 1 + 8 * nb;
+    if (ptr > cap) return QZ_NOT_COMPRESSIBLE;
+    out[0] = (uint8_t)nb;
+    for (int s = 0; s < nb; ++s) {
+        int r = res[s];
+        const bool raw = r < 0 || r >= size[s];
+        if (raw) r = size[s];
+        if (ptr + r >= n || ptr + r > cap) return QZ_NOT_COMPRESSIBLE;
+        put32(out + 1 + 8 * s, (uint32_t)size[s]);
+        put32(out + 5 + 8 * s, (uint32_t)r);
+        if (raw) {
+            if (raw_seg(s, out + ptr)) return QZ_GPU_ERROR;
+        } else {
+            memcpy(out + ptr, segout[s].data(), (size_t)r);
+        }
+        ptr += r;
+    }
+    return ptr;
+}
+
+template <class Fn>
+static void for_segments(int nb, bool parallel, Fn fn)
+{
+    if (parallel && nb > 1) {
+        std::vector<std::thread> th;
+        for (int s = 0; s < nb; ++s) th.emplace_back(fn, s);
+        for (auto& t : th) t.join();
+    } else {
+        for (int s = 0; s < nb; ++s) fn(s);
+    }
+}
+
+// Coder payload for one block from its BWT (host front end). cap: bytes
+// available.
 int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parallel)
 {
     const int nb = qlfc::num_segments(n);
@@ -101,36 +147,62 @@ int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parall
         segout[s].resize((size_t)sn + 64);
         res[s] = qlfc::encode_segment(R, sn, segout[s].data(), sn);
     };
-    if (parallel && nb > 1) {
-        std::vector<std::thread> th;
-        for (int s = 0; s < nb; ++s) th.emplace_back(work, s);
-        for (auto& t : th) t.join();
-    } else {
-        for (int s = 0; s < nb; ++s) work(s);
+    for_segments(nb, parallel, work);
+    return assemble_payload(
+        n, nb, size, segout, res,
+        [&](int sg, uint8_t* dst) {
+            memcpy(dst, bwt + start[sg], (size_t)size[sg]);
+            return 0;
+        },
+        out, cap);
+}
+
+// Coder payload from device front-end results (front.run() done on `dbwt`).
+int coder_compress_front(QlfcFront& front, const uint8_t* dbwt, uint32_t block, int n, uint8_t* out, int cap,
+                         bool parallel, hipStream_t stream)
+{
+    std::vector<int> sidx;
+    for (int i = 0; i < (int)front.segs_.size(); ++i)
+        if (front.segs_[i].block == block) sidx.push_back(i);
+    const int nb = (int)sidx.size();
+    if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_GPU_ERROR;
+    const uint32_t r0 = front.segs_[sidx[0]].r0, r1 = front.segs_[sidx[nb - 1]].r1;
+    const uint32_t R = r1 - r0;
+    std::vector<uint8_t> sym(R), rank(R), tables((size_t)nb * 256);
+    std::vector<uint32_t> len(R);
+    std::vector<uint16_t> nsym(nb);
+    QZ_HIP(hipMemcpyAsync(sym.data(), front.rsym_ + r0, R, hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipMemcpyAsync(rank.data(), front.rrank_ + r0, R, hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipMemcpyAsync(len.data(), front.rlen_ + r0, (size_t)R * 4, hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipMemcpyAsync(tables.data(), front.table_ + (size_t)sidx[0] * 256, (size_t)nb * 256,
+                          hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipMemcpyAsync(nsym.data(), front.nsym_ + sidx[0], (size_t)nb * 2, hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipStreamSynchronize(stream));
+    int size[qlfc::kMaxSegments], start[qlfc::kMaxSegments];
+    for (int k = 0; k < nb; ++k) {
+        size[k] = (int)front.segs_[sidx[k]].size;
+        start[k] = (int)front.segs_[sidx[k]].start;
     }
-    if (nb == 1) {
-        if (res[0] < 0 || 1 + res[0] > cap) return QZ_NOT_COMPRESSIBLE;
-        out[0] = 1;
-        memcpy(out + 1, segout[0].data(), (size_t)res[0]);
-        return 1 + res[0];
-    }
-    int ptr = 1 + 8 * nb;
-    if (ptr > cap) return QZ_NOT_COMPRESSIBLE;
-    out[0] = (uint8_t)nb;
-    for (int s = 0; s < nb; ++s) {
-        const uint8_t* src = segout[s].data();
-        int r = res[s];
-        if (r < 0 || r >= size[s]) {
-            r = size[s];
-            src = bwt + start[s];
-        }
-        if (ptr + r >= n || ptr + r > cap) return QZ_NOT_COMPRESSIBLE;
-        put32(out + 1 + 8 * s, (uint32_t)size[s]);
-        put32(out + 5 + 8 * s, (uint32_t)r);
-        memcpy(out + ptr, src, (size_t)r);
-        ptr += r;
-    }
-    return ptr;
+    std::vector<std::vector<uint8_t>> segout(nb);
+    std::vector<int> res(nb, -1);
+    for_segments(nb, parallel, [&](int k) {
+        const auto& sg = front.segs_[sidx[k]];
+        qlfc::SegRuns Rn;
+        Rn.sym = sym.data() + (sg.r0 - r0);
+        Rn.rank = rank.data() + (sg.r0 - r0);
+        Rn.len = len.data() + (sg.r0 - r0);
+        Rn.count = sg.r1 - sg.r0;
+        Rn.nsym = nsym[k];
+        memcpy(Rn.table, tables.data() + (size_t)k * 256, 256);
+        segout[k].resize((size_t)sg.size + 64);
+        res[k] = qlfc::encode_segment(Rn, (int)sg.size, segout[k].data(), (int)sg.size);
+    });
+    return assemble_payload(
+        n, nb, size, segout, res,
+        [&](int sg, uint8_t* dst) {
+            return hipMemcpy(dst, dbwt + start[sg], (size_t)size[sg], hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+        },
+        out, cap);
 }
 
 int coder_decompress(const uint8_t* in, int in_size, uint8_t* out, int n, bool parallel)
@@ -209,6 +281,32 @@ int finish_block(const uint8_t* input, const uint8_t* bwt, int n, int mode, int 
     return body + kHeader;
 }
 
+// Same as finish_block, with the coder payload produced by a callable.
+template <class Payload>
+static int finish_block_with(const uint8_t* input, int n, int mode, int primary, const int* aux, uint8_t* out,
+                             Payload payload_fn)
+{
+    const uint32_t adler_data = adler32(input, (size_t)n);
+    int num_indexes = (n - 1) / bwt_sample_rate(n);
+    if (n < 64 * 1024) num_indexes = 0;
+    std::vector<uint8_t> payload((size_t)n + 4096);
+    const int result = payload_fn(payload.data(), (int)payload.size());
+    if (result == QZ_GPU_ERROR) return result;
+    if (result < 0 || result + 1 + 4 * num_indexes >= n) return store_block(input, out, n);
+    memcpy(out + kHeader, payload.data(), (size_t)result);
+    for (int t = 0; t < num_indexes; ++t) put32(out + kHeader + result + 4 * t, (uint32_t)(aux[t + 1] - 1));
+    out[kHeader + result + 4 * num_indexes] = (uint8_t)num_indexes;
+    const int body = result + 1 + 4 * num_indexes;
+    put32(out + 0, (uint32_t)(body + kHeader));
+    put32(out + 4, (uint32_t)n);
+    put32(out + 8, (uint32_t)mode);
+    put32(out + 12, (uint32_t)primary);
+    put32(out + 16, adler_data);
+    put32(out + 20, adler32(out + kHeader, (size_t)body));
+    put32(out + 24, adler32(out, 24));
+    return body + kHeader;
+}
+
 int parse_header(const uint8_t* h, int size, BlockHeader* bh)
 {
     if (size < kHeader) return QZ_UNEXPECTED_EOB;
@@ -271,12 +369,20 @@ QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, in
     QZ_TRY(check_params(n, lzpHashSize, lzpMinLen, blockSorter, coder, &mode));
     if (!input || !output) return QZ_BAD_PARAMETER;
     if (n <= kHeader) return store_block(input, output, n);
-    std::vector<uint8_t> bwt((size_t)n);
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    Slot& sl = g.slot();
+    std::vector<uint32_t> offs = {0u, (uint32_t)n};
     int primary = 0;
     std::vector<int> aux(256);
-    QZ_TRY(qzbsc_bwt_encode_batch(input, bwt.data(), &n, 1, &primary, aux.data()));
-    return finish_block(input, bwt.data(), n, mode, primary, aux.data(), output,
-                        (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
+    QZ_TRY(sl.fwd.reserve((uint64_t)n, 1));
+    QZ_HIP(hipMemcpyAsync(sl.fwd.text_, input, (size_t)n, hipMemcpyHostToDevice, sl.stream));
+    QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, &primary, aux.data(), nullptr));
+    QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
+    const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
+    return finish_block_with(input, n, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
+        return coder_compress_front(sl.front, sl.fwd.out_, 0, n, p, cap, par, sl.stream);
+    });
 }
 
 QZBSC_API int bsc_block_info(const unsigned char* blockHeader, int headerSize, int* pBlockSize, int* pDataSize,
@@ -306,6 +412,43 @@ QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned
     }
     if (adler32(output, (size_t)bh.data_size) != bh.adler_data) return QZ_DATA_CORRUPT;
     return LIBBSC_NO_ERROR;
+}
+
+// Device QLFC front end of one BWT block (host buffers): segments, tables
+// and per-run symbol / rank / length. Returns the run count.
+QZBSC_API int qzbsc_qlfc_front(const unsigned char* bwt, int n, int* nseg, int* seg_start, int* seg_size,
+                               int* seg_runs, unsigned char* tables, int* nsym, unsigned char* run_sym,
+                               unsigned char* run_rank, int* run_len)
+{
+    if (!bwt || n < 1 || !nseg || !seg_start || !seg_size || !seg_runs || !tables || !nsym || !run_sym ||
+        !run_rank || !run_len)
+        return QZ_BAD_PARAMETER;
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    Slot& sl = g.slot();
+    QZ_TRY(sl.fwd.reserve((uint64_t)n, 1));
+    QZ_HIP(hipMemcpyAsync(sl.fwd.out_, bwt, (size_t)n, hipMemcpyHostToDevice, sl.stream));
+    std::vector<uint32_t> offs = {0u, (uint32_t)n};
+    QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
+    const auto& segs = sl.front.segs_;
+    const uint32_t R = sl.front.runs_;
+    *nseg = (int)segs.size();
+    std::vector<uint16_t> ns(segs.size());
+    for (size_t i = 0; i < segs.size(); ++i) {
+        seg_start[i] = (int)segs[i].start;
+        seg_size[i] = (int)segs[i].size;
+        seg_runs[i] = (int)(segs[i].r1 - segs[i].r0);
+    }
+    std::vector<uint32_t> len(R);
+    QZ_HIP(hipMemcpyAsync(tables, sl.front.table_, segs.size() * 256, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipMemcpyAsync(ns.data(), sl.front.nsym_, segs.size() * 2, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipMemcpyAsync(run_sym, sl.front.rsym_, R, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipMemcpyAsync(run_rank, sl.front.rrank_, R, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipMemcpyAsync(len.data(), sl.front.rlen_, (size_t)R * 4, hipMemcpyDeviceToHost, sl.stream));
+    QZ_HIP(hipStreamSynchronize(sl.stream));
+    for (size_t i = 0; i < segs.size(); ++i) nsym[i] = ns[i];
+    for (uint32_t r = 0; r < R; ++r) run_len[r] = (int)len[r];
+    return (int)R;
 }
 
 // Test hook: size word and symbol table of one coded segment.

@@ -4,12 +4,14 @@
 // the device without sharing workspace.
 #pragma once
 #include "engine.h"
+#include "qlfc_gpu.h"
 
 namespace qz {
 
 struct Slot {
     BwtEngine fwd;
     IbwtEngine inv;
+    QlfcFront front;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool busy = false;
