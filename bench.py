@@ -73,6 +73,140 @@ def cpu_baseline(blocks, threads: int):
     }
 
 
+def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
+    """Batched vs per-block over blocks of log-uniform size in 100 KB..25 MiB
+    from the mixed generators (one device pass per sub-batch of <= cap
+    bytes, versus one call per block)."""
+    t = time.perf_counter()
+    blocks, kinds = synth.mixed_blocks(count, seed)
+    sizes = np.array([b.size for b in blocks], dtype=np.int32)
+    stream = np.concatenate(blocks)
+    total = int(sizes.sum())
+    log(f"uneven: {count} blocks, {total} B generated in {time.perf_counter() - t:.1f} s")
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(stream).to(dev)
+    d_bwt = torch.empty_like(d_in)
+    d_back = torch.empty_like(d_in)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    torch.cuda.synchronize()
+    # per-block
+    lat = []
+    t0 = time.perf_counter()
+    for b in range(count):
+        ts = time.perf_counter()
+        o = int(offs[b])
+        sz = sizes[b:b + 1]
+        prim, _, _ = bwt.dev_encode_batch(d_in.data_ptr() + o, d_bwt.data_ptr() + o, sz)
+        bwt.dev_decode_batch(d_bwt.data_ptr() + o, d_back.data_ptr() + o, sz, prim)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    t_pb = time.perf_counter() - t0
+    ok_pb = torch.equal(d_back, d_in)
+    d_back.zero_()
+    # batched: contiguous sub-batches of <= cap bytes
+    groups, cur, acc = [], [], 0
+    for b in range(count):
+        if cur and acc + int(sizes[b]) > cap:
+            groups.append(cur)
+            cur, acc = [], 0
+        cur.append(b)
+        acc += int(sizes[b])
+    groups.append(cur)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    blat = []
+    for g in groups:
+        ts = time.perf_counter()
+        o = int(offs[g[0]])
+        sz = sizes[g[0]:g[-1] + 1]
+        prim, _, _ = bwt.dev_encode_batch(d_in.data_ptr() + o, d_bwt.data_ptr() + o, sz)
+        bwt.dev_decode_batch(d_bwt.data_ptr() + o, d_back.data_ptr() + o, sz, prim)
+        blat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    ok_b = torch.equal(d_back, d_in)
+    if not (ok_pb and ok_b):
+        print("bench.py: uneven round trip failed", file=sys.stderr)
+        sys.exit(1)
+    del d_in, d_bwt, d_back
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{count} blocks, log-uniform 100 KB..25 MiB, seeded mix of sequence/header/quality "
+                    f"generators ({total} B); forward BWT + inverse BWT",
+        "blocks": count,
+        "bytes": total,
+        "per_block": {"jobs_per_s": round(count / t_pb, 2), "mb_s": round(total / t_pb / 1e6, 2),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3)},
+        "batched": {"jobs_per_s": round(count / t_b, 2), "mb_s": round(total / t_b / 1e6, 2),
+                    "sub_batches": len(groups), "batch_ms": [round(x, 2) for x in blat]},
+        "batched_vs_per_block_jobs": round(t_pb / t_b, 3),
+    }
+
+
+def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int):
+    """Poisson arrivals of compress (forward BWT) and decompress (inverse
+    BWT) jobs, 70/30, each 1..40 blocks of 100 KB..25 MiB from the mixed
+    generators; continuous batching with HBM-footprint admission."""
+    rng = np.random.default_rng(seed)
+    pool, _ = synth.mixed_blocks(400, seed)
+    templates = []
+    for _ in range(24):
+        k = int(rng.integers(1, 41))
+        idx = rng.choice(len(pool), size=k, replace=False)
+        blks = [pool[i] for i in idx]
+        sizes = np.array([b.size for b in blks], dtype=np.int32)
+        inp = np.concatenate(blks)
+        out, prim, _ = bwt.encode_batch(blks)
+        templates.append({"inp": inp, "sizes": sizes, "bwt": np.concatenate(out),
+                          "prim": np.array(prim, dtype=np.int32), "out": np.empty_like(inp)})
+    mean_bytes = float(np.mean([t["inp"].size for t in templates]))
+    res = {"workload": f"Poisson arrivals, 70% compress (forward BWT) / 30% decompress (inverse BWT), "
+                       f"jobs of 1..40 blocks of 100 KB..25 MiB (mean {mean_bytes / 1e6:.1f} MB), "
+                       f"{jobs_per_rate} jobs per rate, {workers} workers",
+           "rates": []}
+    budget = 200 << 30
+    with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers) as srv:
+        # single-job latency, jobs alone
+        alone = []
+        for t in templates[:6]:
+            j = srv.submit_raw(0, t["inp"], t["sizes"], t["out"])
+            srv.wait(j)
+            alone.append(j.latency_ms)
+        res["single_job_ms"] = round(float(np.median(alone)), 2)
+        for rate in rates:
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=jobs_per_rate))
+            picks = rng.integers(0, len(templates), size=jobs_per_rate)
+            kinds = (rng.random(jobs_per_rate) < 0.3).astype(int)
+            sub = []
+            t0 = time.perf_counter()
+            for a, p, k in zip(arrivals, picks, kinds):
+                dt = t0 + a - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                t = templates[p]
+                if k == 0:
+                    sub.append((srv.submit_raw(0, t["inp"], t["sizes"], t["out"]), t["inp"].size))
+                else:
+                    sub.append((srv.submit_raw(1, t["bwt"], t["sizes"], t["out"], t["prim"]), t["inp"].size))
+            lat, nbat = [], []
+            for j, _ in sub:
+                srv.wait(j)
+                lat.append(j.latency_ms)
+                nbat.append(j.batch_jobs)
+            wall = time.perf_counter() - t0
+            nbytes = sum(b for _, b in sub)
+            res["rates"].append({"offered_jobs_per_s": rate, "jobs_per_s": round(len(sub) / wall, 2),
+                                 "mb_s": round(nbytes / wall / 1e6, 1),
+                                 "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                                 "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                                 "mean_jobs_per_batch": round(float(np.mean(nbat)), 2)})
+            log(f"serving rate {rate}: {res['rates'][-1]}")
+        res["peak_inflight_bytes"] = srv.peak_bytes()
+    # correctness spot check: last template round trip through the server
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +216,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-block", action="store_true", help="skip the per-block form")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-block batch section")
+    ap.add_argument("--uneven-blocks", type=int, default=512)
+    ap.add_argument("--batch-cap", type=int, default=1_000_000_000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
+    ap.add_argument("--serve-jobs", type=int, default=40)
+    ap.add_argument("--serve-workers", type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -180,6 +321,17 @@ def main():
         "bwt_ibwt_frac": round((ALG_BYTES_FWD + ALG_BYTES_INV) * total / (fwd + inv) / 1e9 / HBM_PEAK_GBS, 5),
     }
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_section(torch, bwt, synth, args.uneven_blocks, 4, args.batch_cap)
+        log(f"uneven: {uneven}")
+
+    serving = None
+    if not args.no_serve:
+        from qz_amd import serve
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers)
+
     cpu = None
     if not args.no_cpu:
         thr = max(1, min(args.cpu_threads, os.cpu_count() or 1, len(blocks)))
@@ -214,6 +366,8 @@ def main():
         "per_block": per_block,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "uneven_batch": uneven,
+        "serving": serving,
     }
     if per_block:
         out["batched_vs_per_block"] = round(value / per_block["value"], 3)

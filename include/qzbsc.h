@@ -115,6 +115,28 @@ QZBSC_API int qzbsc_coder_decompress(const unsigned char* in, int size, unsigned
  * libbsc-written blocks. */
 QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, unsigned char* table, int* nsym);
 
+/* Serving loop with continuous batching (no counterpart in the reference;
+ * it replaces qz's per-block rayon fan-out, mod.rs:402-404 and
+ * decompress_impl.rs:117-120, for a long-running service).
+ * create: HBM budget in bytes for the workspaces of batches in flight, a cap
+ * on one batch's input bytes, and the worker count (one engine each).
+ * submit: kind 0 = forward BWT (compress side: fills out, primary_out[b]
+ * and, if aux_out != NULL, 256 ints of sampled indexes per block), kind 1 =
+ * inverse BWT (decompress side: needs primary_in). Buffers are host memory
+ * and must stay valid until wait returns. Returns a job id (> 0) or < 0.
+ * wait: blocks until the job is done; returns its status and, optionally,
+ * its latency, its queueing time and the number of jobs in its batch. */
+QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_bytes, int workers);
+QZBSC_API void qzbsc_server_destroy(void* srv);
+QZBSC_API int qzbsc_server_submit(void* srv, int kind, const unsigned char* in, unsigned char* out, const int* sizes,
+                                  int nblocks, const int* primary_in, int* primary_out, int* aux_out);
+QZBSC_API int qzbsc_server_wait(void* srv, int job, double* latency_ms, double* queue_ms, int* batch_jobs);
+/* Largest footprint (bytes) the server had in flight. */
+QZBSC_API long long qzbsc_server_peak_bytes(void* srv);
+/* Device bytes one batch of `bytes` input in `blocks` blocks needs:
+ * kind 0 forward BWT, kind 1 inverse BWT. */
+QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
+
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
 /* 1 if a HIP device is usable, else 0. */

@@ -35,6 +35,13 @@ _SIGS = {
     "qzbsc_coder_compress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
     "qzbsc_coder_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
     "qzbsc_gpu_available": (C.c_int, []),
+    "qzbsc_server_create": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int]),
+    "qzbsc_server_destroy": (None, [C.c_void_p]),
+    "qzbsc_server_submit": (C.c_int, [C.c_void_p, C.c_int, u8p, u8p, i32p, C.c_int, i32p, i32p, i32p]),
+    "qzbsc_server_wait": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]),
+    "qzbsc_server_peak_bytes": (C.c_longlong, [C.c_void_p]),
+    "qzbsc_footprint": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
 }
 
 _lib = None

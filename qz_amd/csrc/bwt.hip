@@ -518,6 +518,20 @@ static uint32_t class_cap(uint64_t n, int c)
     return (uint32_t)(n / min_size[c] + 1);
 }
 
+uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
+{
+    const uint64_t c = std::max<uint64_t>(n, 1024);
+    const uint32_t cb = std::max<uint32_t>(nblk, 1);
+    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * kRadix;
+    const size_t tmpw =
+        std::max(std::max(radix_tmp_words(c), scan_tmp_words(c)), seg_words + scan_tmp_words(seg_words) + 64) + 1024;
+    uint64_t groups = 0;
+    for (int k = 0; k < kClasses; ++k) groups += class_cap(c, k);
+    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
+    return 2 * (c + 64) + 8 * 4 * c + tmpw * 4 + (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
+           2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + tile_cap * sizeof(TileDesc);
+}
+
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;
@@ -533,8 +547,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&rank_, c * 4));
     QZ_HIP(hipMalloc(&bufA_, c * 4));
     QZ_HIP(hipMalloc(&bufB_, c * 4));
-    QZ_HIP(hipMalloc(&keys0_, c * 8));
-    QZ_HIP(hipMalloc(&keys1_, c * 8));
+    QZ_HIP(hipMalloc(&keys0_, c * 4));
+    QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
     QZ_HIP(hipMalloc(&vals1_, c * 4));
     QZ_HIP(hipMalloc(&tmp_, tmpw * 4));
@@ -615,8 +629,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     uint32_t ntiles = 0;
     const uint32_t hist_words = build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &ntiles);
     QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, s));
-    uint32_t* k32a = (uint32_t*)keys0_;
-    uint32_t* k32b = (uint32_t*)keys1_;
+    uint32_t* k32a = keys0_;
+    uint32_t* k32b = keys1_;
     const uint32_t g = qz_div_up(n, kThreads);
     hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, pp, k32a, vals0_);
     QZ_HIP(hipGetLastError());
@@ -710,8 +724,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
         }
         const TileDesc* ltiles = (const TileDesc*)tiles_;
-        uint32_t* lk0 = (uint32_t*)keys0_;
-        uint32_t* lk1 = (uint32_t*)keys1_;
+        uint32_t* lk0 = keys0_;
+        uint32_t* lk1 = keys1_;
 
         // phase A: gather every key of the round before any rank changes
         if (small_waves) {

@@ -14,6 +14,8 @@ public:
     // off_[0..nblk] must already hold the batch offsets; text/out are device
     // buffers of n bytes (text_/out_ serve as staging for host callers).
     int encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint32_t nblk, hipStream_t s);
+    // Device bytes reserve(n, nblk) allocates (admission control).
+    static uint64_t footprint(uint64_t n, uint32_t nblk);
 
     uint8_t* text_ = nullptr;    // input symbols
     uint8_t* out_ = nullptr;     // BWT output
@@ -31,8 +33,8 @@ private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
-    uint64_t* keys0_ = nullptr;
-    uint64_t* keys1_ = nullptr;
+    uint32_t* keys0_ = nullptr;
+    uint32_t* keys1_ = nullptr;
     uint32_t* vals0_ = nullptr;
     uint32_t* vals1_ = nullptr;
     uint32_t* tmp_ = nullptr;
@@ -57,6 +59,7 @@ public:
     // buffers of n bytes (U_/T_ serve as staging for host callers).
     int decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t nblk, const std::vector<uint32_t>& offs,
                       hipStream_t s);
+    static uint64_t footprint(uint64_t n, uint32_t nblk);
 
     uint8_t* U_ = nullptr;
     uint8_t* T_ = nullptr;

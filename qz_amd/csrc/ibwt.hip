@@ -243,6 +243,15 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
     }
 }
 
+uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
+{
+    const uint64_t c = std::max<uint64_t>(n, 1024);
+    const uint32_t cb = std::max<uint32_t>(nblk, 1);
+    const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / 8 + cb + 1;
+    return 2 * (c + 64) + rows * 8 + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
+           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 5 * nsplit * 4 + 64;
+}
+
 int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;

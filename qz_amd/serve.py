@@ -1,0 +1,91 @@
+"""Python face of the serving loop (qzbsc_server_*): continuous batching of
+forward / inverse BWT jobs with HBM-footprint admission."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._native import lib, u8p, i32p
+from .bwt import BscError
+
+
+class Job:
+    def __init__(self, jid, kind, inp, out, sizes, prim_in, prim_out, aux):
+        self.id, self.kind = jid, kind
+        self._keep = (inp, out, sizes, prim_in)
+        self.out, self.prim_out, self.aux = out, prim_out, aux
+        self.sizes = sizes
+        self.latency_ms = self.queue_ms = None
+        self.batch_jobs = None
+
+    def blocks(self):
+        res, o = [], 0
+        for s in self.sizes.tolist():
+            res.append(self.out[o:o + s])
+            o += s
+        return res
+
+
+class Server:
+    def __init__(self, hbm_budget: int, max_batch_bytes: int = 1 << 30, workers: int = 2):
+        self._h = lib().qzbsc_server_create(hbm_budget, max_batch_bytes, workers)
+        if not self._h:
+            raise BscError(-7, "qzbsc_server_create")
+
+    def close(self):
+        if self._h:
+            lib().qzbsc_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def peak_bytes(self) -> int:
+        return int(lib().qzbsc_server_peak_bytes(self._h))
+
+    def submit_raw(self, kind: int, inp: np.ndarray, sizes: np.ndarray, out: np.ndarray, prim_in=None) -> Job:
+        """Submit with caller-owned contiguous buffers (no copies)."""
+        return self._submit_arrays(kind, inp, np.ascontiguousarray(sizes, dtype=np.int32), out, prim_in)
+
+    def _submit(self, kind, blocks, primaries=None):
+        arrs = [np.ascontiguousarray(b, dtype=np.uint8) for b in blocks]
+        sizes = np.array([a.size for a in arrs], dtype=np.int32)
+        inp = np.concatenate(arrs)
+        out = np.empty_like(inp)
+        return self._submit_arrays(kind, inp, sizes, out, primaries)
+
+    def _submit_arrays(self, kind, inp, sizes, out, primaries):
+        prim_in = np.ascontiguousarray(primaries, dtype=np.int32) if primaries is not None else None
+        prim_out = np.zeros(sizes.size, dtype=np.int32)
+        aux = np.zeros((sizes.size, 256), dtype=np.int32)
+        jid = lib().qzbsc_server_submit(self._h, kind, inp.ctypes.data_as(u8p), out.ctypes.data_as(u8p),
+                                        sizes.ctypes.data_as(i32p), sizes.size,
+                                        prim_in.ctypes.data_as(i32p) if prim_in is not None else None,
+                                        prim_out.ctypes.data_as(i32p), aux.ctypes.data_as(i32p))
+        if jid < 0:
+            raise BscError(jid, "qzbsc_server_submit")
+        return Job(jid, kind, inp, out, sizes, prim_in, prim_out, aux)
+
+    def submit_forward(self, blocks) -> Job:
+        """Compress side: forward BWT of the blocks."""
+        return self._submit(0, blocks)
+
+    def submit_inverse(self, bwts, primaries) -> Job:
+        """Decompress side: inverse BWT of the blocks."""
+        return self._submit(1, bwts, primaries)
+
+    def wait(self, job: Job) -> Job:
+        lat, q, nb = C.c_double(0), C.c_double(0), C.c_int(0)
+        rc = lib().qzbsc_server_wait(self._h, job.id, C.byref(lat), C.byref(q), C.byref(nb))
+        if rc != 0:
+            raise BscError(rc, "qzbsc_server_wait")
+        job.latency_ms, job.queue_ms, job.batch_jobs = lat.value, q.value, nb.value
+        return job
+
+
+def footprint(kind: int, nbytes: int, blocks: int) -> int:
+    return int(lib().qzbsc_footprint(kind, nbytes, blocks))

@@ -59,11 +59,26 @@ def sequence_stream(genome_len: int, n_reads: int, read_len: int, seed: int, sub
     return reads(genome(genome_len, seed), n_reads, read_len, seed + 1, sub_rate, n_rate)
 
 
+def _ascii_uint(v: np.ndarray, width: int):
+    """Decimal digits of v (< 10**width) as a (len, width) uint8 array and the
+    mask of digits to keep (no leading zeros)."""
+    d = np.empty((v.size, width), dtype=np.uint8)
+    x = v.astype(np.int64).copy()
+    for j in range(width - 1, -1, -1):
+        d[:, j] = 48 + (x % 10)
+        x //= 10
+    ndig = np.maximum(1, np.floor(np.log10(np.maximum(v, 1))).astype(np.int64) + 1)
+    keep = np.arange(width)[None, :] >= (width - ndig)[:, None]
+    return d, keep
+
+
 def header_stream(n_reads: int, seed: int, reads_per_tile: int = 4000) -> np.ndarray:
-    """varint(len) + "@INSTR:RUN:FLOWCELL:LANE:TILE:X:Y" per read."""
+    """varint(len) + "@A00123:456:HGK7WDSXY:LANE:TILE:X:Y" per read; tiles
+    increment every `reads_per_tile` reads, x and y are uniform below 30000."""
     rng = np.random.default_rng(seed)
     parts = []
-    chunk = 200_000
+    chunk = 1 << 20
+    prefix = np.frombuffer(b"@A00123:456:HGK7WDSXY:", dtype=np.uint8)
     for s in range(0, n_reads, chunk):
         k = min(chunk, n_reads - s)
         i = np.arange(s, s + k)
@@ -71,13 +86,22 @@ def header_stream(n_reads: int, seed: int, reads_per_tile: int = 4000) -> np.nda
         lane = 1 + (i // (reads_per_tile * 2000)) % 4
         x = rng.integers(0, 30000, size=k)
         y = rng.integers(0, 30000, size=k)
-        names = [f"@A00123:456:HGK7WDSXY:{l}:{t}:{a}:{b}".encode()
-                 for l, t, a, b in zip(lane.tolist(), tile.tolist(), x.tolist(), y.tolist())]
-        buf = bytearray()
-        for nm in names:
-            buf.append(len(nm))  # names are < 128 bytes: one varint byte
-            buf += nm
-        parts.append(np.frombuffer(bytes(buf), dtype=np.uint8))
+        cols, keeps = [], []
+        colon = np.full((k, 1), ord(":"), np.uint8)
+        on = np.ones((k, 1), bool)
+        cols.append(np.broadcast_to(prefix, (k, prefix.size)))
+        keeps.append(np.ones((k, prefix.size), bool))
+        for v, w in ((lane, 1), (tile, 4), (x, 5), (y, 5)):
+            d, kp = _ascii_uint(v, w)
+            cols += [d, colon]
+            keeps += [kp, on]
+        cols, keeps = cols[:-1], keeps[:-1]
+        body = np.concatenate(cols, axis=1)
+        keep = np.concatenate(keeps, axis=1)
+        length = keep.sum(axis=1).astype(np.uint8)  # < 128: one varint byte
+        rows = np.concatenate([length[:, None], body], axis=1)
+        mask = np.concatenate([on, keep], axis=1)
+        parts.append(rows[mask])
     return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint8)
 
 
@@ -85,28 +109,29 @@ _QV = np.array([2, 12, 23, 37], dtype=np.uint8) + 33
 
 
 def quality_stream(n_reads: int, read_len: int, seed: int, p_repeat: float = 0.85,
-                   chunk: int = 1 << 20) -> np.ndarray:
-    """NovaSeq-binned Phred+33 first-order Markov qualities."""
+                   chunk: int = 1 << 17) -> np.ndarray:
+    """NovaSeq-binned Phred+33 qualities, one first-order Markov chain per
+    read: with probability p_repeat the previous value repeats, otherwise a
+    fresh value is drawn from a position profile that decays from Q37
+    toward Q12 over the last 30 cycles."""
     rng = np.random.default_rng(seed)
     head = np.array([0.01, 0.04, 0.15, 0.80])
     tail = np.array([0.05, 0.45, 0.30, 0.20])
-    prof = np.empty((read_len, 4))
-    for pos in range(read_len):
-        t = max(0, pos - (read_len - 30)) / 30.0
-        prof[pos] = (1 - t) * head + t * tail
-    cdf = np.cumsum(prof, axis=1)
+    t = np.clip(np.arange(read_len) - (read_len - 30), 0, None) / 30.0
+    prof = (1 - t)[:, None] * head[None, :] + t[:, None] * tail[None, :]
+    cdf = np.round(np.cumsum(prof, axis=1)[:, :3] * 65536).astype(np.uint32)  # (L, 3), 16-bit fixed point
+    rep_t = int(round(p_repeat * 65536))
+    pos = np.arange(read_len, dtype=np.int32)
     out = np.empty(n_reads * read_len, dtype=np.uint8)
     for s in range(0, n_reads, chunk):
         k = min(chunk, n_reads - s)
-        q = np.empty((k, read_len), dtype=np.uint8)
-        cur = (rng.random(k)[:, None] > cdf[0][None, :]).sum(1)
-        q[:, 0] = cur
-        for pos in range(1, read_len):
-            fresh = (rng.random(k)[:, None] > cdf[pos][None, :]).sum(1)
-            rep = rng.random(k) < p_repeat
-            cur = np.where(rep, cur, fresh)
-            q[:, pos] = cur
-        out[s * read_len:(s + k) * read_len] = _QV[np.minimum(q, 3)].reshape(-1)
+        u = rng.integers(0, 65536, size=(k, read_len), dtype=np.uint32)
+        fresh = ((u > cdf[None, :, 0]).astype(np.uint8) + (u > cdf[None, :, 1]) + (u > cdf[None, :, 2]))
+        rep = rng.integers(0, 65536, size=(k, read_len), dtype=np.uint32) < rep_t
+        rep[:, 0] = False
+        src = np.maximum.accumulate(np.where(rep, 0, pos[None, :]), axis=1)
+        q = np.take_along_axis(fresh, src, axis=1)
+        out[s * read_len:(s + k) * read_len] = _QV[q].reshape(-1)
     return out
 
 
@@ -124,24 +149,28 @@ def cut_blocks(stream: np.ndarray, block: int = BLOCK):
     return [stream[i:i + block] for i in range(0, stream.size, block)]
 
 
-def mixed_blocks(count: int, seed: int, lo: int = 100_000, hi: int = BLOCK, total_cap: int | None = None):
+def mixed_blocks(count: int, seed: int, lo: int = 100_000, hi: int = BLOCK, total_cap: int | None = None,
+                 pool_cap: int = 128 << 20):
     """Blocks of log-uniform size in [lo, hi] from a seeded mix of the three
-    generators (sequence / header / quality)."""
+    generators (sequence / header / quality). Each kind draws its blocks at
+    seeded offsets from one generated pool of at most pool_cap bytes (blocks
+    are transformed independently, so sharing a pool costs no realism in the
+    per-block work and keeps generation to seconds)."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
     kinds = rng.integers(0, 3, size=count)
     if total_cap is not None:
         keep = np.cumsum(sizes) <= total_cap
         sizes, kinds = sizes[keep], kinds[keep]
-    need = [int(sizes[kinds == k].sum()) for k in range(3)]
+    need = [min(pool_cap, max(int(sizes[kinds == k].sum()), int(sizes[kinds == k].max(initial=0)))) for k in range(3)]
+    need = [max(n, int(sizes[kinds == k].max(initial=0))) for k, n in enumerate(need)]
     pools = [
         config2_sequences(need[0] // 150 + 2, seed + 11) if need[0] else np.zeros(0, np.uint8),
         header_stream(need[1] // 38 + 2, seed + 12) if need[1] else np.zeros(0, np.uint8),
         quality_stream(need[2] // 150 + 2, 150, seed + 13) if need[2] else np.zeros(0, np.uint8),
     ]
-    pos = [0, 0, 0]
     blocks = []
     for sz, k in zip(sizes.tolist(), kinds.tolist()):
-        blocks.append(pools[k][pos[k]:pos[k] + sz])
-        pos[k] += sz
+        o = int(rng.integers(0, pools[k].size - sz + 1))
+        blocks.append(pools[k][o:o + sz])
     return blocks, kinds
