@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--uneven-blocks", type=int, default=512)
     ap.add_argument("--batch-cap", type=int, default=1_000_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=2)
@@ -248,6 +250,17 @@ def main():
     log(f"generated {total} B in {len(blocks)} blocks ({time.perf_counter() - t:.1f} s)")
 
     dev = torch.device("cuda:0")
+    if args.calib_copy:
+        # a plain copy of known size, its source first pushed out of the
+        # caches by writing 2 GiB elsewhere
+        src = torch.ones(args.calib_copy << 20, dtype=torch.uint8, device=dev)
+        evict = torch.zeros(2 << 30, dtype=torch.uint8, device=dev)
+        evict.fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst, evict
     d_in = torch.from_numpy(stream).to(dev)
     d_bwt = torch.empty_like(d_in)
     d_back = torch.empty_like(d_in)

@@ -54,8 +54,16 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # static archive named like libbsc's (qz-lib links `libbsc` statically,
+    # crates/qz-lib/src/compression/bsc.rs:91, crates/qz-lib/build.rs:60)
+    ar = BUILD / "liblibbsc.a"
+    if ar.exists():
+        ar.unlink()
+    r = subprocess.run(["ar", "rcs", str(ar), *map(str, objs)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ar failed:\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
+        print(f"built {LIB} and {ar}", file=sys.stderr)
     return LIB
 
 

@@ -36,8 +36,12 @@
 namespace qz {
 
 constexpr int kThreads = 256;
-constexpr int kClasses = 8;
+constexpr int kClasses = 9;
 constexpr int kSmallClasses = 6;  // sizes 2 .. 64
+constexpr int kClsWave = 6;       // 65 .. 1024: one wavefront per group
+constexpr int kClsMed = 7;        // 1025 .. 4096: one workgroup per group
+constexpr int kClsLarge = 8;      // > 4096: segmented radix sort
+constexpr int kWaveMax = 1024;
 constexpr int kMedMax = 4096;
 constexpr uint32_t kPadKey = 0xffffffffu;
 
@@ -65,50 +69,63 @@ __device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ off,
 __device__ __forceinline__ int grp_class(uint32_t size)
 {
     if (size <= 64) return 31 - __builtin_clz(size - 1);  // 2 -> 0, 3..4 -> 1, ..., 33..64 -> 5
-    return size <= (uint32_t)kMedMax ? 6 : 7;
+    if (size <= (uint32_t)kWaveMax) return kClsWave;
+    return size <= (uint32_t)kMedMax ? kClsMed : kClsLarge;
 }
 
 // New unsorted groups are staged in LDS per workgroup and flushed with one
 // device-scope atomic per size class, so the list counters see a few
 // thousand atomics per round rather than one per wave.
 constexpr int kStage = 2048;  // a workgroup touches <= 4096 suffixes -> <= 2048 groups
-struct Stage {
+template <int CAP>
+struct StageT {
     uint32_t cnt[kClasses];
     uint32_t base[kClasses];
     uint32_t n;
-    uint4 e[kStage];  // (class, index in class, start, size)
+    uint4 e[CAP];  // (class, index in class, start, size)
 };
+using Stage = StageT<kStage>;
 
-__device__ __forceinline__ void stage_init(Stage& S)
+template <class S>
+__device__ __forceinline__ void stage_init(S& S_)
 {
-    if (threadIdx.x < kClasses) S.cnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) S.n = 0;
+    if (threadIdx.x < kClasses) S_.cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) S_.n = 0;
 }
 
-__device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t size)
+template <class S>
+__device__ __forceinline__ void stage_push_t(S& S_, uint32_t start, uint32_t size, uint32_t cap)
 {
     const uint32_t c = (uint32_t)grp_class(size);
-    const uint32_t i = atomicAdd(&S.cnt[c], 1u);
-    const uint32_t k = atomicAdd(&S.n, 1u);
-    if (k < kStage) S.e[k] = make_uint4(c, i, start, size);
+    const uint32_t i = atomicAdd(&S_.cnt[c], 1u);
+    const uint32_t k = atomicAdd(&S_.n, 1u);
+    if (k < cap) S_.e[k] = make_uint4(c, i, start, size);
 }
 
-// Whole workgroup; ends with the staged groups in the device lists.
-__device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl)
+template <class S>
+__device__ __forceinline__ void stage_flush_t(S& S_, const GroupLists& gl, uint32_t cap)
 {
     __syncthreads();
-    if (threadIdx.x < kClasses && S.cnt[threadIdx.x])
-        S.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S.cnt[threadIdx.x]);
-    if (threadIdx.x == 0 && S.n > kStage) *gl.overflow = 1u;
+    if (threadIdx.x < kClasses && S_.cnt[threadIdx.x])
+        S_.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S_.cnt[threadIdx.x]);
+    if (threadIdx.x == 0 && S_.n > cap) *gl.overflow = 1u;
     __syncthreads();
-    const uint32_t n = S.n < kStage ? S.n : kStage;
+    const uint32_t n = S_.n < cap ? S_.n : cap;
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        const uint4 e = S.e[k];
-        const uint32_t at = S.base[e.x] + e.y;
+        const uint4 e = S_.e[k];
+        const uint32_t at = S_.base[e.x] + e.y;
         if (at < gl.cap[e.x]) gl.list[e.x][at] = Grp{e.z, e.w};
         else *gl.overflow = 1u;
     }
 }
+
+__device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t size)
+{
+    stage_push_t(S, start, size, kStage);
+}
+
+// Whole workgroup; ends with the staged groups in the device lists.
+__device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { stage_flush_t(S, gl, kStage); }
 
 __device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
                                               uint32_t nblk, uint32_t s, uint32_t h)
@@ -298,7 +315,106 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
     stage_flush(S, gl);
 }
 
-// ---- medium groups (65 .. 4096), one workgroup each, bitonic in LDS -------
+// ---- wave groups (65 .. 1024): a 64-thread workgroup per group, bitonic in
+// LDS; barriers cost nothing with one wave and LDS stays small, so many
+// groups share a CU.
+__global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__ list,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ sa,
+                                                         const uint32_t* __restrict__ rank,
+                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         uint32_t* __restrict__ key2)
+{
+    if (blockIdx.x >= *cnt) return;
+    const Grp g = list[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < g.size; i += QZ_WAVE) {
+        const uint32_t slot = g.start + i;
+        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+    }
+}
+
+__global__ __launch_bounds__(QZ_WAVE) void k_sort_wave(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ key2, uint32_t* __restrict__ sa,
+                                                       uint32_t* __restrict__ rank, GroupLists gl)
+{
+    constexpr int kCap = kWaveMax / 2;
+    __shared__ uint32_t K[kWaveMax];
+    __shared__ uint32_t V[kWaveMax];
+    __shared__ uint32_t H[kWaveMax];
+    __shared__ StageT<kCap> S;
+    if (blockIdx.x >= *cnt) return;  // block-uniform
+    stage_init(S);
+    const Grp g = list[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    uint32_t P = 128;
+    while (P < g.size) P <<= 1;
+    for (uint32_t i = lane; i < P; i += QZ_WAVE) {
+        const bool v = i < g.size;
+        K[i] = v ? key2[g.start + i] : kPadKey;
+        V[i] = v ? sa[g.start + i] : 0u;
+    }
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            __syncthreads();
+            for (uint32_t i = lane; i < P; i += QZ_WAVE) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const bool up = (i & k) == 0;
+                    const uint32_t a = K[i], b = K[p];
+                    if ((a > b) == up) {
+                        K[i] = b;
+                        K[p] = a;
+                        const uint32_t t = V[i];
+                        V[i] = V[p];
+                        V[p] = t;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // head positions: contiguous chunk of E = P/64 per lane, then a wave max-scan
+    const uint32_t E = P / QZ_WAVE;
+    const uint32_t b0 = lane * E;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = b0 + q;
+        const bool head = (i == 0) || K[i] != K[i - 1];
+        if (head) run = i;
+        H[i] = run;
+    }
+    uint32_t incl = run;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
+        if (lane >= (uint32_t)d && o > incl) incl = o;
+    }
+    uint32_t excl = __shfl_up(incl, 1, QZ_WAVE);
+    if (lane == 0) excl = 0;
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = b0 + q;
+        if (H[i] < excl) H[i] = excl;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < g.size; i += QZ_WAVE) {
+        const uint32_t hp = H[i];
+        sa[g.start + i] = V[i];
+        rank[V[i]] = g.start + hp;
+        const bool end = (i + 1 == g.size) || K[i + 1] != K[i];
+        if (end && i > hp) stage_push_t(S, g.start + hp, i - hp + 1, kCap);
+    }
+    stage_flush_t(S, gl, kCap);
+}
+
+// ---- medium groups (1025 .. 4096), one workgroup each, bitonic in LDS -----
+// Wave w owns the contiguous quarter [w*P/4, (w+1)*P/4); stages whose stride
+// stays inside a quarter synchronise the wave only, the few cross-quarter
+// stages the workgroup.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
@@ -326,17 +442,23 @@ __global__ __launch_bounds__(kThreads) void k_sort_med(const Grp* __restrict__ l
     stage_init(S);
     const Grp g = list[blockIdx.x];
     const uint32_t tid = threadIdx.x;
-    uint32_t P = 128;
+    uint32_t P = 2048;
     while (P < g.size) P <<= 1;
     for (uint32_t i = tid; i < P; i += kThreads) {
         const bool v = i < g.size;
         K[i] = v ? key2[g.start + i] : kPadKey;
         V[i] = v ? sa[g.start + i] : 0u;
     }
+    const uint32_t quarter = P / (kThreads / QZ_WAVE);
+    const uint32_t qbase = (tid / QZ_WAVE) * quarter, qlane = tid % QZ_WAVE;
+    bool cross_prev = true;  // the loads above came from all waves
     for (uint32_t k = 2; k <= P; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            __syncthreads();
-            for (uint32_t i = tid; i < P; i += kThreads) {
+            const bool cross = j >= quarter;
+            if (cross || cross_prev) __syncthreads();
+            else wave_sync();
+            cross_prev = cross;
+            for (uint32_t i = qbase + qlane; i < qbase + quarter; i += QZ_WAVE) {
                 const uint32_t p = i ^ j;
                 if (p > i) {
                     const bool up = (i & k) == 0;
@@ -514,7 +636,8 @@ static PackParams make_pack(const uint32_t* hist)
 static uint32_t class_cap(uint64_t n, int c)
 {
     // a class-c group has more than min_size - 1 members
-    static const uint64_t min_size[kClasses] = {2, 3, 5, 9, 17, 33, 65, (uint64_t)kMedMax + 1};
+    static const uint64_t min_size[kClasses] = {2, 3, 5, 9, 17, 33, 65, (uint64_t)kWaveMax + 1,
+                                                (uint64_t)kMedMax + 1};
     return (uint32_t)(n / min_size[c] + 1);
 }
 
@@ -702,8 +825,9 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         const uint32_t small_waves = plan.wave_base[kSmallClasses];
         const uint32_t small_blocks = qz_div_up(small_waves, kThreads / QZ_WAVE);
         const uint32_t small_sort_blocks = qz_div_up(small_waves, (kThreads / QZ_WAVE) * kSmallTasks);
-        const uint32_t nmed = counts[6];
-        const uint32_t nlarge = counts[7];
+        const uint32_t nwave = counts[kClsWave];
+        const uint32_t nmed = counts[kClsMed];
+        const uint32_t nlarge = counts[kClsLarge];
 
         // large groups: host-side prefix of sizes and per-group tiles
         uint32_t M = 0, lt = 0, lhist = 0;
@@ -711,7 +835,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         std::vector<uint32_t> lb;
         if (nlarge) {
             lg.resize(nlarge);
-            QZ_HIP(hipMemcpyAsync(lg.data(), C.list[7], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipMemcpyAsync(lg.data(), C.list[kClsLarge], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
             QZ_HIP(hipStreamSynchronize(s));
             lb.resize(nlarge + 1);
             for (uint32_t i = 0; i < nlarge; ++i) {
@@ -733,13 +857,18 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                                nblk, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
+        if (nwave) {
+            hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave,
+                               sa_, rank_, off_, nblk, h, bufB_);
+            QZ_HIP(hipGetLastError());
+        }
         if (nmed) {
-            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, C.list[6], C.cnt + 6, sa_, rank_, off_,
-                               nblk, h, bufB_);
+            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, sa_,
+                               rank_, off_, nblk, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
-            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[7], lbase_, sa_, rank_,
+            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, sa_, rank_,
                                off_, nblk, h, lk0, vals0_);
             QZ_HIP(hipGetLastError());
         }
@@ -748,9 +877,14 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), 0, s, cl, plan, bufB_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
+        if (nwave) {
+            hipLaunchKernelGGL(k_sort_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave, bufB_,
+                               sa_, rank_, N);
+            QZ_HIP(hipGetLastError());
+        }
         if (nmed) {
-            hipLaunchKernelGGL(k_sort_med, dim3(nmed), dim3(kThreads), 0, s, C.list[6], C.cnt + 6, bufB_, sa_, rank_,
-                               N);
+            hipLaunchKernelGGL(k_sort_med, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, bufB_,
+                               sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
@@ -763,7 +897,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             hipLaunchKernelGGL(k_heads_large, dim3(lt), dim3(kThreads), 0, s, ltiles, lbase_, keys, bufA_);
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, M, tmp_, s)));
-            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[7], lbase_, keys, vals,
+            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, keys, vals,
                                bufA_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }

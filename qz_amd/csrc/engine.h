@@ -81,6 +81,8 @@ private:
     uint32_t* d0_ = nullptr;
     uint32_t* d1_ = nullptr;
     uint32_t* flag_ = nullptr;
+    uint8_t* buf_ = nullptr;       // first symbols of every segment
+    uint32_t* resume_ = nullptr;   // row after those symbols
     void* pinned_ = nullptr;
     uint64_t cap_tiles_ = 0;
     uint64_t cap_split_ = 0;

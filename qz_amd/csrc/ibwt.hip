@@ -131,12 +131,16 @@ __device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ 
 // soon as their current segment ends instead of idling until the longest
 // lane of the wave is done.
 constexpr uint32_t kPool = 1024;  // splitters per workgroup
+constexpr uint32_t kBuf = 64;      // symbols a segment keeps from the first walk
 
+// Phase 2 also keeps the first kBuf symbols of every segment (in walk order)
+// and, for longer segments, the row reached after kBuf steps.
 __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
                                                          uint32_t nsplit, uint32_t split,
-                                                         uint32_t* __restrict__ nxt, uint32_t* __restrict__ len)
+                                                         uint32_t* __restrict__ nxt, uint32_t* __restrict__ len,
+                                                         uint8_t* __restrict__ buf, uint32_t* __restrict__ resume)
 {
     __shared__ uint32_t next;
     const uint32_t pbeg = blockIdx.x * kPool;
@@ -146,19 +150,30 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restri
     uint32_t sid = atomicAdd(&next, 1u);
     bool active = sid < pend;
     uint32_t rb = 0, sb = 0, x = 0, l = 0;
-    uint64_t e = 0;
+    uint64_t e = 0, word = 0;  // symbols of the current 8-step group, low byte first
     if (active) {
         const uint32_t b = find_block_u32(sbase, nblk, sid);
         rb = off[b] + b;
         sb = sbase[b];
         e = E[rb + (sid - sb) * split];
     }
+    uint64_t* buf64 = reinterpret_cast<uint64_t*>(buf);
     while (__ballot(active)) {
         if (active) {
+            if (l < kBuf) {
+                word |= ((e >> 32) & 0xffull) << (8 * (l & 7));
+                if ((l & 7) == 7) {
+                    buf64[(size_t)sid * (kBuf / 8) + (l >> 3)] = word;
+                    word = 0;
+                }
+            }
             ++l;
             x = (uint32_t)e;
+            if (l == kBuf) resume[sid] = x;
             const uint64_t en = E[rb + x];
             if (en & kSplitFlag) {
+                if (l < kBuf && (l & 7)) buf64[(size_t)sid * (kBuf / 8) + (l >> 3)] = word;
+                word = 0;
                 len[sid] = l;
                 nxt[sid] = (x == 0) ? kNil : sb + x / split;
                 sid = atomicAdd(&next, 1u);
@@ -198,14 +213,32 @@ __global__ __launch_bounds__(kIThreads) void k_ib_jump(const uint32_t* __restric
     if (n2 != kNil) *any = 1u;
 }
 
-// Phase 4: emit symbols. Chain step t (0-based from row 0) writes T[n-1-t].
+// Phase 4a: place the buffered symbols, one wave per splitter: step k of a
+// segment starting at chain step t0 writes T[n-1-t0-k].
+__global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ sbase, uint32_t nblk,
+                                                        uint32_t nsplit, const uint32_t* __restrict__ len,
+                                                        const uint32_t* __restrict__ dist,
+                                                        const uint8_t* __restrict__ buf, uint8_t* __restrict__ T)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * kIThreads + threadIdx.x;
+    const uint32_t sid = (uint32_t)(gid / kBuf), k = (uint32_t)(gid % kBuf);
+    if (sid >= nsplit) return;
+    const uint32_t l = len[sid];
+    if (k >= l) return;
+    const uint32_t b = find_block_u32(sbase, nblk, sid);
+    const uint32_t ob = off[b], n = off[b + 1] - ob;
+    const uint32_t t = n + 1u - dist[sid] + k;
+    if (t < n) T[ob + n - 1u - t] = buf[(size_t)sid * kBuf + k];
+}
+
+// Phase 4b: segments longer than kBuf walk on from their resume row.
 __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
-                                                         uint32_t nsplit, uint32_t split,
-                                                         const uint32_t* __restrict__ len,
+                                                         uint32_t nsplit, const uint32_t* __restrict__ len,
                                                          const uint32_t* __restrict__ dist,
-                                                         uint8_t* __restrict__ T)
+                                                         const uint32_t* __restrict__ resume, uint8_t* __restrict__ T)
 {
     __shared__ uint32_t next;
     const uint32_t pbeg = blockIdx.x * kPool;
@@ -213,29 +246,36 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
     if (threadIdx.x == 0) next = pbeg;
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
-    bool active = sid < pend;
     uint32_t rb = 0, l = 0, t = 0, last = 0;
     uint8_t* dst = nullptr;
     uint64_t e = 0;
-    auto setup = [&]() {
-        const uint32_t b = find_block_u32(sbase, nblk, sid);
-        const uint32_t ob = off[b], n = off[b + 1] - ob;
-        rb = ob + b;
-        l = len[sid];
-        t = n + 1u - dist[sid];
-        last = n;
-        dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
-        e = E[rb + (sid - sbase[b]) * split];
+    // next splitter with a tail, or done
+    auto pull = [&]() -> bool {
+        for (;;) {
+            if (sid >= pend) return false;
+            const uint32_t L = len[sid];
+            if (L > kBuf) {
+                const uint32_t b = find_block_u32(sbase, nblk, sid);
+                const uint32_t ob = off[b], n = off[b + 1] - ob;
+                rb = ob + b;
+                l = L - kBuf;
+                t = n + 1u - dist[sid] + kBuf;
+                last = n;
+                dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
+                e = E[rb + resume[sid]];
+                return true;
+            }
+            sid = atomicAdd(&next, 1u);
+        }
     };
-    if (active) setup();
+    bool active = pull();
     while (__ballot(active)) {
         if (active) {
             if (t < last) dst[-(int64_t)t] = (uint8_t)(e >> 32);
             ++t;
             if (--l == 0) {
                 sid = atomicAdd(&next, 1u);
-                active = sid < pend;
-                if (active) setup();
+                active = pull();
             } else {
                 e = E[rb + (uint32_t)e];
             }
@@ -247,9 +287,9 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / 8 + cb + 1;
+    const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / 32 + cb + 1;
     return 2 * (c + 64) + rows * 8 + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
-           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 5 * nsplit * 4 + 64;
+           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + nsplit * kBuf + 64;
 }
 
 int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -260,7 +300,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb;
     const uint64_t tiles = c / kSortTile + cb + 1;
-    const uint64_t nsplit = rows / 8 + cb + 1;  // smallest split step supported is 8
+    const uint64_t nsplit = rows / 32 + cb + 1;  // smallest split step supported is 32
     QZ_HIP(hipMalloc(&U_, c + 64));
     QZ_HIP(hipMalloc(&T_, c + 64));
     QZ_HIP(hipMalloc(&E_, rows * 8));
@@ -276,6 +316,8 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&d0_, nsplit * 4));
     QZ_HIP(hipMalloc(&d1_, nsplit * 4));
     QZ_HIP(hipMalloc(&flag_, 64));
+    QZ_HIP(hipMalloc(&buf_, nsplit * kBuf));
+    QZ_HIP(hipMalloc(&resume_, nsplit * 4));
     QZ_HIP(hipHostMalloc(&pinned_, 64));
     cap_ = c;
     cap_blk_ = cb;
@@ -286,13 +328,14 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 
 void IbwtEngine::release()
 {
-    void* ps[] = {U_, T_, E_, hist_, tmp_, tiles_, off_, sbase_, prim_, nxt0_, nxt1_, len_, d0_, d1_, flag_};
+    void* ps[] = {U_, T_, E_, hist_, tmp_, tiles_, off_, sbase_, prim_, nxt0_, nxt1_, len_, d0_, d1_, flag_, buf_, resume_};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
     U_ = T_ = nullptr;
     E_ = nullptr;
-    hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = nullptr;
+    hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = resume_ = nullptr;
+    buf_ = nullptr;
     tiles_ = nullptr;
     pinned_ = nullptr;
     cap_ = 0;
@@ -306,7 +349,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     uint32_t split = split_;
     if (const char* e = getenv("QZBSC_SPLIT")) {
         const long v = strtol(e, nullptr, 10);
-        if (v >= 8 && v <= 4096) split = (uint32_t)v;
+        if (v >= 32 && v <= 4096) split = (uint32_t)v;
     }
     // block-aligned tiles
     std::vector<TileDesc> td;
@@ -345,7 +388,8 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     QZ_HIP(hipGetLastError());
     const uint32_t gs = qz_div_up(nsplit, kIThreads);
     const uint32_t gp = qz_div_up(nsplit, kPool);
-    hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_);
+    hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_,
+                       buf_, resume_);
     QZ_HIP(hipGetLastError());
     QZ_HIP(hipMemcpyAsync(d0_, len_, (size_t)nsplit * 4, hipMemcpyDeviceToDevice, s));
     uint32_t *na = nxt0_, *nb_ = nxt1_, *da = d0_, *db = d1_;
@@ -358,7 +402,10 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         std::swap(na, nb_);
         std::swap(da, db);
     }
-    hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, len_, da,
+    hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kBuf, kIThreads)), dim3(kIThreads), 0, s, off_,
+                       sbase_, nblk, nsplit, len_, da, buf_, T);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, len_, da, resume_,
                        T);
     QZ_HIP(hipGetLastError());
     return 0;

@@ -181,3 +181,17 @@ def test_bad_parameters():
     assert L.bsc_compress(ap, op, 100, 16, 128, 1, 2, 1) == -4   # LZP not supported yet
     assert L.bsc_compress(ap, op, 100, 3, 128, 1, 2, 1) == -1    # LZP hash size out of range
     assert L.bsc_compress(ap, op, -1, 0, 0, 1, 2, 1) == -1
+
+
+def test_libbsc_payloads_are_rejected_not_misdecoded():
+    """This engine's adaptive model is its own (libbsc's model constants are
+    not restated): libbsc-written payloads must fail with -6, never decode
+    to wrong data silently."""
+    L = qz_amd.lib()
+    for blk in gold_blocks():
+        n = struct.unpack_from("<i", blk, 4)[0]
+        nidx = blk[-1]
+        pay = np.frombuffer(blk[28:len(blk) - 1 - 4 * nidx], np.uint8).copy()
+        out = np.zeros(n, np.uint8)
+        rc = L.qzbsc_coder_decompress(pay.ctypes.data_as(u8p), pay.size, out.ctypes.data_as(u8p), n, 0)
+        assert rc == -6
