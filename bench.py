@@ -211,7 +211,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--reads", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-block", action="store_true", help="skip the per-block form")
@@ -381,6 +381,7 @@ def main():
         "cpu_baseline": cpu,
         "uneven_batch": uneven,
         "serving": serving,
+        "tuning": bwt.tuning(),
     }
     if per_block:
         out["batched_vs_per_block"] = round(value / per_block["value"], 3)

@@ -137,6 +137,14 @@ QZBSC_API long long qzbsc_server_peak_bytes(void* srv);
  * kind 0 forward BWT, kind 1 inverse BWT. */
 QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
 
+/* Radix digit-width tuner (no counterpart in the reference): which = 0 for
+ * the round-0 suffix sort, 1 for the sort of groups larger than 4096. The
+ * first sorts of >= 4M elements alternate 8- and 11-bit digits, timed with
+ * HIP events; then the faster width is kept for the process
+ * (QZBSC_RADIX_BITS=8|11 forces one). Writes the choice (0 = undecided) and
+ * the mean ns per element of each width; returns the number of trials. */
+QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11);
+
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
 /* 1 if a HIP device is usable, else 0. */

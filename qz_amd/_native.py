@@ -42,6 +42,7 @@ _SIGS = {
                                     C.POINTER(C.c_int)]),
     "qzbsc_server_peak_bytes": (C.c_longlong, [C.c_void_p]),
     "qzbsc_footprint": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
+    "qzbsc_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 
 _lib = None

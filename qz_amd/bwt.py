@@ -118,3 +118,14 @@ def dev_decode_batch(d_in_ptr: int, d_out_ptr: int, sizes, primaries):
     if rc != 0:
         raise BscError(rc, "qzbsc_dev_bwt_decode_batch")
     return ms.value
+
+
+def tuning():
+    """Radix digit-width tuner state for the round-0 and large-group sorts."""
+    res = {}
+    for which, name in ((0, "round0"), (1, "large_groups")):
+        ch, a, b = C.c_int(0), C.c_double(0), C.c_double(0)
+        trials = lib().qzbsc_tuning(which, C.byref(ch), C.byref(a), C.byref(b))
+        res[name] = {"digit_bits": ch.value or None, "trials": trials, "ns_per_elem_8": round(a.value, 4),
+                     "ns_per_elem_11": round(b.value, 4)}
+    return res

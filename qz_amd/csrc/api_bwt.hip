@@ -178,4 +178,13 @@ QZBSC_API int qzbsc_bwt_decode(unsigned char* T, int n, int index, unsigned char
     return qzbsc_bwt_decode_batch(T, T, &n, 1, &index);
 }
 
+// Digit-width tuner state (which = 0: round-0 sort, 1: large-group sort):
+// chosen width (0 while undecided) and mean ns per element for 8 / 11 bits.
+// Returns the number of timed trials.
+QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11)
+{
+    if ((which != 0 && which != 1) || !choice || !ns8 || !ns11) return QZ_BAD_PARAMETER;
+    return tune_state(which, choice, ns8, ns11);
+}
+
 }  // extern "C"

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 namespace qz {
@@ -612,6 +613,68 @@ __global__ void k_bwt_index(const uint32_t* __restrict__ rank, const uint32_t* _
         idx[(size_t)b * 256 + k] = (int32_t)(rank[o + k * r] - o + 1u);
 }
 
+// ---- digit-width autotuner -------------------------------------------------
+// The two radix sorts (round 0 and large groups) run with 8- or 11-bit
+// digits: 11 bits saves a pass on 32-bit keys, 8 bits keeps digit runs long
+// enough for coalesced stores. Both orders are identical, so the tuner
+// alternates the two on the first sorts of at least kTuneMin elements, times
+// them with HIP events, and keeps the faster per element for the process.
+// QZBSC_RADIX_BITS=8|11 forces a width.
+struct RadixTuner {
+    std::mutex mu;
+    int choice = 0;
+    double ns[2] = {0, 0};
+    double elems[2] = {0, 0};
+    int trials = 0;
+};
+RadixTuner g_tune[2];  // [0] round 0, [1] large groups
+constexpr uint64_t kTuneMin = 1u << 22;
+constexpr int kTuneTrials = 2;
+
+static int tune_pick(int which, uint64_t n)
+{
+    if (const char* e = getenv("QZBSC_RADIX_BITS")) {
+        const int v = atoi(e);
+        if (v == 8 || v == 11) return v;
+    }
+    RadixTuner& t = g_tune[which];
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (t.choice) return t.choice;
+    if (n < kTuneMin) return 8;
+    return (t.trials & 1) ? 11 : 8;
+}
+
+static void tune_report(int which, int bits, uint64_t n, float ms)
+{
+    if (n < kTuneMin || ms <= 0) return;
+    RadixTuner& t = g_tune[which];
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (t.choice) return;
+    const int k = bits == 11 ? 1 : 0;
+    t.ns[k] += ms * 1e6;
+    t.elems[k] += (double)n;
+    if (++t.trials >= kTuneTrials && t.elems[0] > 0 && t.elems[1] > 0)
+        t.choice = (t.ns[1] / t.elems[1] < t.ns[0] / t.elems[0]) ? 11 : 8;
+}
+
+int tune_state(int which, int* choice, double* ns8, double* ns11)
+{
+    RadixTuner& t = g_tune[which];
+    std::lock_guard<std::mutex> lk(t.mu);
+    *choice = t.choice;
+    *ns8 = t.elems[0] > 0 ? t.ns[0] / t.elems[0] : 0;
+    *ns11 = t.elems[1] > 0 ? t.ns[1] / t.elems[1] : 0;
+    return t.trials;
+}
+
+template <typename K>
+static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
+                         uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* alt)
+{
+    if (bits == 11) return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt);
+    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt);
+}
+
 static int bitlen(uint64_t v)
 {
     int b = 0;
@@ -645,9 +708,9 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * kRadix;
+    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
     const size_t tmpw =
-        std::max(std::max(radix_tmp_words(c), scan_tmp_words(c)), seg_words + scan_tmp_words(seg_words) + 64) + 1024;
+        std::max(scan_tmp_words(c), seg_words + scan_tmp_words(seg_words) + 64) + 1024;
     uint64_t groups = 0;
     for (int k = 0; k < kClasses; ++k) groups += class_cap(c, k);
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
@@ -658,11 +721,13 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;
+    // grow geometrically so a stream of slowly growing batches reallocates
+    // only a few times
+    const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(n, 1024), cap_ ? cap_ + cap_ / 2 : 0);
+    const uint32_t cb = std::max<uint32_t>(std::max<uint32_t>(nblk, 1), cap_blk_ ? cap_blk_ + cap_blk_ / 2 : 0);
     release();
-    const uint64_t c = std::max<uint64_t>(n, 1024);
-    const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * kRadix;
-    size_t tmpw = std::max(std::max(radix_tmp_words(c), scan_tmp_words(c)), seg_words + scan_tmp_words(seg_words) + 64) +
+    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
+    size_t tmpw = std::max(scan_tmp_words(c), seg_words + scan_tmp_words(seg_words) + 64) +
                   1024;
     QZ_HIP(hipMalloc(&text_, c + 64));
     QZ_HIP(hipMalloc(&out_, c + 64));
@@ -689,6 +754,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     tiles_host_.resize(tile_cap);
     QZ_HIP(hipHostMalloc(&pinned_, 4096));
+    for (auto& ev : tev_)
+        if (!ev) QZ_HIP(hipEventCreate(&ev));
     cap_ = c;
     cap_blk_ = cb;
     return 0;
@@ -749,26 +816,28 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                          make_lists(groups_[1], cnt_ + 32, overflow, cap_)};
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
+    const int bits0 = tune_pick(0, n);
     uint32_t ntiles = 0;
-    const uint32_t hist_words = build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &ntiles);
+    const uint32_t hist_words = build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &ntiles, 1u << bits0);
     QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, s));
     uint32_t* k32a = keys0_;
     uint32_t* k32b = keys1_;
     const uint32_t g = qz_div_up(n, kThreads);
     hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, pp, k32a, vals0_);
     QZ_HIP(hipGetLastError());
-    int shifts[16], np = 0;
+    int symbits = 0;
     {
         uint64_t v = 1;
         for (uint32_t k = 0; k < pp.k0; ++k) v *= pp.radix;
         v -= 1;
-        int symbits = 0;
         while (v) { ++symbits; v >>= 1; }
-        for (int sh = 0; sh < symbits; sh += 8) shifts[np++] = sh;
     }
     bool alt = false;
-    QZ_TRY(seg_radix_sort<uint32_t>(k32a, vals0_, k32b, vals1_, (const TileDesc*)tiles_, ntiles, hist_words, shifts, np,
-                              tmp_, s, &alt));
+    QZ_HIP(hipEventRecord(tev_[0], s));
+    QZ_TRY(seg_sort_bits<uint32_t>(bits0, k32a, vals0_, k32b, vals1_, (const TileDesc*)tiles_, ntiles, hist_words,
+                                   symbits, tmp_, s, &alt));
+    QZ_HIP(hipEventRecord(tev_[1], s));
+    bool timed0 = false;
     {
         uint32_t* keys = alt ? k32b : k32a;
         uint32_t* vals = alt ? vals1_ : vals0_;
@@ -784,6 +853,9 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     // ---- doubling rounds ---------------------------------------------------
     uint32_t h = pp.k0;
     int cur = 0;
+    bool large_timed = false;
+    int large_bits = 8;
+    uint64_t large_n = 0;
     uint32_t* hc = (uint32_t*)pinned_;
     for (;;) {
         QZ_HIP(hipMemcpyAsync(hc, cnt_ + 32 * cur, kClasses * 4, hipMemcpyDeviceToHost, s));
@@ -798,6 +870,16 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         if (((volatile uint32_t*)hc)[16]) {
             fprintf(stderr, "qzbsc: group list overflow\n");
             return QZ_GPU_ERROR;
+        }
+        if (!timed0) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, tev_[0], tev_[1]) == hipSuccess) tune_report(0, bits0, n, ms);
+            timed0 = true;
+        }
+        if (large_timed) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, tev_[2], tev_[3]) == hipSuccess) tune_report(1, large_bits, large_n, ms);
+            large_timed = false;
         }
         if (trace_) {
             fprintf(stderr, "qzbsc: round %d h=%u groups", rounds_, h);
@@ -831,6 +913,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
 
         // large groups: host-side prefix of sizes and per-group tiles
         uint32_t M = 0, lt = 0, lhist = 0;
+        int lbits = 8;
         std::vector<Grp> lg;
         std::vector<uint32_t> lb;
         if (nlarge) {
@@ -843,7 +926,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                 M += lg[i].size;
             }
             lb[nlarge] = M;
-            lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt);
+            lbits = tune_pick(1, M);
+            lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt, 1u << lbits);
             QZ_HIP(hipMemcpyAsync(lbase_, lb.data(), (nlarge + 1) * 4, hipMemcpyHostToDevice, s));
             QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
         }
@@ -888,10 +972,12 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (M) {
-            np = 0;
-            const int b2 = bitlen(n);
-            for (int sh = 0; sh < b2; sh += 8) shifts[np++] = sh;
-            QZ_TRY(seg_radix_sort<uint32_t>(lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, shifts, np, tmp_, s, &alt));
+            QZ_HIP(hipEventRecord(tev_[2], s));
+            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, bitlen(n), tmp_, s, &alt));
+            QZ_HIP(hipEventRecord(tev_[3], s));
+            large_timed = true;
+            large_bits = lbits;
+            large_n = M;
             uint32_t* keys = alt ? lk1 : lk0;
             uint32_t* vals = alt ? vals1_ : vals0_;
             hipLaunchKernelGGL(k_heads_large, dim3(lt), dim3(kThreads), 0, s, ltiles, lbase_, keys, bufA_);

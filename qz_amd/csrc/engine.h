@@ -48,7 +48,12 @@ private:
     std::vector<uint32_t> offs_host_;  // block offsets of the current batch (host copy)
   private:
     void* pinned_ = nullptr;
+    hipEvent_t tev_[4] = {nullptr, nullptr, nullptr, nullptr};  // radix-sort timing for the tuner
 };
+
+// Digit-width tuner state: which = 0 round 0, 1 large groups. Returns the
+// number of timed trials; choice is 0 while undecided.
+int tune_state(int which, int* choice, double* ns8, double* ns11);
 
 class IbwtEngine {
 public:

@@ -295,9 +295,11 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
 int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;
+    // grow geometrically so a stream of slowly growing batches reallocates
+    // only a few times
+    const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(n, 1024), cap_ ? cap_ + cap_ / 2 : 0);
+    const uint32_t cb = std::max<uint32_t>(std::max<uint32_t>(nblk, 1), cap_blk_ ? cap_blk_ + cap_blk_ / 2 : 0);
     release();
-    const uint64_t c = std::max<uint64_t>(n, 1024);
-    const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb;
     const uint64_t tiles = c / kSortTile + cb + 1;
     const uint64_t nsplit = rows / 32 + cb + 1;  // smallest split step supported is 32

@@ -114,140 +114,19 @@ static int device_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort of (key, value) pairs, 8-bit digits.
-// Tile = 256 threads x 16 items, striped so that item order inside a tile is
-// index order; ranking inside a wave uses ballots (peer masks), across waves
-// a per-digit prefix in LDS. Stable.
+// Block-segmented LSD radix sort of (key, u32 value) pairs with BITS-bit
+// digits (8 or 11). Tiles of kSortTile never span two blocks of a batch;
+// histograms are laid out [block][digit][tile], so one exclusive scan over
+// them yields, for every (tile, digit), the global output slot: block b's
+// elements stay inside [off[b], off[b+1]) and no block id bits are needed
+// in the keys. Stable.
 // ---------------------------------------------------------------------------
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
-constexpr int kRadix = 256;
+constexpr int kSortTile = 4096;
 
-template <typename K>
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const K* __restrict__ keys, uint32_t n,
-                                                             int shift, uint32_t* __restrict__ hist,
-                                                             uint32_t tiles)
-{
-    __shared__ uint32_t cnt[kSortThreads / QZ_WAVE][kRadix];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
-#pragma unroll
-    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) cnt[w][tid] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int it = 0; it < kSortItems; ++it) {
-        uint64_t idx = base + (uint64_t)it * kSortThreads + tid;
-        if (idx < n) atomicAdd(&cnt[wave][(uint32_t)(keys[idx] >> shift) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    uint32_t t = 0;
-#pragma unroll
-    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) t += cnt[w][tid];
-    hist[(uint64_t)tid * tiles + blockIdx.x] = t;
-}
-
-template <typename K>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const K* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin,
-                                                                K* __restrict__ kout,
-                                                                uint32_t* __restrict__ vout, uint32_t n,
-                                                                int shift,
-                                                                const uint32_t* __restrict__ offs,
-                                                                uint32_t tiles)
-{
-    constexpr int W = kSortThreads / QZ_WAVE;
-    __shared__ uint32_t run[kRadix];
-    __shared__ uint32_t wh[W][kRadix];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
-    run[tid] = offs[(uint64_t)tid * tiles + blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    for (int it = 0; it < kSortItems; ++it) {
-        const uint64_t idx = base + (uint64_t)it * kSortThreads + tid;
-        const bool valid = idx < n;
-        K k = valid ? kin[idx] : (K)0;
-        uint32_t v = valid ? vin[idx] : 0u;
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
-#pragma unroll
-        for (int w = 0; w < W; ++w) wh[w][tid] = 0;
-        __syncthreads();
-        uint64_t peer = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peer &= bit ? m : ~m;
-        }
-        const uint32_t below = popc_below(peer);
-        if (valid && below == 0) wh[wave][d] = (uint32_t)__popcll(peer);
-        __syncthreads();
-        {
-            uint32_t r = run[tid];
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint32_t c = wh[w][tid];
-                wh[w][tid] = r;
-                r += c;
-            }
-            run[tid] = r;
-        }
-        __syncthreads();
-        if (valid) {
-            const uint32_t pos = wh[wave][d] + below;
-            kout[pos] = k;
-            vout[pos] = v;
-        }
-        __syncthreads();
-    }
-}
-
-// Scratch words a radix sort of n elements needs (histograms + their scan).
-static inline size_t radix_tmp_words(uint64_t n)
-{
-    const size_t tiles = qz_div_up(n ? n : 1, kSortTile);
-    return tiles * kRadix + scan_tmp_words(tiles * kRadix) + 64;
-}
-
-// Sorts by the digits at the given shifts (LSD order: first shift first).
-// Result lands in (k0,v0) if the pass count is even, else (k1,v1); *in_alt
-// reports which.
-template <typename K>
-static int radix_sort_pairs(K* k0, uint32_t* v0, K* k1, uint32_t* v1, uint64_t n, const int* shifts,
-                            int passes, uint32_t* tmp, hipStream_t s, bool* in_alt)
-{
-    *in_alt = false;
-    if (n <= 1 || passes == 0) return 0;
-    const uint32_t tiles = qz_div_up(n, kSortTile);
-    uint32_t* hist = tmp;
-    uint32_t* stmp = tmp + (size_t)tiles * kRadix + 32;
-    K* ka = k0;
-    K* kb = k1;
-    uint32_t* va = v0;
-    uint32_t* vb = v1;
-    for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(kSortThreads), 0, s, ka, (uint32_t)n, shifts[p],
-                           hist, tiles);
-        QZ_HIP(hipGetLastError());
-        QZ_TRY((device_scan<OpAdd, false>(hist, hist, (uint64_t)tiles * kRadix, stmp, s)));
-        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(kSortThreads), 0, s, ka, va, kb, vb,
-                           (uint32_t)n, shifts[p], hist, tiles);
-        QZ_HIP(hipGetLastError());
-        K* tk = ka; ka = kb; kb = tk;
-        uint32_t* tv = va; va = vb; vb = tv;
-    }
-    *in_alt = (passes & 1) != 0;
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Block-segmented radix sort of (u32 key, u32 value) pairs. Tiles never span
-// two blocks; histograms are laid out [block][digit][tile], so one exclusive
-// scan over them yields, for every (tile, digit), the global output slot:
-// block b's elements stay inside [off[b], off[b+1]) and no block id bits are
-// needed in the keys.
-// ---------------------------------------------------------------------------
-// Block-aligned tiles; returns the histogram words (256 per tile).
-static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc* out, uint32_t* count)
+// Block-aligned tiles; returns the histogram words (radix per tile).
+static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc* out, uint32_t* count,
+                                   uint32_t radix = 256)
 {
     uint32_t hbase = 0, k = 0;
     for (uint32_t b = 0; b < nblk; ++b) {
@@ -263,7 +142,7 @@ static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc
             d.tlocal = t;
             out[k++] = d;
         }
-        hbase += nt * kRadix;
+        hbase += nt * radix;
     }
     *count = k;
     return hbase;
@@ -271,51 +150,60 @@ static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc
 
 static inline uint32_t max_tiles(uint64_t n, uint32_t nblk) { return (uint32_t)(n / kSortTile + nblk + 1); }
 
-template <typename K>
+template <typename K, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__ keys,
                                                            const TileDesc* __restrict__ tiles, int shift,
                                                            uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t cnt[kSortThreads / QZ_WAVE][kRadix];
+    constexpr uint32_t D = 1u << BITS;
+    constexpr int W = kSortThreads / QZ_WAVE;
+    __shared__ uint32_t cnt[W][D];
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+    for (uint32_t d = tid; d < D; d += kSortThreads)
 #pragma unroll
-    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) cnt[w][tid] = 0;
+        for (int w = 0; w < W; ++w) cnt[w][d] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < td.len; i += kSortThreads)
-        atomicAdd(&cnt[wave][(uint32_t)(keys[td.start + i] >> shift) & (kRadix - 1)], 1u);
+        atomicAdd(&cnt[wave][(uint32_t)(keys[td.start + i] >> shift) & (D - 1)], 1u);
     __syncthreads();
-    uint32_t t = 0;
+    for (uint32_t d = tid; d < D; d += kSortThreads) {
+        uint32_t t = 0;
 #pragma unroll
-    for (int w = 0; w < kSortThreads / QZ_WAVE; ++w) t += cnt[w][tid];
-    hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] = t;
+        for (int w = 0; w < W; ++w) t += cnt[w][d];
+        hist[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] = t;
+    }
 }
 
 // One LSD pass over a tile of <= 4096 pairs. Each wave ranks its own 1024
-// consecutive elements (16 iterations of 64) with ballot peer masks and
-// wave-private LDS counters; the tile is then permuted by digit in LDS and
-// written out so that consecutive lanes store consecutive slots of a digit
-// run. Stable: element order is wave-major, then iteration, then lane.
-template <typename K>
+// consecutive elements (16 iterations of 64) with BITS ballots per element
+// and wave-private LDS counters; the tile is then permuted by digit in LDS
+// and written out so that consecutive lanes store consecutive slots of a
+// digit run. Stable: element order is wave-major, then iteration, then lane.
+template <typename K, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                               const TileDesc* __restrict__ tiles, int shift,
                                                               const uint32_t* __restrict__ offs)
 {
+    constexpr uint32_t D = 1u << BITS;
+    constexpr uint32_t R = D / kSortThreads;  // digits per thread in the prefix (1 or 8)
     constexpr int W = kSortThreads / QZ_WAVE;
     constexpr int IT = kSortTile / kSortThreads;  // 16
     __shared__ K sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
-    __shared__ uint32_t wcnt[W][kRadix];
-    __shared__ uint32_t lstart[kRadix];
-    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t wcnt[W][D];
+    __shared__ uint32_t lstart[D];
+    __shared__ uint32_t gbase[D];
     __shared__ uint32_t wtot[W];
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
+    for (uint32_t d = tid; d < D; d += kSortThreads) {
 #pragma unroll
-    for (int w = 0; w < W; ++w) wcnt[w][tid] = 0;
-    gbase[tid] = offs[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal];
+        for (int w = 0; w < W; ++w) wcnt[w][d] = 0;
+        gbase[d] = offs[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal];
+    }
     __syncthreads();
     K kk[IT];
     uint32_t vv[IT];
@@ -325,10 +213,10 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
         const uint32_t e = wave * (IT * QZ_WAVE) + it * QZ_WAVE + lane;
         const bool valid = e < td.len;
         const K k = valid ? kin[td.start + e] : (K)0;
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        const uint32_t d = (uint32_t)(k >> shift) & (D - 1);
         uint64_t peer = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peer &= bit ? m : ~m;
@@ -342,26 +230,39 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     }
     __syncthreads();
     {
-        // digit tid: wave offsets and the tile-local start of the digit run
+        // digits [tid*R, tid*R+R): wave offsets per digit, then the tile-local
+        // start of every digit run by a block scan of per-thread totals
         uint32_t tot = 0;
+        uint32_t dtot[R];
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const uint32_t c = wcnt[w][tid];
-            wcnt[w][tid] = tot;
-            tot += c;
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t d = tid * R + r;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint32_t c = wcnt[w][d];
+                wcnt[w][d] = acc;
+                acc += c;
+            }
+            dtot[r] = acc;
+            tot += acc;
         }
         const uint32_t incl = wave_inclusive<OpAdd>(tot);
         if (lane == QZ_WAVE - 1) wtot[wave] = incl;
         __syncthreads();
-        uint32_t pre = 0;
-        for (uint32_t w = 0; w < wave; ++w) pre += wtot[w];
-        lstart[tid] = pre + incl - tot;
+        uint32_t run = incl - tot;
+        for (uint32_t w = 0; w < wave; ++w) run += wtot[w];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            lstart[tid * R + r] = run;
+            run += dtot[r];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         if (rk[it] != 0xffffffffu) {
-            const uint32_t d = (uint32_t)(kk[it] >> shift) & (kRadix - 1);
+            const uint32_t d = (uint32_t)(kk[it] >> shift) & (D - 1);
             const uint32_t pos = lstart[d] + wcnt[wave][d] + rk[it];
             sk[pos] = kk[it];
             sv[pos] = vv[it];
@@ -370,30 +271,32 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     __syncthreads();
     for (uint32_t i = tid; i < td.len; i += kSortThreads) {
         const K k = sk[i];
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        const uint32_t d = (uint32_t)(k >> shift) & (D - 1);
         const uint32_t o = gbase[d] + (i - lstart[d]);
         kout[o] = k;
         vout[o] = sv[i];
     }
 }
 
-// tmp: hist_words + scan scratch. Result in (k0,v0) for even pass counts.
-template <typename K>
+// Sorts by key bits [0, key_bits) with BITS-bit digits. `tiles` must have
+// been built with radix 1 << BITS (hist_words from build_tiles). tmp holds
+// hist_words + scan scratch. Result in (k0,v0) for even pass counts.
+template <typename K, int BITS>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
-                          uint32_t hist_words, const int* shifts, int passes, uint32_t* tmp, hipStream_t s,
-                          bool* in_alt)
+                          uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* in_alt)
 {
     *in_alt = false;
     uint32_t* hist = tmp;
     uint32_t* stmp = tmp + hist_words + 32;
     K *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
-    for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_seg_hist<K>, dim3(ntiles), dim3(kSortThreads), 0, s, ka, tiles, shifts[p], hist);
+    int passes = 0;
+    for (int sh = 0; sh < key_bits; sh += BITS, ++passes) {
+        hipLaunchKernelGGL((k_seg_hist<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, tiles, sh, hist);
         QZ_HIP(hipGetLastError());
         QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
-        hipLaunchKernelGGL(k_seg_scatter<K>, dim3(ntiles), dim3(kSortThreads), 0, s, ka, va, kb, vb, tiles,
-                           shifts[p], hist);
+        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, va, kb, vb, tiles,
+                           sh, hist);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
         uint32_t* tv = va; va = vb; vb = tv;
@@ -401,5 +304,8 @@ static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDe
     *in_alt = (passes & 1) != 0;
     return 0;
 }
+
+// Largest digit width the sorts use (sizes histogram scratch).
+constexpr int kMaxRadixBits = 11;
 
 }  // namespace qz

@@ -245,9 +245,11 @@ __global__ __launch_bounds__(kRankThreads) void k_chunk_rank(const uint8_t* __re
 int QlfcFront::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;
+    // grow geometrically so a stream of slowly growing batches reallocates
+    // only a few times
+    const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(n, 1024), cap_ ? cap_ + cap_ / 2 : 0);
+    const uint32_t cb = std::max<uint32_t>(std::max<uint32_t>(nblk, 1), cap_blk_ ? cap_blk_ + cap_blk_ / 2 : 0);
     release();
-    const uint64_t c = std::max<uint64_t>(n, 1024);
-    const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t nseg = (uint64_t)cb * 8;
     const uint64_t nchunks = c / kChunk + nseg + 1;
     QZ_HIP(hipMalloc(&flags_, c * 4));

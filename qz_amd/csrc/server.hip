@@ -238,11 +238,12 @@ QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_by
     if (hbm_budget <= 0 || max_batch_bytes <= 0 || workers < 1 || workers > 16) return nullptr;
     if (!pool_device_ok()) return nullptr;
     pool_set_limit(std::max(4, workers + 2));
-    // every worker keeps its workspace resident at its largest batch: cap the
-    // batch so that all workers together stay inside the budget
+    // every worker keeps its workspace resident at its largest batch (grown
+    // by at most 1.5x): cap the batch so that all workers together stay
+    // inside the budget
     uint64_t cap = (uint64_t)max_batch_bytes;
     while (cap > (1u << 20) &&
-           (uint64_t)workers * std::max(BwtEngine::footprint(cap, 64), IbwtEngine::footprint(cap, 64)) >
+           (uint64_t)workers * 3 / 2 * std::max(BwtEngine::footprint(cap, 64), IbwtEngine::footprint(cap, 64)) >
                (uint64_t)hbm_budget)
         cap -= cap / 8;
     return new Server((uint64_t)hbm_budget, cap, workers);

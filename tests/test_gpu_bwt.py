@@ -71,3 +71,18 @@ def test_bwt_config1_block(gpu):
     blk = synth.config1_block(seed=1)
     assert blk.size == 15_000_000
     _check([blk], gpu)
+
+
+def test_digit_widths_give_identical_order(gpu, monkeypatch):
+    """8- and 11-bit radix digits (the tuner's two choices) sort alike, on
+    round 0 and on groups larger than 4096 (headers give large groups)."""
+    blocks = [synth.header_stream(60_000, 31), synth.quality_stream(20_000, 150, 32)]
+    res = []
+    for bits in ("8", "11"):
+        monkeypatch.setenv("QZBSC_RADIX_BITS", bits)
+        out, prim, aux = gpu.bwt.encode_batch(blocks)
+        res.append((out, prim))
+    for b, blk in enumerate(blocks):
+        u, p, _ = O.bwt_aux(blk)
+        for out, prim in res:
+            assert np.array_equal(out[b], u) and prim[b] == p
