@@ -86,3 +86,18 @@ def test_digit_widths_give_identical_order(gpu, monkeypatch):
         u, p, _ = O.bwt_aux(blk)
         for out, prim in res:
             assert np.array_equal(out[b], u) and prim[b] == p
+
+
+def test_pathological_repeats(gpu):
+    """Deep suffix sorting: one-symbol and short-period blocks need ~log2(n)
+    doubling rounds with every suffix in one large group."""
+    rng = np.random.default_rng(41)
+    blocks = [np.full(2_000_000, 71, np.uint8),
+              np.tile(np.frombuffer(b"ACGTTGCA", np.uint8), 250_000),
+              np.tile(rng.integers(0, 4, 1000).astype(np.uint8), 1_500),
+              np.concatenate([np.full(1_000_000, 0, np.uint8), np.full(1, 1, np.uint8),
+                              np.full(1_000_000, 0, np.uint8)])]
+    for blk in blocks:
+        print(f"n={blk.size}", flush=True)
+        _check([blk], gpu)
+    _check(blocks, gpu)
