@@ -316,9 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
     stage_flush(S, gl);
 }
 
-// ---- wave groups (65 .. 1024): a 64-thread workgroup per group, bitonic in
-// LDS; barriers cost nothing with one wave and LDS stays small, so many
-// groups share a CU.
+// ---- key gathers of the wave and medium classes (phase A) ----------------
 __global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__ list,
                                                          const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
@@ -334,88 +332,6 @@ __global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__
     }
 }
 
-__global__ __launch_bounds__(QZ_WAVE) void k_sort_wave(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
-                                                       const uint32_t* __restrict__ key2, uint32_t* __restrict__ sa,
-                                                       uint32_t* __restrict__ rank, GroupLists gl)
-{
-    constexpr int kCap = kWaveMax / 2;
-    __shared__ uint32_t K[kWaveMax];
-    __shared__ uint32_t V[kWaveMax];
-    __shared__ uint32_t H[kWaveMax];
-    __shared__ StageT<kCap> S;
-    if (blockIdx.x >= *cnt) return;  // block-uniform
-    stage_init(S);
-    const Grp g = list[blockIdx.x];
-    const uint32_t lane = threadIdx.x;
-    uint32_t P = 128;
-    while (P < g.size) P <<= 1;
-    for (uint32_t i = lane; i < P; i += QZ_WAVE) {
-        const bool v = i < g.size;
-        K[i] = v ? key2[g.start + i] : kPadKey;
-        V[i] = v ? sa[g.start + i] : 0u;
-    }
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            __syncthreads();
-            for (uint32_t i = lane; i < P; i += QZ_WAVE) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const bool up = (i & k) == 0;
-                    const uint32_t a = K[i], b = K[p];
-                    if ((a > b) == up) {
-                        K[i] = b;
-                        K[p] = a;
-                        const uint32_t t = V[i];
-                        V[i] = V[p];
-                        V[p] = t;
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // head positions: contiguous chunk of E = P/64 per lane, then a wave max-scan
-    const uint32_t E = P / QZ_WAVE;
-    const uint32_t b0 = lane * E;
-    uint32_t run = 0;
-    for (uint32_t q = 0; q < E; ++q) {
-        const uint32_t i = b0 + q;
-        const bool head = (i == 0) || K[i] != K[i - 1];
-        if (head) run = i;
-        H[i] = run;
-    }
-    uint32_t incl = run;
-    for (int d = 1; d < QZ_WAVE; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
-        if (lane >= (uint32_t)d && o > incl) incl = o;
-    }
-    uint32_t excl = __shfl_up(incl, 1, QZ_WAVE);
-    if (lane == 0) excl = 0;
-    for (uint32_t q = 0; q < E; ++q) {
-        const uint32_t i = b0 + q;
-        if (H[i] < excl) H[i] = excl;
-    }
-    __syncthreads();
-    for (uint32_t i = lane; i < g.size; i += QZ_WAVE) {
-        const uint32_t hp = H[i];
-        sa[g.start + i] = V[i];
-        rank[V[i]] = g.start + hp;
-        const bool end = (i + 1 == g.size) || K[i + 1] != K[i];
-        if (end && i > hp) stage_push_t(S, g.start + hp, i - hp + 1, kCap);
-    }
-    stage_flush_t(S, gl, kCap);
-}
-
-// ---- medium groups (1025 .. 4096), one workgroup each, bitonic in LDS -----
-// Wave w owns the contiguous quarter [w*P/4, (w+1)*P/4); stages whose stride
-// stays inside a quarter synchronise the wave only, the few cross-quarter
-// stages the workgroup.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
@@ -430,91 +346,246 @@ __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_sort_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
-                                                       const uint32_t* __restrict__ key2, uint32_t* __restrict__ sa,
-                                                       uint32_t* __restrict__ rank, GroupLists gl)
+__device__ __forceinline__ void wave_sync()
 {
-    __shared__ uint32_t K[kMedMax];
-    __shared__ uint32_t V[kMedMax];
-    __shared__ uint32_t H[kMedMax];
-    __shared__ uint32_t wtot[kThreads / QZ_WAVE];
-    __shared__ Stage S;
-    if (blockIdx.x >= *cnt) return;  // block-uniform
-    stage_init(S);
-    const Grp g = list[blockIdx.x];
-    const uint32_t tid = threadIdx.x;
-    uint32_t P = 2048;
-    while (P < g.size) P <<= 1;
-    for (uint32_t i = tid; i < P; i += kThreads) {
-        const bool v = i < g.size;
-        K[i] = v ? key2[g.start + i] : kPadKey;
-        V[i] = v ? sa[g.start + i] : 0u;
-    }
-    const uint32_t quarter = P / (kThreads / QZ_WAVE);
-    const uint32_t qbase = (tid / QZ_WAVE) * quarter, qlane = tid % QZ_WAVE;
-    bool cross_prev = true;  // the loads above came from all waves
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            const bool cross = j >= quarter;
-            if (cross || cross_prev) __syncthreads();
-            else wave_sync();
-            cross_prev = cross;
-            for (uint32_t i = qbase + qlane; i < qbase + quarter; i += QZ_WAVE) {
-                const uint32_t p = i ^ j;
-                if (p > i) {
-                    const bool up = (i & k) == 0;
-                    const uint32_t a = K[i], b = K[p];
-                    if ((a > b) == up) {
-                        K[i] = b;
-                        K[p] = a;
-                        const uint32_t t = V[i];
-                        V[i] = V[p];
-                        V[p] = t;
-                    }
-                }
-            }
-        }
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// ---- wave (65 .. 1024) and medium (1025 .. 4096) groups: one workgroup per
+// group, stable LSD radix sort in LDS -----------------------------------------
+// Keys are ranked relative to the group's minimum, so a group needs
+// ceil(bits(max - min) / 9) passes (none when all keys are equal), with the
+// digit width spread evenly over them. Wave w owns the contiguous chunk
+// [w*C, (w+1)*C) of the current order and ranks 64 keys at a time with one
+// ballot per digit bit against wave-private LDS counters, so ranking needs no
+// barrier; one scan over (digit, wave) then gives every key its slot.
+constexpr int kLdsDigitMax = 9;
+constexpr int kLdsBins = 1 << kLdsDigitMax;
+
+struct StageHdr {
+    uint32_t cnt[kClasses];
+    uint32_t base[kClasses];
+    uint32_t n;
+};
+
+// Packed stage entry: (start, class << 24 | index-in-class << 13 | size), for
+// subgroups of groups <= 4096 (index < 2048, size < 8192).
+__device__ __forceinline__ void stagep_push(StageHdr& S, uint2* E, uint32_t start, uint32_t size)
+{
+    const uint32_t c = (uint32_t)grp_class(size);
+    const uint32_t i = atomicAdd(&S.cnt[c], 1u);
+    const uint32_t k = atomicAdd(&S.n, 1u);
+    E[k] = make_uint2(start, (c << 24) | (i << 13) | size);
+}
+
+__device__ __forceinline__ void stagep_flush(StageHdr& S, const uint2* E, const GroupLists& gl)
+{
     __syncthreads();
-    // head positions: per-thread chunk of E elements, then a block max-scan
-    const uint32_t E = P / kThreads > 0 ? P / kThreads : 1;
-    const uint32_t b0 = tid * E;
-    uint32_t run = 0;
-    for (uint32_t q = 0; q < E; ++q) {
-        const uint32_t i = b0 + q;
-        if (i < P) {
-            const bool head = (i == 0) || K[i] != K[i - 1];
-            if (head) run = i;
-            H[i] = run;
-        }
+    if (threadIdx.x < kClasses && S.cnt[threadIdx.x])
+        S.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S.cnt[threadIdx.x]);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < S.n; k += blockDim.x) {
+        const uint2 e = E[k];
+        const uint32_t c = e.y >> 24;
+        const uint32_t at = S.base[c] + ((e.y >> 13) & 2047u);
+        if (at < gl.cap[c]) gl.list[c][at] = Grp{e.x, e.y & 8191u};
+        else *gl.overflow = 1u;
     }
-    // exclusive max across threads
-    uint32_t incl = run;
-    const uint32_t lane = lane_id(), wave = tid / QZ_WAVE;
+}
+
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    uint32_t incl = v;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
+        if (lane >= (uint32_t)d) incl += o;
+    }
+    if (WAVES == 1) return incl - v;
+    if (lane == QZ_WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t i = 0; i < w; ++i) pre += wsum[i];
+    return pre + incl - v;
+}
+
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* wsum)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    uint32_t incl = v;
     for (int d = 1; d < QZ_WAVE; d <<= 1) {
         const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
         if (lane >= (uint32_t)d && o > incl) incl = o;
     }
-    if (lane == QZ_WAVE - 1) wtot[wave] = incl;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t w = 0; w < wave; ++w) pre = wtot[w] > pre ? wtot[w] : pre;
     uint32_t excl = __shfl_up(incl, 1, QZ_WAVE);
     if (lane == 0) excl = 0;
-    if (excl < pre) excl = pre;
-    for (uint32_t q = 0; q < E; ++q) {
-        const uint32_t i = b0 + q;
-        if (i < P && H[i] < excl) H[i] = excl;
+    if (WAVES == 1) return excl;
+    if (lane == QZ_WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    for (uint32_t i = 0; i < w; ++i) excl = wsum[i] > excl ? wsum[i] : excl;
+    return excl;
+}
+
+// Sorts n <= WAVES*64*NB (key, value) pairs of (K, V); returns true when the
+// result ends in (K2, V2). Starts and ends with a workgroup barrier.
+template <int WAVES, int NB>
+__device__ bool lds_radix(uint32_t* K, uint32_t* V, uint32_t* K2, uint32_t* V2, uint32_t* cntr, uint32_t* wsum,
+                          uint32_t n, uint32_t kmin, uint32_t bits)
+{
+    constexpr uint32_t T = WAVES * QZ_WAVE;
+    constexpr uint32_t DPT = kLdsBins / T;  // counters per thread in the scan
+    if (bits == 0) return false;
+    const uint32_t passes = ceil_div(bits, (uint32_t)kLdsDigitMax);
+    const uint32_t db = ceil_div(bits, passes);
+    const uint32_t dmask = (1u << db) - 1u;
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    const uint32_t chunk = ceil_div(n, T) * QZ_WAVE;
+    const uint32_t lo = w * chunk;
+    const uint32_t hi = lo + chunk < n ? lo + chunk : n;
+    uint32_t* Cw = cntr + w * kLdsBins;
+    uint32_t *Ki = K, *Vi = V, *Ko = K2, *Vo = V2;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t sh = p * db;
+        for (uint32_t i = lane; i < kLdsBins; i += QZ_WAVE) Cw[i] = 0;
+        wave_sync();
+        uint32_t kk[NB], rk[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t base = lo + b * QZ_WAVE;
+            kk[b] = 0;
+            rk[b] = 0;
+            if (base < hi) {  // wave-uniform
+                const uint32_t e = base + lane;
+                const bool valid = e < hi;
+                const uint32_t k = valid ? Ki[e] : kmin;
+                const uint32_t d = ((k - kmin) >> sh) & dmask;
+                uint64_t m = __ballot(valid);
+                for (uint32_t j = 0; j < db; ++j) {
+                    const bool bit = (d >> j) & 1u;
+                    const uint64_t bb = __ballot(bit);
+                    m &= bit ? bb : ~bb;
+                }
+                const uint32_t pre = Cw[d];
+                const uint32_t below = popc_below(m);
+                const uint32_t tot = (uint32_t)__popcll(m);
+                kk[b] = k;
+                rk[b] = pre + below;
+                if (valid && below + 1 == tot) Cw[d] = pre + tot;
+            }
+        }
+        __syncthreads();
+        {
+            const uint32_t d0 = threadIdx.x * DPT;
+            uint32_t tot = 0;
+            for (uint32_t q = 0; q < DPT; ++q)
+                for (int v = 0; v < WAVES; ++v) tot += cntr[v * kLdsBins + d0 + q];
+            uint32_t run = block_excl_sum<WAVES>(tot, wsum);
+            for (uint32_t q = 0; q < DPT; ++q)
+                for (int v = 0; v < WAVES; ++v) {
+                    const uint32_t x = cntr[v * kLdsBins + d0 + q];
+                    cntr[v * kLdsBins + d0 + q] = run;
+                    run += x;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t e = lo + b * QZ_WAVE + lane;
+            if (e < hi) {
+                const uint32_t d = ((kk[b] - kmin) >> sh) & dmask;
+                const uint32_t dst = Cw[d] + rk[b];
+                Ko[dst] = kk[b];
+                Vo[dst] = Vi[e];
+            }
+        }
+        __syncthreads();
+        uint32_t* t = Ki; Ki = Ko; Ko = t;
+        t = Vi; Vi = Vo; Vo = t;
+    }
+    return (passes & 1u) != 0;
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restrict__ list,
+                                                              const uint32_t* __restrict__ cnt,
+                                                              const uint32_t* __restrict__ key2,
+                                                              uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
+                                                              GroupLists gl)
+{
+    constexpr uint32_t T = WAVES * QZ_WAVE;
+    constexpr uint32_t MAXN = WAVES == 1 ? kWaveMax : kMedMax;
+    constexpr int NB = MAXN / T;
+    static_assert(kLdsBins % T == 0, "scan layout");
+    __shared__ uint32_t K[MAXN], V[MAXN], K2[MAXN], V2[MAXN];
+    __shared__ uint32_t cntr[WAVES * kLdsBins];
+    __shared__ uint32_t wsum[2 * WAVES];
+    __shared__ StageHdr S;
+    if (blockIdx.x >= *cnt) return;  // block-uniform
+    const Grp g = list[blockIdx.x];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / QZ_WAVE;
+    if (tid < kClasses) S.cnt[tid] = 0;
+    if (tid == 0) S.n = 0;
+    uint32_t kmin = 0xffffffffu, kmax = 0;
+    for (uint32_t i = tid; i < g.size; i += T) {
+        const uint32_t k = key2[g.start + i];
+        K[i] = k;
+        V[i] = sa[g.start + i];
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+    }
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t a = __shfl_xor(kmin, d, QZ_WAVE), b = __shfl_xor(kmax, d, QZ_WAVE);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    if (lane == 0) {
+        wsum[w] = kmin;
+        wsum[WAVES + w] = kmax;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < g.size; i += kThreads) {
-        const uint32_t hp = H[i];
-        sa[g.start + i] = V[i];
-        rank[V[i]] = g.start + hp;
-        const bool end = (i + 1 == g.size) || K[i + 1] != K[i];
-        if (end && i > hp) stage_push(S, g.start + hp, i - hp + 1);
+    for (int v = 0; v < WAVES; ++v) {
+        kmin = wsum[v] < kmin ? wsum[v] : kmin;
+        kmax = wsum[WAVES + v] > kmax ? wsum[WAVES + v] : kmax;
     }
-    stage_flush(S, gl);
+    __syncthreads();
+    const uint32_t span = kmax - kmin;
+    const uint32_t bits = span ? 32u - (uint32_t)__builtin_clz(span) : 0u;
+    const bool flip = lds_radix<WAVES, NB>(K, V, K2, V2, cntr, wsum, g.size, kmin, bits);
+    const uint32_t* Kf = flip ? K2 : K;
+    const uint32_t* Vf = flip ? V2 : V;
+    uint32_t* H = flip ? K : K2;
+    uint2* E = reinterpret_cast<uint2*>(flip ? V : V2);  // MAXN/2 entries >= subgroups of size >= 2
+    // head position of every slot: per-thread chunk, then an exclusive max-scan
+    const uint32_t per = ceil_div(g.size, T);
+    const uint32_t b0 = tid * per;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t i = b0 + q;
+        if (i < g.size) {
+            if (i == 0 || Kf[i] != Kf[i - 1]) run = i;
+            H[i] = run;
+        }
+    }
+    const uint32_t excl = block_excl_max<WAVES>(run, wsum);
+    for (uint32_t q = 0; q < per; ++q) {
+        const uint32_t i = b0 + q;
+        if (i < g.size && H[i] < excl) H[i] = excl;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < g.size; i += T) {
+        const uint32_t hp = H[i];
+        const uint32_t v = Vf[i];
+        sa[g.start + i] = v;
+        rank[v] = g.start + hp;
+        const bool end = (i + 1 == g.size) || Kf[i + 1] != Kf[i];
+        if (end && i > hp) stagep_push(S, E, g.start + hp, i - hp + 1);
+    }
+    stagep_flush(S, E, gl);
 }
 
 // ---- large groups (> 4096): block-segmented radix sort of key2, one -----
@@ -926,6 +997,18 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                 M += lg[i].size;
             }
             lb[nlarge] = M;
+            if (trace_) {
+                uint64_t sz[5] = {0, 0, 0, 0, 0};  // elements in groups <= 8K, 16K, 64K, 1M, larger
+                uint32_t mx = 0;
+                for (uint32_t i = 0; i < nlarge; ++i) {
+                    const uint32_t z = lg[i].size;
+                    sz[z <= 8192 ? 0 : z <= 16384 ? 1 : z <= 65536 ? 2 : z <= (1u << 20) ? 3 : 4] += z;
+                    mx = std::max(mx, z);
+                }
+                fprintf(stderr, "qzbsc:   large M=%u max=%u by size <=8K %llu <=16K %llu <=64K %llu <=1M %llu more %llu\n", M,
+                        mx, (unsigned long long)sz[0], (unsigned long long)sz[1], (unsigned long long)sz[2],
+                        (unsigned long long)sz[3], (unsigned long long)sz[4]);
+            }
             lbits = tune_pick(1, M);
             lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt, 1u << lbits);
             QZ_HIP(hipMemcpyAsync(lbase_, lb.data(), (nlarge + 1) * 4, hipMemcpyHostToDevice, s));
@@ -962,12 +1045,12 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
-            hipLaunchKernelGGL(k_sort_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave, bufB_,
+            hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave, bufB_,
                                sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
-            hipLaunchKernelGGL(k_sort_med, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, bufB_,
+            hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, bufB_,
                                sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
