@@ -144,6 +144,45 @@ def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
     }
 
 
+def graph_section(torch, bwt, synth, sizes=(100_000, 1_000_000, 4_000_000), jobs: int = 40):
+    """Per-job latency of one-block forward + inverse BWT jobs on the same
+    device buffers, eager launches versus the HIP-graph replay of the fixed
+    launch sequences (inverse BWT, forward round 0)."""
+    dev = torch.device("cuda:0")
+    stream = synth.sequence_stream(5_000_000, 40_000, 150, 9, 0.002, 0.0001)
+    rows = []
+    for n in sizes:
+        blk = stream[:n]
+        d_in = torch.from_numpy(np.ascontiguousarray(blk)).to(dev)
+        d_bwt = torch.empty_like(d_in)
+        d_back = torch.empty_like(d_in)
+        sz = np.array([n], dtype=np.int32)
+        torch.cuda.synchronize()
+        res = {"bytes": n}
+        for mode in ("0", "1"):
+            os.environ["QZBSC_GRAPH"] = mode
+            lat = []
+            for j in range(jobs + 2):  # two warm calls: eager, then capture
+                ts = time.perf_counter()
+                prim, _, _ = bwt.dev_encode_batch(d_in.data_ptr(), d_bwt.data_ptr(), sz)
+                bwt.dev_decode_batch(d_bwt.data_ptr(), d_back.data_ptr(), sz, prim)
+                if j >= 2:
+                    lat.append((time.perf_counter() - ts) * 1e3)
+            res["graph_p50_ms" if mode == "1" else "eager_p50_ms"] = round(float(np.percentile(lat, 50)), 4)
+        torch.cuda.synchronize()
+        if not torch.equal(d_back, d_in):
+            print("bench.py: graph section round trip failed", file=sys.stderr)
+            sys.exit(1)
+        res["speedup"] = round(res["eager_p50_ms"] / res["graph_p50_ms"], 3)
+        rows.append(res)
+        del d_in, d_bwt, d_back
+    os.environ.pop("QZBSC_GRAPH", None)
+    c, r = bwt.graph_stats()
+    return {"workload": f"{jobs} one-block jobs per size on the same device buffers (sequence stream), "
+                        "forward + inverse BWT, host wall time per job",
+            "rows": rows, "captures": c, "replays": r}
+
+
 def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int):
     """Poisson arrivals of compress (forward BWT) and decompress (inverse
     BWT) jobs, 70/30, each 1..40 blocks of 100 KB..25 MiB from the mixed
@@ -220,6 +259,7 @@ def main():
     ap.add_argument("--uneven-blocks", type=int, default=512)
     ap.add_argument("--batch-cap", type=int, default=1_000_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
+    ap.add_argument("--no-graph", action="store_true", help="skip the eager-vs-graph section")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
@@ -334,6 +374,11 @@ def main():
         "bwt_ibwt_frac": round((ALG_BYTES_FWD + ALG_BYTES_INV) * total / (fwd + inv) / 1e9 / HBM_PEAK_GBS, 5),
     }
 
+    graphs = None
+    if not args.no_graph:
+        graphs = graph_section(torch, bwt, synth)
+        log(f"graphs: {graphs}")
+
     uneven = None
     if not args.no_uneven:
         uneven = uneven_section(torch, bwt, synth, args.uneven_blocks, 4, args.batch_cap)
@@ -379,6 +424,7 @@ def main():
         "per_block": per_block,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "graph_replay": graphs,
         "uneven_batch": uneven,
         "serving": serving,
         "tuning": bwt.tuning(),

@@ -145,6 +145,12 @@ QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
  * the mean ns per element of each width; returns the number of trials. */
 QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11);
 
+/* HIP graph replay counts of this process (no reference counterpart).
+ * Fixed launch sequences (the inverse BWT, round 0 of the forward sort) are
+ * captured the second time in a row that an engine sees a batch shape and
+ * replayed while the shape repeats; QZBSC_GRAPH=0 turns replay off. */
+QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long* replays);
+
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
 /* 1 if a HIP device is usable, else 0. */

@@ -43,6 +43,7 @@ _SIGS = {
     "qzbsc_server_peak_bytes": (C.c_longlong, [C.c_void_p]),
     "qzbsc_footprint": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
     "qzbsc_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "qzbsc_graph_stats": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
 }
 
 _lib = None

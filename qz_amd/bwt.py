@@ -129,3 +129,10 @@ def tuning():
         res[name] = {"digit_bits": ch.value or None, "trials": trials, "ns_per_elem_8": round(a.value, 4),
                      "ns_per_elem_11": round(b.value, 4)}
     return res
+
+
+def graph_stats():
+    """(captures, replays) of HIP graphs in this process."""
+    c, r = C.c_ulonglong(0), C.c_ulonglong(0)
+    lib().qzbsc_graph_stats(C.byref(c), C.byref(r))
+    return c.value, r.value

@@ -187,4 +187,12 @@ QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11)
     return tune_state(which, choice, ns8, ns11);
 }
 
+QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long* replays)
+{
+    if (!captures || !replays) return QZ_BAD_PARAMETER;
+    *captures = g_graph_captures.load();
+    *replays = g_graph_replays.load();
+    return 0;
+}
+
 }  // extern "C"

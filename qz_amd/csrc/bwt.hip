@@ -158,23 +158,59 @@ struct PackParams {
 
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           const uint32_t* __restrict__ off, uint32_t nblk,
-                                                          PackParams pp, uint32_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals)
+                                                          const PackParams* __restrict__ ppd,
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     __shared__ uint16_t code[256];
-    code[threadIdx.x] = pp.code[threadIdx.x];
+    code[threadIdx.x] = ppd->code[threadIdx.x];
+    const uint32_t radix = ppd->radix, k0 = ppd->k0;
     __syncthreads();
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     const uint32_t end = off[find_block(off, nblk, i) + 1];
     uint32_t key = 0;
-    for (uint32_t j = 0; j < pp.k0; ++j) {
+    for (uint32_t j = 0; j < k0; ++j) {
         const uint32_t p = i + j;
         const uint32_t c = (p < end) ? code[text[p]] : 0u;
-        key = key * pp.radix + c;
+        key = key * radix + c;
     }
     keys[i] = key;
     vals[i] = i;
+}
+
+// Symbol packing from the byte histogram (one workgroup of 256): present
+// symbols get codes 1..sigma in byte order, 0 means past the block end, and
+// k0 is the largest k with (sigma+1)^k <= 2^32.
+__global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp)
+{
+    __shared__ uint32_t wsum[kThreads / QZ_WAVE];
+    const uint32_t c = threadIdx.x, lane = lane_id(), w = c / QZ_WAVE;
+    const uint32_t present = hist[c] ? 1u : 0u;
+    uint32_t incl = present;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
+        if (lane >= (uint32_t)d) incl += o;
+    }
+    if (lane == QZ_WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    for (uint32_t i = 0; i < w; ++i) incl += wsum[i];
+    pp->code[c] = present ? (uint16_t)incl : (uint16_t)0;
+    if (c == kThreads - 1) {
+        const uint64_t radix = (uint64_t)incl + 1;
+        uint64_t v = 1;
+        uint32_t k = 0;
+        while (k < 32 && v * radix <= (1ull << 32)) {
+            v *= radix;
+            ++k;
+        }
+        pp->radix = (uint32_t)radix;
+        pp->k0 = k ? k : 1;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_zero_u32(uint32_t* __restrict__ p, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += kThreads) p[i] = 0;
 }
 
 // hv[j] = j at group heads (key change or block start), 0 elsewhere.
@@ -715,6 +751,15 @@ static int tune_pick(int which, uint64_t n)
     return (t.trials & 1) ? 11 : 8;
 }
 
+// True while the tuner still times sorts of n elements (its picks alternate).
+static bool tune_wants(int which, uint64_t n)
+{
+    if (getenv("QZBSC_RADIX_BITS")) return false;
+    RadixTuner& t = g_tune[which];
+    std::lock_guard<std::mutex> lk(t.mu);
+    return !t.choice && n >= kTuneMin;
+}
+
 static void tune_report(int which, int bits, uint64_t n, float ms)
 {
     if (n < kTuneMin || ms <= 0) return;
@@ -753,20 +798,6 @@ static int bitlen(uint64_t v)
     return b;
 }
 
-static PackParams make_pack(const uint32_t* hist)
-{
-    PackParams pp;
-    int sigma = 0;
-    for (int c = 0; c < 256; ++c) pp.code[c] = hist[c] ? (uint16_t)(++sigma) : 0;
-    pp.radix = (uint32_t)sigma + 1;
-    // largest k with radix^k <= 2^32
-    uint64_t v = 1;
-    uint32_t k = 0;
-    while (k < 32 && v * pp.radix <= (1ull << 32)) { v *= pp.radix; ++k; }
-    pp.k0 = k ? k : 1;
-    return pp;
-}
-
 static uint32_t class_cap(uint64_t n, int c)
 {
     // a class-c group has more than min_size - 1 members
@@ -786,7 +817,8 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     for (int k = 0; k < kClasses; ++k) groups += class_cap(c, k);
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
     return 2 * (c + 64) + 8 * 4 * c + tmpw * 4 + (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
-           2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + tile_cap * sizeof(TileDesc);
+           2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
+           sizeof(PackParams);
 }
 
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -823,6 +855,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;  // round 0 or large groups
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
+    QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
+    QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
     tiles_host_.resize(tile_cap);
     QZ_HIP(hipHostMalloc(&pinned_, 4096));
     for (auto& ev : tev_)
@@ -835,7 +869,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 void BwtEngine::release()
 {
     void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, tmp_, off_, idx_, hist_,
-                  cnt_, groups_[0], groups_[1], lbase_, tiles_};
+                  cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_};
+    g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
@@ -843,7 +878,7 @@ void BwtEngine::release()
     sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = cnt_ = lbase_ = nullptr;
     keys0_ = keys1_ = nullptr;
     groups_[0] = groups_[1] = nullptr;
-    tiles_ = nullptr;
+    tiles_ = tiles0_ = ppd_ = nullptr;
     idx_ = nullptr;
     pinned_ = nullptr;
     cap_ = 0;
@@ -866,63 +901,80 @@ static GroupLists make_lists(void* base, uint32_t* cnt, uint32_t* overflow, uint
 }
 
 // Sorts the suffixes of the batch in text (device); off_ holds the offsets.
+// Round 0 as one launch sequence with no host round trip, so it can be
+// replayed as a graph: clear the counters, count symbols, derive the packing
+// on the device, build the packed prefix keys, sort them (32 key bits),
+// then mark group heads and emit SA, ranks and the unsorted groups.
+int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
+                              uint32_t hist_words, bool timed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, 64u);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
+                       text, n, hist_);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_);
+    QZ_HIP(hipGetLastError());
+    const uint32_t g = qz_div_up(n, kThreads);
+    hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, (const PackParams*)ppd_,
+                       keys0_, vals0_);
+    QZ_HIP(hipGetLastError());
+    bool alt = false;
+    if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
+    QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
+                                   32, tmp_, s, &alt));
+    if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
+    uint32_t* keys = alt ? keys1_ : keys0_;
+    uint32_t* vals = alt ? vals1_ : vals0_;
+    hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, off_, nblk, bufA_);
+    QZ_HIP(hipGetLastError());
+    QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, n, tmp_, s)));
+    uint32_t* overflow = cnt_ + 2 * kClasses;
+    const GroupLists gl = make_lists(groups_[0], cnt_, overflow, cap_);
+    hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n, off_,
+                       nblk, bufA_, sa_, rank_, gl);
+    QZ_HIP(hipGetLastError());
+    return 0;
+}
+
+// Sorts the suffixes of the batch in text (device); off_ holds the offsets.
 int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s)
 {
     rounds_ = 0;
     trace_ = getenv("QZBSC_TRACE") != nullptr;
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
-    QZ_HIP(hipMemsetAsync(hist_, 0, 256 * 4, s));
-    QZ_HIP(hipMemsetAsync(cnt_, 0, 2 * 32 * 4, s));
-    hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
-                       text, n, hist_);
-    QZ_HIP(hipGetLastError());
-    uint32_t hh[256];
-    QZ_HIP(hipMemcpyAsync(hh, hist_, sizeof(hh), hipMemcpyDeviceToHost, s));
-    QZ_HIP(hipStreamSynchronize(s));
-    const PackParams pp = make_pack(hh);
-    k0_ = pp.k0;
-
     uint32_t* overflow = cnt_ + 2 * kClasses;
     GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_),
                          make_lists(groups_[1], cnt_ + 32, overflow, cap_)};
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
     const int bits0 = tune_pick(0, n);
+    const bool timed0 = tune_wants(0, n);  // the tuner is still timing: run eagerly with events
     uint32_t ntiles = 0;
-    const uint32_t hist_words = build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &ntiles, 1u << bits0);
-    QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, s));
-    uint32_t* k32a = keys0_;
-    uint32_t* k32b = keys1_;
-    const uint32_t g = qz_div_up(n, kThreads);
-    hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, pp, k32a, vals0_);
-    QZ_HIP(hipGetLastError());
-    int symbits = 0;
-    {
-        uint64_t v = 1;
-        for (uint32_t k = 0; k < pp.k0; ++k) v *= pp.radix;
-        v -= 1;
-        while (v) { ++symbits; v >>= 1; }
-    }
-    bool alt = false;
-    QZ_HIP(hipEventRecord(tev_[0], s));
-    QZ_TRY(seg_sort_bits<uint32_t>(bits0, k32a, vals0_, k32b, vals1_, (const TileDesc*)tiles_, ntiles, hist_words,
-                                   symbits, tmp_, s, &alt));
-    QZ_HIP(hipEventRecord(tev_[1], s));
-    bool timed0 = false;
-    {
-        uint32_t* keys = alt ? k32b : k32a;
-        uint32_t* vals = alt ? vals1_ : vals0_;
-        hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, off_, nblk, bufA_);
-        QZ_HIP(hipGetLastError());
-        QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, n, tmp_, s)));
-        hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n,
-                           off_, nblk, bufA_, sa_, rank_, gls[0]);
-        QZ_HIP(hipGetLastError());
+    const uint32_t hist_words = count_tiles(offs_host_.data(), nblk, &ntiles, 1u << bits0);
+    auto prepare0 = [&](hipStream_t st) -> int {
+        uint32_t nt = 0;
+        build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &nt, 1u << bits0);
+        QZ_HIP(hipMemcpyAsync(tiles0_, tiles_host_.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    auto launch0 = [&](hipStream_t st) -> int {
+        return enqueue_round0(text, n, nblk, bits0, ntiles, hist_words, timed0, st);
+    };
+    if (timed0) {
+        g0_.drop();
+        QZ_TRY(prepare0(s));
+        QZ_TRY(launch0(s));
+    } else {
+        std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_};
+        key.insert(key.end(), offs_host_.begin(), offs_host_.end());
+        QZ_TRY(g0_.run(key, s, prepare0, launch0));
     }
     ++rounds_;
 
     // ---- doubling rounds ---------------------------------------------------
-    uint32_t h = pp.k0;
+    uint32_t h = 0;  // k0, read back with the first counts
     int cur = 0;
     bool large_timed = false;
     int large_bits = 8;
@@ -931,7 +983,13 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     for (;;) {
         QZ_HIP(hipMemcpyAsync(hc, cnt_ + 32 * cur, kClasses * 4, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipMemcpyAsync(hc + 16, overflow, 4, hipMemcpyDeviceToHost, s));
+        if (rounds_ == 1)
+            QZ_HIP(hipMemcpyAsync(hc + 20, &((const PackParams*)ppd_)->k0, 4, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipStreamSynchronize(s));
+        if (rounds_ == 1) {
+            h = ((volatile uint32_t*)hc)[20];
+            k0_ = h;
+        }
         uint32_t counts[kClasses];
         uint64_t total = 0;
         for (int k = 0; k < kClasses; ++k) {
@@ -942,10 +1000,9 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             fprintf(stderr, "qzbsc: group list overflow\n");
             return QZ_GPU_ERROR;
         }
-        if (!timed0) {
+        if (timed0 && rounds_ == 1) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, tev_[0], tev_[1]) == hipSuccess) tune_report(0, bits0, n, ms);
-            timed0 = true;
         }
         if (large_timed) {
             float ms = 0;
@@ -1056,6 +1113,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         if (M) {
             QZ_HIP(hipEventRecord(tev_[2], s));
+            bool alt = false;
             QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, bitlen(n), tmp_, s, &alt));
             QZ_HIP(hipEventRecord(tev_[3], s));
             large_timed = true;

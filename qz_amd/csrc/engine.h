@@ -2,6 +2,7 @@
 // workspace, grown on demand and kept resident between calls.
 #pragma once
 #include "common.h"
+#include "graph.h"
 #include <vector>
 
 namespace qz {
@@ -31,6 +32,11 @@ public:
 
 private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
+    int enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
+                       uint32_t hist_words, bool timed, hipStream_t s);
+    void* tiles0_ = nullptr;       // round-0 radix tiles (device; read by the round-0 graph)
+    void* ppd_ = nullptr;          // symbol packing of round 0 (device)
+    GraphCache g0_;                // round 0 as one graph
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
     uint32_t* keys0_ = nullptr;
@@ -91,6 +97,9 @@ private:
     void* pinned_ = nullptr;
     uint64_t cap_tiles_ = 0;
     uint64_t cap_split_ = 0;
+    GraphCache graph_;             // the whole launch sequence as one graph
+    std::vector<TileDesc> td_host_;
+    std::vector<uint32_t> sb_host_;
 };
 
 }  // namespace qz

@@ -342,6 +342,7 @@ void IbwtEngine::release()
     pinned_ = nullptr;
     cap_ = 0;
     cap_blk_ = 0;
+    graph_.reset();
 }
 
 // off_ and prim_ are staged; reads U, writes T (device).
@@ -353,64 +354,68 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         const long v = strtol(e, nullptr, 10);
         if (v >= 32 && v <= 4096) split = (uint32_t)v;
     }
-    // block-aligned tiles
-    std::vector<TileDesc> td;
-    td.reserve(n / kSortTile + nblk + 1);
-    uint32_t hbase = 0;
-    std::vector<uint32_t> sb(nblk + 1);
+    // block-aligned tiles and per-block splitter bases (sizes only; the
+    // descriptors are built when the graph is not replayed)
+    uint32_t ntiles = 0;
+    const uint32_t hbase = count_tiles(offs.data(), nblk, &ntiles, 256);
     uint32_t nsplit = 0;
-    for (uint32_t b = 0; b < nblk; ++b) {
-        const uint32_t o = offs[b], nb = offs[b + 1] - o;
-        const uint32_t nt = qz_div_up(nb, kSortTile);
-        for (uint32_t t = 0; t < nt; ++t) {
-            TileDesc d;
-            d.start = o + t * kSortTile;
-            d.len = std::min<uint32_t>(kSortTile, nb - t * kSortTile);
-            d.block = b;
-            d.hbase = hbase;
-            d.ntiles = nt;
-            d.tlocal = t;
-            td.push_back(d);
+    for (uint32_t b = 0; b < nblk; ++b) nsplit += (offs[b + 1] - offs[b]) / split + 1;
+    if (ntiles > cap_tiles_ || nsplit > cap_split_) return QZ_BAD_PARAMETER;
+    auto prepare = [&](hipStream_t st) -> int {
+        // host staging lives in the engine, so the copies may complete late
+        td_host_.resize(ntiles);
+        uint32_t nt = 0;
+        build_tiles(offs.data(), nblk, td_host_.data(), &nt, 256);
+        sb_host_.resize(nblk + 1);
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            sb_host_[b] = acc;
+            acc += (offs[b + 1] - offs[b]) / split + 1;
         }
-        hbase += nt * 256;
-        sb[b] = nsplit;
-        nsplit += nb / split + 1;
-    }
-    sb[nblk] = nsplit;
-    if (td.size() > cap_tiles_ || nsplit > cap_split_) return QZ_BAD_PARAMETER;
-    QZ_HIP(hipMemcpyAsync(tiles_, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
-    QZ_HIP(hipMemcpyAsync(sbase_, sb.data(), sb.size() * 4, hipMemcpyHostToDevice, s));
-    const uint32_t ntiles = (uint32_t)td.size();
-    hipLaunchKernelGGL(k_ib_hist, dim3(ntiles), dim3(kIThreads), 0, s, U, (const TileDesc*)tiles_, hist_);
-    QZ_HIP(hipGetLastError());
-    QZ_TRY((device_scan<OpAdd, false>(hist_, hist_, hbase, tmp_, s)));
-    hipLaunchKernelGGL(k_ib_rank, dim3(ntiles), dim3(kIThreads), 0, s, U, (const TileDesc*)tiles_, hist_, off_, prim_, split, E_);
-    QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, s, off_, prim_, nblk, split, E_);
-    QZ_HIP(hipGetLastError());
-    const uint32_t gs = qz_div_up(nsplit, kIThreads);
-    const uint32_t gp = qz_div_up(nsplit, kPool);
-    hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, split, nxt0_, len_,
-                       buf_, resume_);
-    QZ_HIP(hipGetLastError());
-    QZ_HIP(hipMemcpyAsync(d0_, len_, (size_t)nsplit * 4, hipMemcpyDeviceToDevice, s));
-    uint32_t *na = nxt0_, *nb_ = nxt1_, *da = d0_, *db = d1_;
-    // ceil(log2(nsplit)) + 1 rounds always suffice
-    int rounds = 1;
-    while ((1u << rounds) < nsplit && rounds < 31) ++rounds;
-    for (int r = 0; r <= rounds; ++r) {
-        hipLaunchKernelGGL(k_ib_jump, dim3(gs), dim3(kIThreads), 0, s, na, da, nb_, db, nsplit, flag_);
+        sb_host_[nblk] = acc;
+        QZ_HIP(hipMemcpyAsync(tiles_, td_host_.data(), (size_t)nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+        QZ_HIP(hipMemcpyAsync(sbase_, sb_host_.data(), sb_host_.size() * 4, hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    auto launch = [&](hipStream_t st) -> int {
+        hipLaunchKernelGGL(k_ib_hist, dim3(ntiles), dim3(kIThreads), 0, st, U, (const TileDesc*)tiles_, hist_);
         QZ_HIP(hipGetLastError());
-        std::swap(na, nb_);
-        std::swap(da, db);
-    }
-    hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kBuf, kIThreads)), dim3(kIThreads), 0, s, off_,
-                       sbase_, nblk, nsplit, len_, da, buf_, T);
-    QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, s, E_, off_, sbase_, nblk, nsplit, len_, da, resume_,
-                       T);
-    QZ_HIP(hipGetLastError());
-    return 0;
+        QZ_TRY((device_scan<OpAdd, false>(hist_, hist_, hbase, tmp_, st)));
+        hipLaunchKernelGGL(k_ib_rank, dim3(ntiles), dim3(kIThreads), 0, st, U, (const TileDesc*)tiles_, hist_, off_,
+                           prim_, split, E_);
+        QZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, st, off_, prim_, nblk, split, E_);
+        QZ_HIP(hipGetLastError());
+        const uint32_t gs = qz_div_up(nsplit, kIThreads);
+        const uint32_t gp = qz_div_up(nsplit, kPool);
+        hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, nxt0_,
+                           len_, buf_, resume_);
+        QZ_HIP(hipGetLastError());
+        // list ranking; the first round reads the segment lengths directly,
+        // later rounds alternate between d0_ and d1_
+        const uint32_t* din = len_;
+        uint32_t *na = nxt0_, *nb_ = nxt1_, *dout = d0_;
+        // ceil(log2(nsplit)) + 1 rounds always suffice
+        int rounds = 1;
+        while ((1u << rounds) < nsplit && rounds < 31) ++rounds;
+        for (int r = 0; r <= rounds; ++r) {
+            hipLaunchKernelGGL(k_ib_jump, dim3(gs), dim3(kIThreads), 0, st, na, din, nb_, dout, nsplit, flag_);
+            QZ_HIP(hipGetLastError());
+            std::swap(na, nb_);
+            din = dout;
+            dout = (dout == d0_) ? d1_ : d0_;
+        }
+        hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kBuf, kIThreads)), dim3(kIThreads), 0, st,
+                           off_, sbase_, nblk, nsplit, len_, din, buf_, T);
+        QZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, len_, din,
+                           resume_, T);
+        QZ_HIP(hipGetLastError());
+        return 0;
+    };
+    std::vector<uint64_t> key = {n, nblk, split, (uint64_t)(uintptr_t)U, (uint64_t)(uintptr_t)T, cap_, cap_blk_};
+    key.insert(key.end(), offs.begin(), offs.end());
+    return graph_.run(key, s, prepare, launch);
 }
 
 }  // namespace qz

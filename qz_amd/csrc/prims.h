@@ -148,6 +148,15 @@ static inline uint32_t build_tiles(const uint32_t* offs, uint32_t nblk, TileDesc
     return hbase;
 }
 
+// The tile count and histogram words build_tiles would produce, in O(nblk).
+static inline uint32_t count_tiles(const uint32_t* offs, uint32_t nblk, uint32_t* count, uint32_t radix = 256)
+{
+    uint32_t k = 0;
+    for (uint32_t b = 0; b < nblk; ++b) k += qz_div_up(offs[b + 1] - offs[b], kSortTile);
+    *count = k;
+    return k * radix;
+}
+
 static inline uint32_t max_tiles(uint64_t n, uint32_t nblk) { return (uint32_t)(n / kSortTile + nblk + 1); }
 
 template <typename K, int BITS>
