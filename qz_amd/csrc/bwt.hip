@@ -156,26 +156,55 @@ struct PackParams {
     uint32_t k0;         // symbols per 32-bit key
 };
 
+// Packed prefix key of every position: the next k0 symbol codes in radix
+// sigma+1, code 0 past the block end. Each thread builds 16 consecutive keys
+// with a sliding window (drop the leading code, append one), restarting at
+// block starts, and stores them as four 16-byte vectors.
+constexpr int kKeyItems = 16;
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           const uint32_t* __restrict__ off, uint32_t nblk,
                                                           const PackParams* __restrict__ ppd,
-                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                                                          uint32_t* __restrict__ keys)
 {
     __shared__ uint16_t code[256];
     code[threadIdx.x] = ppd->code[threadIdx.x];
     const uint32_t radix = ppd->radix, k0 = ppd->k0;
     __syncthreads();
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t end = off[find_block(off, nblk, i) + 1];
-    uint32_t key = 0;
-    for (uint32_t j = 0; j < k0; ++j) {
-        const uint32_t p = i + j;
-        const uint32_t c = (p < end) ? code[text[p]] : 0u;
-        key = key * radix + c;
+    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kKeyItems;
+    if (i0 >= n) return;
+    uint32_t top = 1;  // weight of the leading code
+    for (uint32_t j = 1; j < k0; ++j) top *= radix;
+    uint32_t end = 0, key = 0;
+    uint32_t out[kKeyItems];
+#pragma unroll
+    for (int q = 0; q < kKeyItems; ++q) {
+        const uint32_t i = i0 + q;
+        if (i < n) {
+            if (i >= end) {  // first position or a new block: full key
+                end = off[find_block(off, nblk, i) + 1];
+                key = 0;
+                for (uint32_t j = 0; j < k0; ++j) {
+                    const uint32_t p = i + j;
+                    key = key * radix + ((p < end) ? code[text[p]] : 0u);
+                }
+            } else {
+                const uint32_t lead = code[text[i - 1]];
+                const uint32_t p = i + k0 - 1;
+                key = (key - lead * top) * radix + ((p < end) ? code[text[p]] : 0u);
+            }
+        }
+        out[q] = key;
     }
-    keys[i] = key;
-    vals[i] = i;
+    if (i0 + kKeyItems <= n) {
+        uint4* dst = reinterpret_cast<uint4*>(keys + i0);
+#pragma unroll
+        for (int v = 0; v < kKeyItems / 4; ++v)
+            dst[v] = make_uint4(out[4 * v], out[4 * v + 1], out[4 * v + 2], out[4 * v + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kKeyItems; ++q)
+            if (i0 + q < n) keys[i0 + q] = out[q];
+    }
 }
 
 // Symbol packing from the byte histogram (one workgroup of 256): present
@@ -343,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
         }
         if (valid) {
             sa[start + e] = val;
-            rank[val] = start + hp;
+            if (hp) rank[val] = start + hp;  // hp == 0: still the group's start
             const bool end = (e + 1 == size) || knext != key;
             if (end && e > hp) stage_push(S, start + hp, e - hp + 1);
         }
@@ -617,7 +646,7 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
         const uint32_t hp = H[i];
         const uint32_t v = Vf[i];
         sa[g.start + i] = v;
-        rank[v] = g.start + hp;
+        if (hp) rank[v] = g.start + hp;  // hp == 0: still the group's start
         const bool end = (i + 1 == g.size) || Kf[i + 1] != Kf[i];
         if (end && i > hp) stagep_push(S, E, g.start + hp, i - hp + 1);
     }
@@ -678,7 +707,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
         const uint32_t s = vals[j];
         const uint32_t hs = g0 + (gstart[j] - b0);  // slot of the subgroup head
         sa[slot] = s;
-        rank[s] = hs;
+        if (hs != g0) rank[s] = hs;  // the first subgroup keeps the group's start
         const bool end = (j + 1 == b1) || keys[j + 1] != k;
         if (end && slot > hs) stage_push(S, hs, slot - hs + 1);
     }
@@ -785,10 +814,12 @@ int tune_state(int which, int* choice, double* ns8, double* ns11)
 
 template <typename K>
 static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
-                         uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* alt)
+                         uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* alt,
+                         bool iota_vals = false)
 {
-    if (bits == 11) return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt);
-    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt);
+    if (bits == 11)
+        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt, iota_vals);
+    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt, iota_vals);
 }
 
 static int bitlen(uint64_t v)
@@ -917,13 +948,13 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_);
     QZ_HIP(hipGetLastError());
     const uint32_t g = qz_div_up(n, kThreads);
-    hipLaunchKernelGGL(k_round0_keys, dim3(g), dim3(kThreads), 0, s, text, n, off_, nblk, (const PackParams*)ppd_,
-                       keys0_, vals0_);
+    hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, off_,
+                       nblk, (const PackParams*)ppd_, keys0_);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
     QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
-                                   32, tmp_, s, &alt));
+                                   32, tmp_, s, &alt, true));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;

@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
         const uint32_t old = wcnt[wave][d];
         if (valid && below == 0) wcnt[wave][d] = old + (uint32_t)__popcll(peer);
         kk[it] = k;
-        vv[it] = valid ? vin[td.start + e] : 0u;
+        vv[it] = valid ? (vin ? vin[td.start + e] : td.start + e) : 0u;  // no vin: values are positions
         rk[it] = valid ? (old + below) : 0xffffffffu;
     }
     __syncthreads();
@@ -292,7 +292,8 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 // hist_words + scan scratch. Result in (k0,v0) for even pass counts.
 template <typename K, int BITS>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
-                          uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* in_alt)
+                          uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* in_alt,
+                          bool iota_vals = false)
 {
     *in_alt = false;
     uint32_t* hist = tmp;
@@ -304,7 +305,9 @@ static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDe
         hipLaunchKernelGGL((k_seg_hist<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, tiles, sh, hist);
         QZ_HIP(hipGetLastError());
         QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
-        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, va, kb, vb, tiles,
+        // iota_vals: the first pass takes each key's position as its value
+        const uint32_t* vin = (iota_vals && passes == 0) ? nullptr : va;
+        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, vin, kb, vb, tiles,
                            sh, hist);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
