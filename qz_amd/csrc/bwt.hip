@@ -855,6 +855,7 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 {
     if (n <= cap_ && nblk <= cap_blk_) return 0;
+    const MemTrace mt("forward", n, nblk);
     // grow geometrically so a stream of slowly growing batches reallocates
     // only a few times
     const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(n, 1024), cap_ ? cap_ + cap_ / 2 : 0);

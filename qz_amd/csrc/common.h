@@ -1,8 +1,10 @@
 // Shared helpers for the gfx950 block-sorting kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 
 #define QZ_WAVE 64
 
@@ -54,3 +56,24 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
+
+namespace qz {
+// QZBSC_TRACE_MEM=1 prints every workspace reallocation and its duration.
+struct MemTrace {
+    const char* what;
+    unsigned long long n;
+    unsigned nblk;
+    std::chrono::steady_clock::time_point t0;
+    bool on;
+    MemTrace(const char* w, unsigned long long n_, unsigned nb)
+        : what(w), n(n_), nblk(nb), t0(std::chrono::steady_clock::now()), on(getenv("QZBSC_TRACE_MEM") != nullptr)
+    {
+    }
+    ~MemTrace()
+    {
+        if (!on) return;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fprintf(stderr, "qzbsc: %s workspace regrown for %llu bytes, %u blocks in %.1f ms\n", what, n, nblk, ms);
+    }
+};
+}  // namespace qz

@@ -61,10 +61,18 @@ uint64_t job_footprint(int kind, uint64_t bytes, uint32_t blocks)
 
 class Server {
 public:
+    // Starts the workers and returns once every one holds its resident
+    // forward and inverse workspaces for a full batch (init_rc() != 0 if
+    // any could not).
     Server(uint64_t budget, uint64_t max_batch, int workers) : budget_(budget), max_batch_(max_batch)
     {
+        trace_ = getenv("QZBSC_TRACE_SERVE") != nullptr;
         for (int w = 0; w < workers; ++w) threads_.emplace_back([this] { work(); });
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return ready_ == workers; });
     }
+    int init_rc() const { return init_rc_; }
+    static uint32_t batch_blocks(uint64_t bytes) { return (uint32_t)(bytes / 65536 + 64); }
     ~Server()
     {
         {
@@ -108,6 +116,15 @@ private:
     void work()
     {
         SlotGuard g;
+        {
+            int rc = g.ok() ? 0 : QZ_GPU_ERROR;
+            if (rc == 0) rc = g.slot().fwd.reserve(max_batch_, batch_blocks(max_batch_));
+            if (rc == 0) rc = g.slot().inv.reserve(max_batch_, batch_blocks(max_batch_));
+            std::lock_guard<std::mutex> lk(mu_);
+            if (rc) init_rc_ = rc;
+            ++ready_;
+        }
+        cv_done_.notify_all();
         for (;;) {
             std::vector<Job*> batch;
             uint64_t fp = 0;
@@ -151,6 +168,12 @@ private:
             }
             const double t0 = now_ms();
             int rc = g.ok() ? run_batch(g.slot(), batch) : QZ_GPU_ERROR;
+            if (trace_) {
+                uint64_t nb = 0;
+                for (Job* j : batch) nb += j->bytes;
+                fprintf(stderr, "qzbsc: worker batch kind %d jobs %zu bytes %llu in %.1f ms (queue wait of the first %.1f ms)\n",
+                        batch[0]->kind, batch.size(), (unsigned long long)nb, now_ms() - t0, t0 - batch[0]->t_submit);
+            }
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 inflight_ -= fp;
@@ -222,6 +245,8 @@ private:
     std::unordered_map<int, std::unique_ptr<Job>> jobs_;
     std::vector<std::thread> threads_;
     uint64_t budget_, max_batch_;
+    bool trace_ = false;
+    int ready_ = 0, init_rc_ = 0;
     uint64_t inflight_ = 0, peak_ = 0;
     int next_id_ = 0;
     bool stop_ = false;
@@ -238,15 +263,21 @@ QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_by
     if (hbm_budget <= 0 || max_batch_bytes <= 0 || workers < 1 || workers > 16) return nullptr;
     if (!pool_device_ok()) return nullptr;
     pool_set_limit(std::max(4, workers + 2));
-    // every worker keeps its workspace resident at its largest batch (grown
-    // by at most 1.5x): cap the batch so that all workers together stay
-    // inside the budget
+    // every worker keeps a forward and an inverse workspace for a full batch
+    // resident from the start: cap the batch so that all of them together
+    // stay inside the budget
     uint64_t cap = (uint64_t)max_batch_bytes;
-    while (cap > (1u << 20) &&
-           (uint64_t)workers * 3 / 2 * std::max(BwtEngine::footprint(cap, 64), IbwtEngine::footprint(cap, 64)) >
-               (uint64_t)hbm_budget)
-        cap -= cap / 8;
-    return new Server((uint64_t)hbm_budget, cap, workers);
+    auto resident = [&](uint64_t c) {
+        const uint32_t nb = Server::batch_blocks(c);
+        return (uint64_t)workers * (BwtEngine::footprint(c, nb) + IbwtEngine::footprint(c, nb));
+    };
+    while (cap > (1u << 20) && resident(cap) > (uint64_t)hbm_budget) cap -= cap / 8;
+    Server* s = new Server((uint64_t)hbm_budget, cap, workers);
+    if (s->init_rc() != 0) {
+        delete s;
+        return nullptr;
+    }
+    return s;
 }
 
 QZBSC_API void qzbsc_server_destroy(void* srv) { delete (Server*)srv; }
