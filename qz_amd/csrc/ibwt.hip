@@ -15,7 +15,7 @@
 //      one exclusive scan laid out [block][symbol][tile], then a stable
 //      wave-ballot ranking pass that writes one 64-bit entry per row:
 //      LF (bits 0..31) | symbol (32..39) | splitter flag (40).
-//   2. every S-th row is a splitter; each splitter walks LF until the next
+//   2. every S-th row (S = 64 by default) is a splitter; each splitter walks LF until the next
 //      splitter, recording the segment length and successor.
 //   3. list ranking of the splitter lists by pointer jumping gives each
 //      segment its distance from the chain start.
@@ -131,7 +131,9 @@ __device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ 
 // soon as their current segment ends instead of idling until the longest
 // lane of the wave is done.
 constexpr uint32_t kPool = 1024;  // splitters per workgroup
-constexpr uint32_t kBuf = 64;      // symbols a segment keeps from the first walk
+constexpr uint32_t kMinSplit = 64; // smallest splitter spacing (sizes the buffers)
+constexpr uint32_t kBuf = 128;     // symbols a segment keeps from the first walk (2 B per row)
+constexpr uint32_t kChunks = kBuf / 8;
 
 // Phase 2 also keeps the first kBuf symbols of every segment (in walk order)
 // and, for longer segments, the row reached after kBuf steps.
@@ -213,8 +215,9 @@ __global__ __launch_bounds__(kIThreads) void k_ib_jump(const uint32_t* __restric
     if (n2 != kNil) *any = 1u;
 }
 
-// Phase 4a: place the buffered symbols, one wave per splitter: step k of a
-// segment starting at chain step t0 writes T[n-1-t0-k].
+// Phase 4a: place the buffered symbols, eight per thread: step k of a
+// segment starting at chain step t0 writes T[n-1-t0-k]; thread q of a
+// splitter takes steps 8q .. 8q+7 from one 8-byte word of the buffer.
 __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ sbase, uint32_t nblk,
                                                         uint32_t nsplit, const uint32_t* __restrict__ len,
@@ -222,14 +225,20 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
                                                         const uint8_t* __restrict__ buf, uint8_t* __restrict__ T)
 {
     const uint64_t gid = (uint64_t)blockIdx.x * kIThreads + threadIdx.x;
-    const uint32_t sid = (uint32_t)(gid / kBuf), k = (uint32_t)(gid % kBuf);
+    const uint32_t sid = (uint32_t)(gid / kChunks), q = (uint32_t)(gid % kChunks);
     if (sid >= nsplit) return;
     const uint32_t l = len[sid];
-    if (k >= l) return;
+    const uint32_t k0 = q * 8;
+    if (k0 >= l) return;
     const uint32_t b = find_block_u32(sbase, nblk, sid);
     const uint32_t ob = off[b], n = off[b + 1] - ob;
-    const uint32_t t = n + 1u - dist[sid] + k;
-    if (t < n) T[ob + n - 1u - t] = buf[(size_t)sid * kBuf + k];
+    const uint32_t t0 = n + 1u - dist[sid] + k0;
+    const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
+    const uint32_t m = l - k0 < 8u ? l - k0 : 8u;
+    for (uint32_t j = 0; j < m; ++j) {
+        const uint32_t t = t0 + j;
+        if (t < n) T[ob + n - 1u - t] = (uint8_t)(w >> (8 * j));
+    }
 }
 
 // Phase 4b: segments longer than kBuf walk on from their resume row.
@@ -287,7 +296,7 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / 32 + cb + 1;
+    const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
     return 2 * (c + 64) + rows * 8 + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
            tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + nsplit * kBuf + 64;
 }
@@ -303,7 +312,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     release();
     const uint64_t rows = c + cb;
     const uint64_t tiles = c / kSortTile + cb + 1;
-    const uint64_t nsplit = rows / 32 + cb + 1;  // smallest split step supported is 32
+    const uint64_t nsplit = rows / kMinSplit + cb + 1;
     QZ_HIP(hipMalloc(&U_, c + 64));
     QZ_HIP(hipMalloc(&T_, c + 64));
     QZ_HIP(hipMalloc(&E_, rows * 8));
@@ -353,7 +362,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     uint32_t split = split_;
     if (const char* e = getenv("QZBSC_SPLIT")) {
         const long v = strtol(e, nullptr, 10);
-        if (v >= 32 && v <= 4096) split = (uint32_t)v;
+        if (v >= kMinSplit && v <= 4096) split = (uint32_t)v;
     }
     // block-aligned tiles and per-block splitter bases (sizes only; the
     // descriptors are built when the graph is not replayed)
@@ -406,7 +415,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
             din = dout;
             dout = (dout == d0_) ? d1_ : d0_;
         }
-        hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kBuf, kIThreads)), dim3(kIThreads), 0, st,
+        hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kChunks, kIThreads)), dim3(kIThreads), 0, st,
                            off_, sbase_, nblk, nsplit, len_, din, buf_, T);
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, len_, din,
