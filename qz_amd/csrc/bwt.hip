@@ -57,14 +57,40 @@ struct GroupLists {
     uint32_t* overflow;   // set if a list would overflow (cannot happen by construction)
 };
 
-__device__ __forceinline__ uint32_t find_block(const uint32_t* __restrict__ off, uint32_t nblk, uint32_t i)
-{
-    uint32_t lo = 0, hi = nblk;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= i) lo = mid; else hi = mid;
+// Block of a batch position: off[] holds the block offsets; tab[k] is the
+// block holding position k << kTabShift (tab has one entry more than there
+// are buckets, the last one nblk - 1), so a lookup searches only between the
+// blocks of its own and of the next bucket: usually one or two candidates.
+constexpr int kTabShift = 12;
+struct BlockMap {
+    const uint32_t* off;
+    const uint32_t* tab;
+    uint32_t nblk;
+    __device__ __forceinline__ uint32_t find(uint32_t i) const
+    {
+        const uint32_t k = i >> kTabShift;
+        uint32_t lo = tab[k], hi = tab[k + 1] + 1;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= i) lo = mid; else hi = mid;
+        }
+        return lo;
     }
-    return lo;
+};
+
+// Host side of BlockMap::find: tab[k] = last block b with off[b] <= k << kTabShift.
+static void build_block_tab(const std::vector<uint32_t>& offs, uint32_t n, std::vector<uint32_t>& tab)
+{
+    const uint32_t nblk = (uint32_t)offs.size() - 1;
+    const uint32_t nb = (n >> kTabShift) + 1;
+    tab.resize((size_t)nb + 1);
+    uint32_t b = 0;
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint64_t p = (uint64_t)k << kTabShift;
+        while (b + 1 < nblk && offs[b + 1] <= p) ++b;
+        tab[k] = b;
+    }
+    tab[nb] = nblk - 1;
 }
 
 __device__ __forceinline__ int grp_class(uint32_t size)
@@ -128,10 +154,10 @@ __device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t si
 // Whole workgroup; ends with the staged groups in the device lists.
 __device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { stage_flush_t(S, gl, kStage); }
 
-__device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off,
-                                              uint32_t nblk, uint32_t s, uint32_t h)
+__device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const BlockMap& bm, uint32_t s,
+                                              uint32_t h)
 {
-    const uint32_t end = off[find_block(off, nblk, s) + 1];
+    const uint32_t end = bm.off[bm.find(s) + 1];
     const uint64_t p = (uint64_t)s + h;
     return (p < end) ? rank[(uint32_t)p] + 1u : 0u;
 }
@@ -162,7 +188,7 @@ struct PackParams {
 // block starts, and stores them as four 16-byte vectors.
 constexpr int kKeyItems = 16;
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
-                                                          const uint32_t* __restrict__ off, uint32_t nblk,
+                                                          BlockMap bm,
                                                           const PackParams* __restrict__ ppd,
                                                           uint32_t* __restrict__ keys)
 {
@@ -181,7 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         const uint32_t i = i0 + q;
         if (i < n) {
             if (i >= end) {  // first position or a new block: full key
-                end = off[find_block(off, nblk, i) + 1];
+                end = bm.off[bm.find(i) + 1];
                 key = 0;
                 for (uint32_t j = 0; j < k0; ++j) {
                     const uint32_t p = i + j;
@@ -244,12 +270,12 @@ __global__ __launch_bounds__(kThreads) void k_zero_u32(uint32_t* __restrict__ p,
 
 // hv[j] = j at group heads (key change or block start), 0 elsewhere.
 __global__ __launch_bounds__(kThreads) void k_heads0(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     const uint32_t* __restrict__ off, uint32_t nblk,
+                                                     BlockMap bm,
                                                      uint32_t* __restrict__ hv)
 {
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
     if (j >= n) return;
-    const bool head = (j == 0) || keys[j] != keys[j - 1] || off[find_block(off, nblk, j)] == j;
+    const bool head = (j == 0) || keys[j] != keys[j - 1] || bm.off[bm.find(j)] == j;
     hv[j] = head ? j : 0u;
 }
 
@@ -258,7 +284,7 @@ constexpr int kFinItems = 16;  // suffixes per thread in the finishing kernels
 // Round-0 finish: SA, ranks, and the unsorted groups.
 __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ vals, uint32_t n,
-                                                     const uint32_t* __restrict__ off, uint32_t nblk,
+                                                     BlockMap bm,
                                                      const uint32_t* __restrict__ gstart, uint32_t* __restrict__ sa,
                                                      uint32_t* __restrict__ rank, GroupLists gl)
 {
@@ -274,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
         const uint32_t g = gstart[j];
         sa[j] = s;
         rank[s] = g;
-        const bool end = (j + 1 == n) || keys[j + 1] != k || off[find_block(off, nblk, j) + 1] == j + 1;
+        const bool end = (j + 1 == n) || keys[j + 1] != k || bm.off[bm.find(j) + 1] == j + 1;
         if (end && j > g) stage_push(S, g, j - g + 1);
     }
     stage_flush(S, gl);
@@ -311,7 +337,7 @@ struct CurLists {
 __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPlan plan,
                                                            const uint32_t* __restrict__ sa,
                                                            const uint32_t* __restrict__ rank,
-                                                           const uint32_t* __restrict__ off, uint32_t nblk,
+                                                           BlockMap bm,
                                                            uint32_t h, uint32_t* __restrict__ key2)
 {
     const uint32_t gwave = blockIdx.x * (kThreads / QZ_WAVE) + threadIdx.x / QZ_WAVE;
@@ -321,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPla
     const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
     if (valid) {
         const uint32_t slot = start + e;
-        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], h);
     }
 }
 
@@ -386,28 +412,28 @@ __global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__
                                                          const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         BlockMap bm, uint32_t h,
                                                          uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < g.size; i += QZ_WAVE) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], h);
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         BlockMap bm, uint32_t h,
                                                          uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < g.size; i += kThreads) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, off, nblk, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], h);
     }
 }
 
@@ -661,7 +687,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
                                                          const uint32_t* __restrict__ lbase,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         const uint32_t* __restrict__ off, uint32_t nblk, uint32_t h,
+                                                         BlockMap bm, uint32_t h,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     const TileDesc td = tiles[blockIdx.x];
@@ -669,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
         const uint32_t j = td.start + i;
         const uint32_t s = sa[g0 + (j - b0)];
-        keys[j] = round_key(rank, off, nblk, s, h);
+        keys[j] = round_key(rank, bm, s, h);
         vals[j] = s;
     }
 }
@@ -719,13 +745,13 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
 __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict__ text,
                                                       const uint32_t* __restrict__ sa,
                                                       const uint32_t* __restrict__ rank,
-                                                      const uint32_t* __restrict__ off, uint32_t nblk,
+                                                      BlockMap bm,
                                                       uint32_t n, uint8_t* __restrict__ out)
 {
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
     if (j >= n) return;
-    const uint32_t b = find_block(off, nblk, j);
-    const uint32_t o = off[b], e = off[b + 1];
+    const uint32_t b = bm.find(j);
+    const uint32_t o = bm.off[b], e = bm.off[b + 1];
     const uint32_t local = j - o;
     const uint32_t p = rank[o] - o + 1u;
     uint8_t c;
@@ -735,12 +761,12 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict_
 }
 
 // Per block: primary index and sampled indexes I[k] = 1 + rank(k*r).
-__global__ void k_bwt_index(const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint32_t nblk,
+__global__ void k_bwt_index(const uint32_t* __restrict__ rank, BlockMap bm,
                             int32_t* __restrict__ idx /* nblk x 256 */)
 {
     const uint32_t b = blockIdx.x;
-    if (b >= nblk) return;
-    const uint32_t o = off[b], n = off[b + 1] - o;
+    if (b >= bm.nblk) return;
+    const uint32_t o = bm.off[b], n = bm.off[b + 1] - o;
     uint32_t r = n / 8;
     r |= r >> 1; r |= r >> 2; r |= r >> 4; r |= r >> 8; r |= r >> 16;
     r = (r >> 1) + 1;
@@ -849,7 +875,7 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
     return 2 * (c + 64) + 8 * 4 * c + tmpw * 4 + (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
            2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
-           sizeof(PackParams);
+           sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
 }
 
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -889,6 +915,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
+    QZ_HIP(hipMalloc(&btab_, ((c >> kTabShift) + 2) * 4));
     tiles_host_.resize(tile_cap);
     QZ_HIP(hipHostMalloc(&pinned_, 4096));
     for (auto& ev : tev_)
@@ -901,7 +928,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 void BwtEngine::release()
 {
     void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, tmp_, off_, idx_, hist_,
-                  cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_};
+                  cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -911,6 +938,7 @@ void BwtEngine::release()
     keys0_ = keys1_ = nullptr;
     groups_[0] = groups_[1] = nullptr;
     tiles_ = tiles0_ = ppd_ = nullptr;
+    btab_ = nullptr;
     idx_ = nullptr;
     pinned_ = nullptr;
     cap_ = 0;
@@ -940,6 +968,7 @@ static GroupLists make_lists(void* base, uint32_t* cnt, uint32_t* overflow, uint
 int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
                               uint32_t hist_words, bool timed, hipStream_t s)
 {
+    const BlockMap bmap_{off_, btab_, nblk};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, 64u);
     QZ_HIP(hipGetLastError());
@@ -949,8 +978,8 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_);
     QZ_HIP(hipGetLastError());
     const uint32_t g = qz_div_up(n, kThreads);
-    hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, off_,
-                       nblk, (const PackParams*)ppd_, keys0_);
+    hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
+                        (const PackParams*)ppd_, keys0_);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
@@ -959,13 +988,13 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;
-    hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, off_, nblk, bufA_);
+    hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, bmap_, bufA_);
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, n, tmp_, s)));
     uint32_t* overflow = cnt_ + 2 * kClasses;
     const GroupLists gl = make_lists(groups_[0], cnt_, overflow, cap_);
-    hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n, off_,
-                       nblk, bufA_, sa_, rank_, gl);
+    hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n, bmap_,
+                        bufA_, sa_, rank_, gl);
     QZ_HIP(hipGetLastError());
     return 0;
 }
@@ -973,6 +1002,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
 // Sorts the suffixes of the batch in text (device); off_ holds the offsets.
 int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s)
 {
+    const BlockMap bmap_{off_, btab_, nblk};
     rounds_ = 0;
     trace_ = getenv("QZBSC_TRACE") != nullptr;
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
@@ -1109,23 +1139,23 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
 
         // phase A: gather every key of the round before any rank changes
         if (small_waves) {
-            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), 0, s, cl, plan, sa_, rank_, off_,
-                               nblk, h, bufB_);
+            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), 0, s, cl, plan, sa_, rank_, bmap_,
+                                h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
             hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave,
-                               sa_, rank_, off_, nblk, h, bufB_);
+                               sa_, rank_, bmap_, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
             hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, sa_,
-                               rank_, off_, nblk, h, bufB_);
+                               rank_, bmap_, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
             hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, sa_, rank_,
-                               off_, nblk, h, lk0, vals0_);
+                               bmap_, h, lk0, vals0_);
             QZ_HIP(hipGetLastError());
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
@@ -1169,11 +1199,15 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
 
 int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint32_t nblk, hipStream_t s)
 {
+    const BlockMap bmap_{off_, btab_, nblk};
+    if (offs_host_.size() != (size_t)nblk + 1 || offs_host_.back() != n) return QZ_BAD_PARAMETER;
+    build_block_tab(offs_host_, n, btab_host_);
+    QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
     QZ_TRY(sort_suffixes(text, n, nblk, s));
-    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), 0, s, text, sa_, rank_, off_, nblk, n,
+    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), 0, s, text, sa_, rank_, bmap_, n,
                        out);
     QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, off_, nblk, idx_);
+    hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
     QZ_HIP(hipGetLastError());
     return 0;
 }

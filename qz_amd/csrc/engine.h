@@ -36,6 +36,8 @@ private:
                        uint32_t hist_words, bool timed, hipStream_t s);
     void* tiles0_ = nullptr;       // round-0 radix tiles (device; read by the round-0 graph)
     void* ppd_ = nullptr;          // symbol packing of round 0 (device)
+    uint32_t* btab_ = nullptr;     // position bucket -> block (device), see BlockMap
+    std::vector<uint32_t> btab_host_;
     GraphCache g0_;                // round 0 as one graph
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
