@@ -268,40 +268,89 @@ __global__ __launch_bounds__(kThreads) void k_zero_u32(uint32_t* __restrict__ p,
     for (uint32_t i = threadIdx.x; i < n; i += kThreads) p[i] = 0;
 }
 
-// hv[j] = j at group heads (key change or block start), 0 elsewhere.
-__global__ __launch_bounds__(kThreads) void k_heads0(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     BlockMap bm,
-                                                     uint32_t* __restrict__ hv)
+
+// Round-0 group heads, tile by tile over the sorted radix tiles (4096 slots,
+// never spanning two blocks). A slot is a head at its block's start or where
+// the key changes. lh[t] = the last head slot of tile t (0 if none: max-scan
+// identity, slots being >= 0).
+__device__ __forceinline__ bool slot_head(const uint32_t* __restrict__ keys, const TileDesc& td, uint32_t i)
 {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= n) return;
-    const bool head = (j == 0) || keys[j] != keys[j - 1] || bm.off[bm.find(j)] == j;
-    hv[j] = head ? j : 0u;
+    const uint32_t j = td.start + i;
+    return (i == 0 && td.tlocal == 0) || keys[j] != keys[j - 1];
 }
 
-constexpr int kFinItems = 16;  // suffixes per thread in the finishing kernels
+__global__ __launch_bounds__(kThreads) void k_tile_lasthead(const uint32_t* __restrict__ keys,
+                                                            const TileDesc* __restrict__ tiles,
+                                                            uint32_t* __restrict__ lh)
+{
+    __shared__ uint32_t wmax[kThreads / QZ_WAVE];
+    const TileDesc td = tiles[blockIdx.x];
+    uint32_t m = 0;
+    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads)
+        if (slot_head(keys, td, i)) m = td.start + i;  // i grows: the last head wins
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0) wmax[threadIdx.x / QZ_WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0;
+        for (int w = 0; w < kThreads / QZ_WAVE; ++w) r = wmax[w] > r ? wmax[w] : r;
+        lh[blockIdx.x] = r;
+    }
+}
 
-// Round-0 finish: SA, ranks, and the unsorted groups.
+// Round-0 finish, one tile per workgroup: SA, ranks and the unsorted groups.
+// pre[t] = the last head slot before tile t (exclusive max-scan of lh). Wave
+// w takes the tile's quarter [1024w, 1024w+1024) in 16 coalesced steps and
+// carries the running group start with a wave max-scan; its starting value
+// comes from pre[t] and the earlier quarters' last heads.
 __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ vals, uint32_t n,
-                                                     BlockMap bm,
-                                                     const uint32_t* __restrict__ gstart, uint32_t* __restrict__ sa,
+                                                     const uint32_t* __restrict__ vals,
+                                                     const TileDesc* __restrict__ tiles,
+                                                     const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                      uint32_t* __restrict__ rank, GroupLists gl)
 {
+    constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);  // 1024 slots per wave
     __shared__ Stage S;
+    __shared__ uint32_t qlast[kThreads / QZ_WAVE];
     stage_init(S);
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    const uint32_t qb = w * Q;
+    uint32_t m = 0;
+    for (uint32_t i = qb + lane; i < qb + Q && i < td.len; i += QZ_WAVE)
+        if (slot_head(keys, td, i)) m = td.start + i;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
+        m = o > m ? o : m;
+    }
+    if (lane == 0) qlast[w] = m;
     __syncthreads();
-    const uint32_t base = blockIdx.x * (kThreads * kFinItems);
-    for (int it = 0; it < kFinItems; ++it) {
-        const uint32_t j = base + it * kThreads + threadIdx.x;
-        if (j >= n) break;
-        const uint32_t k = keys[j];
-        const uint32_t s = vals[j];
-        const uint32_t g = gstart[j];
-        sa[j] = s;
-        rank[s] = g;
-        const bool end = (j + 1 == n) || keys[j + 1] != k || bm.off[bm.find(j) + 1] == j + 1;
-        if (end && j > g) stage_push(S, g, j - g + 1);
+    uint32_t carry = pre[blockIdx.x];
+    for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
+    const bool last_tile = td.tlocal + 1 == td.ntiles;
+    for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
+        const uint32_t i = b + lane;
+        const bool valid = i < td.len;
+        const uint32_t j = td.start + i;
+        const uint32_t k = valid ? keys[j] : 0u;
+        const bool head = valid && slot_head(keys, td, i);
+        uint32_t g = head ? j : 0u;
+        for (int d = 1; d < QZ_WAVE; d <<= 1) {
+            const uint32_t o = __shfl_up(g, d, QZ_WAVE);
+            if (lane >= (uint32_t)d && o > g) g = o;
+        }
+        if (carry > g) g = carry;
+        carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
+        if (valid) {
+            const uint32_t s = vals[j];
+            sa[j] = s;
+            rank[s] = g;
+            const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
+            if (end && j > g) stage_push(S, g, j - g + 1);
+        }
     }
     stage_flush(S, gl);
 }
@@ -988,13 +1037,15 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;
-    hipLaunchKernelGGL(k_heads0, dim3(g), dim3(kThreads), 0, s, keys, n, bmap_, bufA_);
+    // group starts: last head per tile, an exclusive max-scan over tiles,
+    // then tile-local scans inside the finishing kernel
+    hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
     QZ_HIP(hipGetLastError());
-    QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, n, tmp_, s)));
+    QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
     uint32_t* overflow = cnt_ + 2 * kClasses;
     const GroupLists gl = make_lists(groups_[0], cnt_, overflow, cap_);
-    hipLaunchKernelGGL(k_final0, dim3(qz_div_up(n, kThreads * kFinItems)), dim3(kThreads), 0, s, keys, vals, n, bmap_,
-                        bufA_, sa_, rank_, gl);
+    hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, (const TileDesc*)tiles0_, bufA_, sa_,
+                       rank_, gl);
     QZ_HIP(hipGetLastError());
     return 0;
 }
