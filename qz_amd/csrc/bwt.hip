@@ -749,42 +749,60 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_heads_large(const TileDesc* __restrict__ tiles,
-                                                          const uint32_t* __restrict__ lbase,
-                                                          const uint32_t* __restrict__ keys, uint32_t* __restrict__ hv)
-{
-    const TileDesc td = tiles[blockIdx.x];
-    const uint32_t b0 = lbase[td.block];
-    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
-        const uint32_t j = td.start + i;
-        const bool head = (j == b0) || keys[j] != keys[j - 1];
-        hv[j] = head ? j : 0u;
-    }
-}
-
+// Large-group finish, one radix tile per workgroup (tiles never span two
+// groups; td.block is the group's index in the list). Same scheme as
+// k_final0 over the sorted keys array, whose index j maps to SA slot
+// g0 + (j - b0) of group g0 = list[td.block].start, b0 = lbase[td.block].
 __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __restrict__ tiles,
                                                           const Grp* __restrict__ list,
                                                           const uint32_t* __restrict__ lbase,
                                                           const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ vals,
-                                                          const uint32_t* __restrict__ gstart, uint32_t* __restrict__ sa,
+                                                          const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                           uint32_t* __restrict__ rank, GroupLists gl)
 {
+    constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);
     __shared__ Stage S;
+    __shared__ uint32_t qlast[kThreads / QZ_WAVE];
     stage_init(S);
-    __syncthreads();
     const TileDesc td = tiles[blockIdx.x];
-    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block], b1 = lbase[td.block + 1];
-    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
+    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    const uint32_t qb = w * Q;
+    uint32_t m = 0;
+    for (uint32_t i = qb + lane; i < qb + Q && i < td.len; i += QZ_WAVE)
+        if (slot_head(keys, td, i)) m = td.start + i;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
+        m = o > m ? o : m;
+    }
+    if (lane == 0) qlast[w] = m;
+    __syncthreads();
+    uint32_t carry = pre[blockIdx.x];
+    for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
+    const bool last_tile = td.tlocal + 1 == td.ntiles;
+    for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
+        const uint32_t i = b + lane;
+        const bool valid = i < td.len;
         const uint32_t j = td.start + i;
-        const uint32_t k = keys[j];
-        const uint32_t slot = g0 + (j - b0);
-        const uint32_t s = vals[j];
-        const uint32_t hs = g0 + (gstart[j] - b0);  // slot of the subgroup head
-        sa[slot] = s;
-        if (hs != g0) rank[s] = hs;  // the first subgroup keeps the group's start
-        const bool end = (j + 1 == b1) || keys[j + 1] != k;
-        if (end && slot > hs) stage_push(S, hs, slot - hs + 1);
+        const uint32_t k = valid ? keys[j] : 0u;
+        const bool head = valid && slot_head(keys, td, i);
+        uint32_t g = head ? j : 0u;
+        for (int d = 1; d < QZ_WAVE; d <<= 1) {
+            const uint32_t o = __shfl_up(g, d, QZ_WAVE);
+            if (lane >= (uint32_t)d && o > g) g = o;
+        }
+        if (carry > g) g = carry;
+        carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
+        if (valid) {
+            const uint32_t s = vals[j];
+            const uint32_t slot = g0 + (j - b0);
+            const uint32_t hs = g0 + (g - b0);  // slot of the subgroup head
+            sa[slot] = s;
+            if (hs != g0) rank[s] = hs;  // the first subgroup keeps the group's start
+            const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
+            if (end && slot > hs) stage_push(S, hs, slot - hs + 1);
+        }
     }
     stage_flush(S, gl);
 }
@@ -1234,9 +1252,9 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             large_n = M;
             uint32_t* keys = alt ? lk1 : lk0;
             uint32_t* vals = alt ? vals1_ : vals0_;
-            hipLaunchKernelGGL(k_heads_large, dim3(lt), dim3(kThreads), 0, s, ltiles, lbase_, keys, bufA_);
+            hipLaunchKernelGGL(k_tile_lasthead, dim3(lt), dim3(kThreads), 0, s, keys, ltiles, bufA_);
             QZ_HIP(hipGetLastError());
-            QZ_TRY((device_scan<OpMax, true>(bufA_, bufA_, M, tmp_, s)));
+            QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
             hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, keys, vals,
                                bufA_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());

@@ -78,7 +78,7 @@ public:
     uint8_t* T_ = nullptr;
     uint32_t* off_ = nullptr;
     uint32_t* prim_ = nullptr;
-    uint32_t split_ = 64;
+    uint32_t split_ = 0;  // splitter spacing; 0 picks one by batch size
     uint64_t cap_ = 0;
     uint32_t cap_blk_ = 0;
 

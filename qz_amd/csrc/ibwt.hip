@@ -15,7 +15,8 @@
 //      one exclusive scan laid out [block][symbol][tile], then a stable
 //      wave-ballot ranking pass that writes one 64-bit entry per row:
 //      LF (bits 0..31) | symbol (32..39) | splitter flag (40).
-//   2. every S-th row (S = 64 by default) is a splitter; each splitter walks LF until the next
+//   2. every S-th row (S = 32, or 64 for batches of 128 MiB and more) is a
+//      splitter; each splitter walks LF until the next
 //      splitter, recording the segment length and successor.
 //   3. list ranking of the splitter lists by pointer jumping gives each
 //      segment its distance from the chain start.
@@ -130,23 +131,30 @@ __device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ 
 // geometric, so lanes pull splitters from a per-workgroup pool in LDS as
 // soon as their current segment ends instead of idling until the longest
 // lane of the wave is done.
-constexpr uint32_t kPool = 1024;  // splitters per workgroup
-constexpr uint32_t kMinSplit = 64; // smallest splitter spacing (sizes the buffers)
-constexpr uint32_t kBuf = 128;     // symbols a segment keeps from the first walk (2 B per row)
-constexpr uint32_t kChunks = kBuf / 8;
+constexpr uint32_t kMinSplit = 32;  // splitter spacing limits (sizes the per-splitter arrays)
+constexpr uint32_t kMaxSplit = 256;
+constexpr uint64_t kWideSplitBytes = 128ull << 20;  // batches from here on use spacing 64
+constexpr uint32_t kPoolMax = 512;  // splitters per walking workgroup (swept: 256..2048)
+// A segment keeps its first 2 * split symbols from the first walk, so the
+// buffer costs 2 bytes per row whatever the spacing.
+__host__ __device__ constexpr uint32_t seg_buf(uint32_t split) { return 2 * split; }
+// Buffer bytes for any spacing in [kMinSplit, kMaxSplit]: sum over blocks of
+// (rows_b / split + 1) * 2 * split <= 2 * rows + 2 * kMaxSplit * blocks.
+static inline uint64_t seg_buf_bytes(uint64_t rows, uint64_t blocks) { return 2 * rows + 2ull * kMaxSplit * blocks + 64; }
 
 // Phase 2 also keeps the first kBuf symbols of every segment (in walk order)
 // and, for longer segments, the row reached after kBuf steps.
 __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
-                                                         uint32_t nsplit, uint32_t split,
+                                                         uint32_t nsplit, uint32_t split, uint32_t pool,
                                                          uint32_t* __restrict__ nxt, uint32_t* __restrict__ len,
                                                          uint8_t* __restrict__ buf, uint32_t* __restrict__ resume)
 {
     __shared__ uint32_t next;
-    const uint32_t pbeg = blockIdx.x * kPool;
-    const uint32_t pend = min(pbeg + kPool, nsplit);
+    const uint32_t kBuf = seg_buf(split);
+    const uint32_t pbeg = blockIdx.x * pool;
+    const uint32_t pend = min(pbeg + pool, nsplit);
     if (threadIdx.x == 0) next = pbeg;
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
@@ -220,10 +228,12 @@ __global__ __launch_bounds__(kIThreads) void k_ib_jump(const uint32_t* __restric
 // splitter takes steps 8q .. 8q+7 from one 8-byte word of the buffer.
 __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ sbase, uint32_t nblk,
-                                                        uint32_t nsplit, const uint32_t* __restrict__ len,
+                                                        uint32_t nsplit, uint32_t split,
+                                                        const uint32_t* __restrict__ len,
                                                         const uint32_t* __restrict__ dist,
                                                         const uint8_t* __restrict__ buf, uint8_t* __restrict__ T)
 {
+    const uint32_t kChunks = seg_buf(split) / 8;
     const uint64_t gid = (uint64_t)blockIdx.x * kIThreads + threadIdx.x;
     const uint32_t sid = (uint32_t)(gid / kChunks), q = (uint32_t)(gid % kChunks);
     if (sid >= nsplit) return;
@@ -245,13 +255,15 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
 __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
-                                                         uint32_t nsplit, const uint32_t* __restrict__ len,
+                                                         uint32_t nsplit, uint32_t split, uint32_t pool,
+                                                         const uint32_t* __restrict__ len,
                                                          const uint32_t* __restrict__ dist,
                                                          const uint32_t* __restrict__ resume, uint8_t* __restrict__ T)
 {
     __shared__ uint32_t next;
-    const uint32_t pbeg = blockIdx.x * kPool;
-    const uint32_t pend = min(pbeg + kPool, nsplit);
+    const uint32_t kBuf = seg_buf(split);
+    const uint32_t pbeg = blockIdx.x * pool;
+    const uint32_t pend = min(pbeg + pool, nsplit);
     if (threadIdx.x == 0) next = pbeg;
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
@@ -298,7 +310,7 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
     return 2 * (c + 64) + rows * 8 + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
-           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + nsplit * kBuf + 64;
+           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + seg_buf_bytes(rows, cb) + 64;
 }
 
 int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -328,7 +340,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&d0_, nsplit * 4));
     QZ_HIP(hipMalloc(&d1_, nsplit * 4));
     QZ_HIP(hipMalloc(&flag_, 64));
-    QZ_HIP(hipMalloc(&buf_, nsplit * kBuf));
+    QZ_HIP(hipMalloc(&buf_, seg_buf_bytes(rows, cb)));
     QZ_HIP(hipMalloc(&resume_, nsplit * 4));
     QZ_HIP(hipHostMalloc(&pinned_, 64));
     cap_ = c;
@@ -359,10 +371,17 @@ void IbwtEngine::release()
 int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t nblk, const std::vector<uint32_t>& offs,
                               hipStream_t s)
 {
-    uint32_t split = split_;
+    // Splitter spacing: longer segments mean fewer splitters to rank, but a
+    // batch needs enough segments in flight to hide the walks' latency.
+    uint32_t split = split_ ? split_ : (n >= kWideSplitBytes ? 64u : 32u);
     if (const char* e = getenv("QZBSC_SPLIT")) {
         const long v = strtol(e, nullptr, 10);
-        if (v >= kMinSplit && v <= 4096) split = (uint32_t)v;
+        if (v >= kMinSplit && v <= kMaxSplit) split = (uint32_t)v;
+    }
+    uint32_t pool = kPoolMax;  // splitters per walking workgroup
+    if (const char* e = getenv("QZBSC_POOL")) {
+        const long v = strtol(e, nullptr, 10);
+        if (v >= kIThreads && v <= 4096) pool = (uint32_t)v;
     }
     // block-aligned tiles and per-block splitter bases (sizes only; the
     // descriptors are built when the graph is not replayed)
@@ -397,9 +416,9 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, st, off_, prim_, nblk, split, E_);
         QZ_HIP(hipGetLastError());
         const uint32_t gs = qz_div_up(nsplit, kIThreads);
-        const uint32_t gp = qz_div_up(nsplit, kPool);
-        hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, nxt0_,
-                           len_, buf_, resume_);
+        const uint32_t gp = qz_div_up(nsplit, pool);
+        hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, pool,
+                           nxt0_, len_, buf_, resume_);
         QZ_HIP(hipGetLastError());
         // list ranking; the first round reads the segment lengths directly,
         // later rounds alternate between d0_ and d1_
@@ -415,15 +434,15 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
             din = dout;
             dout = (dout == d0_) ? d1_ : d0_;
         }
-        hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * kChunks, kIThreads)), dim3(kIThreads), 0, st,
-                           off_, sbase_, nblk, nsplit, len_, din, buf_, T);
+        hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * (seg_buf(split) / 8), kIThreads)),
+                           dim3(kIThreads), 0, st, off_, sbase_, nblk, nsplit, split, len_, din, buf_, T);
         QZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, len_, din,
-                           resume_, T);
+        hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, pool,
+                           len_, din, resume_, T);
         QZ_HIP(hipGetLastError());
         return 0;
     };
-    std::vector<uint64_t> key = {n, nblk, split, (uint64_t)(uintptr_t)U, (uint64_t)(uintptr_t)T, cap_, cap_blk_};
+    std::vector<uint64_t> key = {n, nblk, split, pool, (uint64_t)(uintptr_t)U, (uint64_t)(uintptr_t)T, cap_, cap_blk_};
     key.insert(key.end(), offs.begin(), offs.end());
     return graph_.run(key, s, prepare, launch);
 }
