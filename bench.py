@@ -39,6 +39,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(total: int):
+    """HBM bytes per launch of the forward and inverse launch sequences, from
+    the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised under profiles/
+    (copy-calibrated), when they were taken on this workload size."""
+    p = ROOT / "profiles" / "traffic_seq58x25MiB.json"
+    try:
+        d = json.loads(p.read_text())
+    except (OSError, ValueError):
+        return None
+    if d.get("input_bytes") != total:
+        return None
+    return d
+
+
 def cpu_baseline(blocks, threads: int):
     """Oracle BWT + inverse BWT, one block per thread (qz's rayon model)."""
     from oracle import oracle as O
@@ -370,7 +384,7 @@ def main():
         "traffic": None,
         "launch_ms": round(fwd * 1e3, 3),
         "inverse": {"achieved": round(ach_inv, 2), "frac": round(ach_inv / HBM_PEAK_GBS, 5),
-                    "launch_ms": round(inv * 1e3, 3)},
+                    "launch_ms": round(inv * 1e3, 3), "traffic": None},
         "bwt_ibwt_frac": round((ALG_BYTES_FWD + ALG_BYTES_INV) * total / (fwd + inv) / 1e9 / HBM_PEAK_GBS, 5),
     }
 
@@ -378,6 +392,15 @@ def main():
     if not args.no_graph:
         graphs = graph_section(torch, bwt, synth)
         log(f"graphs: {graphs}")
+
+    tr = measured_traffic(total)
+    if tr:
+        roof["traffic"] = tr["forward"]["hbm_bytes"]
+        roof["inverse"]["traffic"] = tr["inverse"]["hbm_bytes"]
+        roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
+                                "workload (profiles/traffic_seq58x25MiB.json), scaled by a 256 MiB copy: "
+                                f"{tr['forward']['hbm_bytes_per_input_byte']} B per input byte forward, "
+                                f"{tr['inverse']['hbm_bytes_per_input_byte']} inverse")
 
     uneven = None
     if not args.no_uneven:

@@ -153,6 +153,10 @@ QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long
 
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
+/* Releases the resident workspaces of every idle engine (no reference
+ * counterpart: for long-running processes between workloads). Returns the
+ * bytes released. */
+QZBSC_API long long qzbsc_trim(void);
 /* 1 if a HIP device is usable, else 0. */
 QZBSC_API int qzbsc_gpu_available(void);
 

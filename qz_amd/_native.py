@@ -44,6 +44,7 @@ _SIGS = {
     "qzbsc_footprint": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
     "qzbsc_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "qzbsc_graph_stats": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
+    "qzbsc_trim": (C.c_longlong, []),
 }
 
 _lib = None

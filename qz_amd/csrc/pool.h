@@ -22,6 +22,8 @@ struct Slot {
 Slot* pool_acquire();
 void pool_release(Slot* s);
 int pool_set_limit(int n);
+// Releases the workspaces of every idle slot; returns their footprint in bytes.
+uint64_t pool_trim();
 int pool_device_ok();
 
 class SlotGuard {

@@ -58,6 +58,21 @@ void pool_release(Slot* s)
     g_cv.notify_one();
 }
 
+uint64_t pool_trim()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    uint64_t freed = 0;
+    for (auto& s : g_slots) {
+        if (s->busy) continue;
+        freed += s->fwd.cap_ ? BwtEngine::footprint(s->fwd.cap_, s->fwd.cap_blk_) : 0;
+        freed += s->inv.cap_ ? IbwtEngine::footprint(s->inv.cap_, s->inv.cap_blk_) : 0;
+        s->fwd.release();
+        s->inv.release();
+        s->front.release();
+    }
+    return freed;
+}
+
 int pool_set_limit(int n)
 {
     if (n < 1 || n > 64) return QZ_BAD_PARAMETER;
@@ -70,3 +85,4 @@ int pool_set_limit(int n)
 
 extern "C" QZBSC_API int qzbsc_set_max_engines(int n) { return qz::pool_set_limit(n); }
 extern "C" QZBSC_API int qzbsc_gpu_available(void) { return qz::pool_device_ok(); }
+extern "C" QZBSC_API long long qzbsc_trim(void) { return (long long)qz::pool_trim(); }

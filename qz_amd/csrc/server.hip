@@ -266,13 +266,19 @@ QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_by
     // every worker keeps a forward and an inverse workspace for a full batch
     // resident from the start: cap the batch so that all of them together
     // stay inside the budget
+    // idle engines of the pool give their workspaces back first; the budget
+    // is then also held to what the device has free
+    pool_trim();
+    uint64_t budget = (uint64_t)hbm_budget;
+    size_t dfree = 0, dtotal = 0;
+    if (hipMemGetInfo(&dfree, &dtotal) == hipSuccess && dfree) budget = std::min<uint64_t>(budget, dfree / 10 * 9);
     uint64_t cap = (uint64_t)max_batch_bytes;
     auto resident = [&](uint64_t c) {
         const uint32_t nb = Server::batch_blocks(c);
         return (uint64_t)workers * (BwtEngine::footprint(c, nb) + IbwtEngine::footprint(c, nb));
     };
-    while (cap > (1u << 20) && resident(cap) > (uint64_t)hbm_budget) cap -= cap / 8;
-    Server* s = new Server((uint64_t)hbm_budget, cap, workers);
+    while (cap > (1u << 20) && resident(cap) > budget) cap -= cap / 8;
+    Server* s = new Server(budget, cap, workers);
     if (s->init_rc() != 0) {
         delete s;
         return nullptr;
