@@ -83,7 +83,10 @@ public:
     uint32_t cap_blk_ = 0;
 
 private:
-    uint64_t* E_ = nullptr;
+    uint32_t* E_ = nullptr;        // per row: LF | splitter flag
+    uint32_t* fst_ = nullptr;      // per block: F-run start row of each present symbol
+    uint8_t* fsym_ = nullptr;      // per block: the present symbols
+    uint32_t* fsig_ = nullptr;     // per block: number of present symbols
     uint32_t* hist_ = nullptr;
     uint32_t* tmp_ = nullptr;
     void* tiles_ = nullptr;

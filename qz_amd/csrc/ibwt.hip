@@ -13,8 +13,9 @@
 // Phases:
 //   1. block-aligned tiles of 4096 symbols: per-tile symbol histograms,
 //      one exclusive scan laid out [block][symbol][tile], then a stable
-//      wave-ballot ranking pass that writes one 64-bit entry per row:
-//      LF (bits 0..31) | symbol (32..39) | splitter flag (40).
+//      wave-ballot ranking pass that writes one 32-bit entry per row:
+//      LF (bits 0..30) | splitter flag (31). The symbol a row emits, L[row],
+//      is F[LF(row)], read from a compact per-block table of the F column.
 //   2. every S-th row (S = 32, or 64 for batches of 128 MiB and more) is a
 //      splitter; each splitter walks LF until the next
 //      splitter, recording the segment length and successor.
@@ -31,7 +32,8 @@
 namespace qz {
 
 constexpr int kIThreads = 256;
-constexpr uint64_t kSplitFlag = 1ull << 40;
+constexpr uint32_t kSplitFlag = 1u << 31;  // row entry: LF in bits 0..30, splitter flag in bit 31
+constexpr uint32_t kLFMask = kSplitFlag - 1;
 constexpr uint32_t kNil = 0xffffffffu;
 
 __global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict__ U,
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
                                                         const uint32_t* __restrict__ hist,
                                                         const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ prim,
-                                                        uint32_t split, uint64_t* __restrict__ E)
+                                                        uint32_t split, uint32_t* __restrict__ E)
 {
     constexpr int W = kIThreads / QZ_WAVE;
     __shared__ uint32_t run[256];
@@ -99,9 +101,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
             const uint32_t u = td.start - ob + i;          // local BWT index
             const uint32_t row = u < p ? u : u + 1u;
             const uint32_t lf = 1u + wh[wave][c] + below;  // local row
-            uint64_t e = (uint64_t)lf | ((uint64_t)c << 32);
-            if (row % split == 0) e |= kSplitFlag;
-            E[rb + row] = e;
+            E[rb + row] = (row % split == 0) ? (lf | kSplitFlag) : lf;
         }
         __syncthreads();
     }
@@ -109,13 +109,98 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
 
 // Sentinel rows: LF(p) = 0.
 __global__ void k_ib_sentinel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ prim, uint32_t nblk,
-                              uint32_t split, uint64_t* __restrict__ E)
+                              uint32_t split, uint32_t* __restrict__ E)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblk) return;
     const uint32_t p = prim[b];
-    E[off[b] + b + p] = (p % split == 0) ? kSplitFlag : 0ull;
+    E[off[b] + b + p] = (p % split == 0) ? kSplitFlag : 0u;
 }
+
+// F column of every block, compactly: its present symbols in byte order and
+// the local row where each one's run of F starts (1 + #smaller symbols; row 0
+// belongs to the sentinel). The symbol emitted at a row is L[row] =
+// F[LF(row)], so the walks find it from the LF they already hold. One
+// workgroup per block (the block's first tile), one thread per byte value.
+__global__ __launch_bounds__(kIThreads) void k_ib_ftable(const TileDesc* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ hist,
+                                                          const uint32_t* __restrict__ off,
+                                                          uint32_t* __restrict__ fst, uint8_t* __restrict__ fsym,
+                                                          uint32_t* __restrict__ fsig)
+{
+    __shared__ uint32_t wsum[kIThreads / QZ_WAVE];
+    const TileDesc td = tiles[blockIdx.x];
+    if (td.tlocal != 0) return;  // block-uniform
+    const uint32_t c = threadIdx.x, lane = lane_id(), w = c / QZ_WAVE;
+    const uint32_t b = td.block, ob = off[b], nb = off[b + 1] - ob;
+    const uint32_t cum = hist[(size_t)td.hbase + (size_t)c * td.ntiles] - ob;
+    const uint32_t nxt = (c + 1 < 256) ? hist[(size_t)td.hbase + (size_t)(c + 1) * td.ntiles] - ob : nb;
+    const uint32_t present = nxt > cum ? 1u : 0u;
+    uint32_t incl = present;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, QZ_WAVE);
+        if (lane >= (uint32_t)d) incl += o;
+    }
+    if (lane == QZ_WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    for (uint32_t i = 0; i < w; ++i) incl += wsum[i];
+    if (present) {
+        fst[(size_t)b * 256 + incl - 1] = 1u + cum;
+        fsym[(size_t)b * 256 + incl - 1] = (uint8_t)c;
+    }
+    if (c == kIThreads - 1) fsig[b] = incl;
+}
+
+// Symbol whose run of F holds local row x (x >= 1; row 0 gives the first).
+// A walking lane holds its block's table in registers when the block has at
+// most kRegSyms symbols (DNA, binned qualities): the lookup is then compares
+// only. Global loads return in issue order, so a lookup through memory would
+// wait behind the walk's own long-latency load; larger alphabets take that
+// path (binary search over the table).
+constexpr int kRegSyms = 8;
+struct FTable {
+    uint32_t start[kRegSyms];
+    uint32_t syms[kRegSyms / 4];  // four symbols per word, low byte first
+    const uint32_t* fst;
+    const uint8_t* fsym;
+    uint32_t sig;
+
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ fst_all, const uint8_t* __restrict__ fsym_all,
+                                         const uint32_t* __restrict__ fsig, uint32_t b)
+    {
+        fst = fst_all + (size_t)b * 256;
+        fsym = fsym_all + (size_t)b * 256;
+        sig = fsig[b];
+        if (sig <= (uint32_t)kRegSyms) {
+#pragma unroll
+            for (int k = 0; k < kRegSyms; ++k) start[k] = (uint32_t)k < sig ? fst[k] : 0xffffffffu;
+#pragma unroll
+            for (int w = 0; w < kRegSyms / 4; ++w) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if ((uint32_t)(4 * w + q) < sig) v |= (uint32_t)fsym[4 * w + q] << (8 * q);
+                syms[w] = v;
+            }
+        }
+    }
+    __device__ __forceinline__ uint8_t lookup(uint32_t x) const
+    {
+        if (sig <= (uint32_t)kRegSyms) {
+            uint32_t sym = syms[0] & 0xffu;
+#pragma unroll
+            for (int k = 1; k < kRegSyms; ++k)
+                if (start[k] <= x) sym = (syms[k / 4] >> (8 * (k % 4))) & 0xffu;
+            return (uint8_t)sym;
+        }
+        uint32_t lo = 0, hi = sig;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (fst[mid] <= x) lo = mid; else hi = mid;
+        }
+        return fsym[lo];
+    }
+};
 
 __device__ __forceinline__ uint32_t find_block_u32(const uint32_t* __restrict__ base, uint32_t nblk, uint32_t i)
 {
@@ -144,9 +229,12 @@ static inline uint64_t seg_buf_bytes(uint64_t rows, uint64_t blocks) { return 2 
 
 // Phase 2 also keeps the first kBuf symbols of every segment (in walk order)
 // and, for longer segments, the row reached after kBuf steps.
-__global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restrict__ E,
+__global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint32_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
+                                                         const uint32_t* __restrict__ fst,
+                                                         const uint8_t* __restrict__ fsym,
+                                                         const uint32_t* __restrict__ fsig,
                                                          uint32_t nsplit, uint32_t split, uint32_t pool,
                                                          uint32_t* __restrict__ nxt, uint32_t* __restrict__ len,
                                                          uint8_t* __restrict__ buf, uint32_t* __restrict__ resume)
@@ -159,28 +247,31 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restri
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
     bool active = sid < pend;
-    uint32_t rb = 0, sb = 0, x = 0, l = 0;
-    uint64_t e = 0, word = 0;  // symbols of the current 8-step group, low byte first
+    uint32_t rb = 0, sb = 0, x = 0, l = 0, e = 0;
+    FTable ft;
+    ft.sig = 0;
+    uint64_t word = 0;  // symbols of the current 8-step group, low byte first
     if (active) {
         const uint32_t b = find_block_u32(sbase, nblk, sid);
         rb = off[b] + b;
         sb = sbase[b];
+        ft.load(fst, fsym, fsig, b);
         e = E[rb + (sid - sb) * split];
     }
     uint64_t* buf64 = reinterpret_cast<uint64_t*>(buf);
     while (__ballot(active)) {
         if (active) {
+            x = e & kLFMask;
+            const uint32_t en = E[rb + x];  // issued before the symbol lookup, which it does not need
             if (l < kBuf) {
-                word |= ((e >> 32) & 0xffull) << (8 * (l & 7));
+                word |= (uint64_t)ft.lookup(x) << (8 * (l & 7));
                 if ((l & 7) == 7) {
                     buf64[(size_t)sid * (kBuf / 8) + (l >> 3)] = word;
                     word = 0;
                 }
             }
             ++l;
-            x = (uint32_t)e;
             if (l == kBuf) resume[sid] = x;
-            const uint64_t en = E[rb + x];
             if (en & kSplitFlag) {
                 if (l < kBuf && (l & 7)) buf64[(size_t)sid * (kBuf / 8) + (l >> 3)] = word;
                 word = 0;
@@ -193,6 +284,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint64_t* __restri
                     const uint32_t b = find_block_u32(sbase, nblk, sid);
                     rb = off[b] + b;
                     sb = sbase[b];
+                    ft.load(fst, fsym, fsig, b);
                     e = E[rb + (sid - sb) * split];
                 }
             } else {
@@ -252,9 +344,12 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
 }
 
 // Phase 4b: segments longer than kBuf walk on from their resume row.
-__global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restrict__ E,
+__global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint32_t* __restrict__ E,
                                                          const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ sbase, uint32_t nblk,
+                                                         const uint32_t* __restrict__ fst,
+                                                         const uint8_t* __restrict__ fsym,
+                                                         const uint32_t* __restrict__ fsig,
                                                          uint32_t nsplit, uint32_t split, uint32_t pool,
                                                          const uint32_t* __restrict__ len,
                                                          const uint32_t* __restrict__ dist,
@@ -267,9 +362,10 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
     if (threadIdx.x == 0) next = pbeg;
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
-    uint32_t rb = 0, l = 0, t = 0, last = 0;
+    uint32_t rb = 0, l = 0, t = 0, last = 0, e = 0;
+    FTable ft;
+    ft.sig = 0;
     uint8_t* dst = nullptr;
-    uint64_t e = 0;
     // next splitter with a tail, or done
     auto pull = [&]() -> bool {
         for (;;) {
@@ -283,6 +379,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
                 t = n + 1u - dist[sid] + kBuf;
                 last = n;
                 dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
+                ft.load(fst, fsym, fsig, b);
                 e = E[rb + resume[sid]];
                 return true;
             }
@@ -292,13 +389,13 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint64_t* __restri
     bool active = pull();
     while (__ballot(active)) {
         if (active) {
-            if (t < last) dst[-(int64_t)t] = (uint8_t)(e >> 32);
+            const uint32_t x = e & kLFMask;
+            if (--l != 0) e = E[rb + x];  // issued before the symbol lookup
+            if (t < last) dst[-(int64_t)t] = ft.lookup(x);
             ++t;
-            if (--l == 0) {
+            if (l == 0) {
                 sid = atomicAdd(&next, 1u);
                 active = pull();
-            } else {
-                e = E[rb + (uint32_t)e];
             }
         }
     }
@@ -309,7 +406,7 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
-    return 2 * (c + 64) + rows * 8 + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
+    return 2 * (c + 64) + rows * 4 + (uint64_t)cb * (256 * 5 + 4) + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
            tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + seg_buf_bytes(rows, cb) + 64;
 }
 
@@ -327,7 +424,10 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     const uint64_t nsplit = rows / kMinSplit + cb + 1;
     QZ_HIP(hipMalloc(&U_, c + 64));
     QZ_HIP(hipMalloc(&T_, c + 64));
-    QZ_HIP(hipMalloc(&E_, rows * 8));
+    QZ_HIP(hipMalloc(&E_, rows * 4));
+    QZ_HIP(hipMalloc(&fst_, (size_t)cb * 256 * 4));
+    QZ_HIP(hipMalloc(&fsym_, (size_t)cb * 256));
+    QZ_HIP(hipMalloc(&fsig_, (size_t)cb * 4));
     QZ_HIP(hipMalloc(&hist_, tiles * 256 * 4));
     QZ_HIP(hipMalloc(&tmp_, (scan_tmp_words(tiles * 256) + 64) * 4));
     QZ_HIP(hipMalloc(&tiles_, tiles * sizeof(TileDesc)));
@@ -352,12 +452,15 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 
 void IbwtEngine::release()
 {
-    void* ps[] = {U_, T_, E_, hist_, tmp_, tiles_, off_, sbase_, prim_, nxt0_, nxt1_, len_, d0_, d1_, flag_, buf_, resume_};
+    void* ps[] = {U_,   T_,   E_,    hist_, tmp_, tiles_,  off_, sbase_, prim_, nxt0_, nxt1_,
+                  len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
     U_ = T_ = nullptr;
     E_ = nullptr;
+    fst_ = fsig_ = nullptr;
+    fsym_ = nullptr;
     hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = resume_ = nullptr;
     buf_ = nullptr;
     tiles_ = nullptr;
@@ -415,10 +518,13 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, st, off_, prim_, nblk, split, E_);
         QZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ib_ftable, dim3(ntiles), dim3(kIThreads), 0, st, (const TileDesc*)tiles_, hist_, off_,
+                           fst_, fsym_, fsig_);
+        QZ_HIP(hipGetLastError());
         const uint32_t gs = qz_div_up(nsplit, kIThreads);
         const uint32_t gp = qz_div_up(nsplit, pool);
-        hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, pool,
-                           nxt0_, len_, buf_, resume_);
+        hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
+                           nsplit, split, pool, nxt0_, len_, buf_, resume_);
         QZ_HIP(hipGetLastError());
         // list ranking; the first round reads the segment lengths directly,
         // later rounds alternate between d0_ and d1_
@@ -437,8 +543,8 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * (seg_buf(split) / 8), kIThreads)),
                            dim3(kIThreads), 0, st, off_, sbase_, nblk, nsplit, split, len_, din, buf_, T);
         QZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, nsplit, split, pool,
-                           len_, din, resume_, T);
+        hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
+                           nsplit, split, pool, len_, din, resume_, T);
         QZ_HIP(hipGetLastError());
         return 0;
     };
