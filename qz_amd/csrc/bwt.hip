@@ -50,11 +50,17 @@ struct Grp {
     uint32_t start, size;
 };
 
+// Where a round emits its new unsorted groups: one pool in any order (at
+// most n/2 groups of size >= 2 exist, so n/2 entries always suffice), with
+// per-class counters. Before the next round k_bucket moves the pool into one
+// list ordered by class, each class a contiguous run, which the next round's
+// kernels read.
 struct GroupLists {
-    Grp* list[kClasses];
-    uint32_t* cnt;        // kClasses counters
-    uint32_t cap[kClasses];
-    uint32_t* overflow;   // set if a list would overflow (cannot happen by construction)
+    Grp* pool;
+    uint32_t* top;        // pool fill counter
+    uint32_t* cnt;        // kClasses per-class counters
+    uint32_t cap;         // pool entries
+    uint32_t* overflow;   // set if the pool would overflow (cannot happen by construction)
 };
 
 // Block of a batch position: off[] holds the block offsets; tab[k] is the
@@ -101,58 +107,146 @@ __device__ __forceinline__ int grp_class(uint32_t size)
 }
 
 // New unsorted groups are staged in LDS per workgroup and flushed with one
-// device-scope atomic per size class, so the list counters see a few
-// thousand atomics per round rather than one per wave.
+// pool atomic and one counter atomic per size class, so the device counters
+// see a few thousand atomics per round rather than one per wave.
 constexpr int kStage = 2048;  // a workgroup touches <= 4096 suffixes -> <= 2048 groups
+struct StageHdr {
+    uint32_t cnt[kClasses];
+    uint32_t n;
+    uint32_t base;
+};
 template <int CAP>
 struct StageT {
-    uint32_t cnt[kClasses];
-    uint32_t base[kClasses];
-    uint32_t n;
-    uint4 e[CAP];  // (class, index in class, start, size)
+    StageHdr h;
+    uint2 e[CAP];  // (start, size)
 };
 using Stage = StageT<kStage>;
 
-template <class S>
-__device__ __forceinline__ void stage_init(S& S_)
+__device__ __forceinline__ void stage_init(StageHdr& H)
 {
-    if (threadIdx.x < kClasses) S_.cnt[threadIdx.x] = 0;
-    if (threadIdx.x == 0) S_.n = 0;
+    if (threadIdx.x < kClasses) H.cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) H.n = 0;
 }
 
-template <class S>
-__device__ __forceinline__ void stage_push_t(S& S_, uint32_t start, uint32_t size, uint32_t cap)
+__device__ __forceinline__ void stage_push_e(StageHdr& H, uint2* E, uint32_t start, uint32_t size, uint32_t cap)
 {
-    const uint32_t c = (uint32_t)grp_class(size);
-    const uint32_t i = atomicAdd(&S_.cnt[c], 1u);
-    const uint32_t k = atomicAdd(&S_.n, 1u);
-    if (k < cap) S_.e[k] = make_uint4(c, i, start, size);
+    atomicAdd(&H.cnt[grp_class(size)], 1u);
+    const uint32_t k = atomicAdd(&H.n, 1u);
+    if (k < cap) E[k] = make_uint2(start, size);
 }
 
-template <class S>
-__device__ __forceinline__ void stage_flush_t(S& S_, const GroupLists& gl, uint32_t cap)
+// Whole workgroup; ends with the staged groups in the device pool.
+__device__ __forceinline__ void stage_flush_e(StageHdr& H, const uint2* E, const GroupLists& gl, uint32_t cap)
 {
     __syncthreads();
-    if (threadIdx.x < kClasses && S_.cnt[threadIdx.x])
-        S_.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S_.cnt[threadIdx.x]);
-    if (threadIdx.x == 0 && S_.n > cap) *gl.overflow = 1u;
+    const uint32_t n = H.n < cap ? H.n : cap;
+    if (threadIdx.x < kClasses && H.cnt[threadIdx.x]) atomicAdd(&gl.cnt[threadIdx.x], H.cnt[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (H.n > cap) *gl.overflow = 1u;
+        H.base = n ? atomicAdd(gl.top, n) : 0u;
+    }
     __syncthreads();
-    const uint32_t n = S_.n < cap ? S_.n : cap;
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        const uint4 e = S_.e[k];
-        const uint32_t at = S_.base[e.x] + e.y;
-        if (at < gl.cap[e.x]) gl.list[e.x][at] = Grp{e.z, e.w};
+        const uint32_t at = H.base + k;
+        if (at < gl.cap) gl.pool[at] = Grp{E[k].x, E[k].y};
         else *gl.overflow = 1u;
     }
 }
 
+template <int CAP>
+__device__ __forceinline__ void stage_init(StageT<CAP>& S_) { stage_init(S_.h); }
+template <int CAP>
+__device__ __forceinline__ void stage_push_t(StageT<CAP>& S_, uint32_t start, uint32_t size, uint32_t cap)
+{
+    stage_push_e(S_.h, S_.e, start, size, cap);
+}
+template <int CAP>
+__device__ __forceinline__ void stage_flush_t(StageT<CAP>& S_, const GroupLists& gl, uint32_t cap)
+{
+    stage_flush_e(S_.h, S_.e, gl, cap);
+}
+
+// Pool -> list ordered by class: class c starts at the sum of the smaller
+// classes' counts. Each workgroup moves one contiguous range of kBucketRange
+// pool entries and keeps their order within every class (wave-major, then
+// step, then lane, ranked by ballots), so a class list is made of long runs
+// of groups emitted by the same workgroup: neighbouring slots, which the
+// next round's kernels read together. fill[] (zeroed) hands out the places
+// of each range's run within its class.
+constexpr uint32_t kBucketRange = 4096;
+__global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, uint32_t total,
+                                                const uint32_t* __restrict__ cnt, Grp* __restrict__ list,
+                                                uint32_t* __restrict__ fill)
+{
+    constexpr uint32_t W = 256 / QZ_WAVE, Q = kBucketRange / W;  // 1024 entries per wave
+    __shared__ uint32_t co[kClasses];
+    __shared__ uint32_t wb[W][kClasses];  // per wave: class counts, then write bases
+    const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
+    const uint32_t q0 = blockIdx.x * kBucketRange + w * Q;
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            co[c] = acc;
+            acc += cnt[c];
+        }
+    }
+    uint32_t run[kClasses];
+#pragma unroll
+    for (int c = 0; c < kClasses; ++c) run[c] = 0;
+    for (uint32_t b = q0; b < q0 + Q && b < total; b += QZ_WAVE) {  // wave-uniform
+        const uint32_t i = b + lane;
+        const int cls = i < total ? grp_class(pool[i].size) : kClasses;
+#pragma unroll
+        for (int c = 0; c < kClasses; ++c) run[c] += (uint32_t)__popcll(__ballot(cls == c));
+    }
+    if (lane < (uint32_t)kClasses) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int c = 0; c < kClasses; ++c) v = (lane == (uint32_t)c) ? run[c] : v;
+        wb[w][lane] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kClasses) {
+        const uint32_t c = threadIdx.x;
+        uint32_t tot = 0;
+        for (uint32_t v = 0; v < W; ++v) tot += wb[v][c];
+        uint32_t base = tot ? atomicAdd(&fill[c], tot) + co[c] : 0u;
+        for (uint32_t v = 0; v < W; ++v) {
+            const uint32_t x = wb[v][c];
+            wb[v][c] = base;
+            base += x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kClasses; ++c) run[c] = wb[w][c];
+    for (uint32_t b = q0; b < q0 + Q && b < total; b += QZ_WAVE) {
+        const uint32_t i = b + lane;
+        Grp g = {0, 0};
+        int cls = kClasses;
+        if (i < total) {
+            g = pool[i];
+            cls = grp_class(g.size);
+        }
+        uint32_t dst = 0;
+#pragma unroll
+        for (int c = 0; c < kClasses; ++c) {
+            const uint64_t m = __ballot(cls == c);
+            if (cls == c) dst = run[c] + popc_below(m);
+            run[c] += (uint32_t)__popcll(m);
+        }
+        if (cls < kClasses) list[dst] = g;
+    }
+}
+
+
 __device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t size)
 {
-    stage_push_t(S, start, size, kStage);
+    stage_push_e(S.h, S.e, start, size, kStage);
 }
 
 // Whole workgroup; ends with the staged groups in the device lists.
-__device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { stage_flush_t(S, gl, kStage); }
+__device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { stage_flush_e(S.h, S.e, gl, kStage); }
 
 __device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const BlockMap& bm, uint32_t s,
                                               uint32_t h)
@@ -267,7 +361,6 @@ __global__ __launch_bounds__(kThreads) void k_zero_u32(uint32_t* __restrict__ p,
 {
     for (uint32_t i = threadIdx.x; i < n; i += kThreads) p[i] = 0;
 }
-
 
 // Round-0 group heads, tile by tile over the sorted radix tiles (4096 slots,
 // never spanning two blocks). A slot is a head at its block's start or where
@@ -505,37 +598,6 @@ __device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a
 constexpr int kLdsDigitMax = 9;
 constexpr int kLdsBins = 1 << kLdsDigitMax;
 
-struct StageHdr {
-    uint32_t cnt[kClasses];
-    uint32_t base[kClasses];
-    uint32_t n;
-};
-
-// Packed stage entry: (start, class << 24 | index-in-class << 13 | size), for
-// subgroups of groups <= 4096 (index < 2048, size < 8192).
-__device__ __forceinline__ void stagep_push(StageHdr& S, uint2* E, uint32_t start, uint32_t size)
-{
-    const uint32_t c = (uint32_t)grp_class(size);
-    const uint32_t i = atomicAdd(&S.cnt[c], 1u);
-    const uint32_t k = atomicAdd(&S.n, 1u);
-    E[k] = make_uint2(start, (c << 24) | (i << 13) | size);
-}
-
-__device__ __forceinline__ void stagep_flush(StageHdr& S, const uint2* E, const GroupLists& gl)
-{
-    __syncthreads();
-    if (threadIdx.x < kClasses && S.cnt[threadIdx.x])
-        S.base[threadIdx.x] = atomicAdd(&gl.cnt[threadIdx.x], S.cnt[threadIdx.x]);
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < S.n; k += blockDim.x) {
-        const uint2 e = E[k];
-        const uint32_t c = e.y >> 24;
-        const uint32_t at = S.base[c] + ((e.y >> 13) & 2047u);
-        if (at < gl.cap[c]) gl.list[c][at] = Grp{e.x, e.y & 8191u};
-        else *gl.overflow = 1u;
-    }
-}
-
 template <int WAVES>
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum)
 {
@@ -668,8 +730,7 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
     if (blockIdx.x >= *cnt) return;  // block-uniform
     const Grp g = list[blockIdx.x];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / QZ_WAVE;
-    if (tid < kClasses) S.cnt[tid] = 0;
-    if (tid == 0) S.n = 0;
+    stage_init(S);
     uint32_t kmin = 0xffffffffu, kmax = 0;
     for (uint32_t i = tid; i < g.size; i += T) {
         const uint32_t k = key2[g.start + i];
@@ -723,9 +784,9 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
         sa[g.start + i] = v;
         if (hp) rank[v] = g.start + hp;  // hp == 0: still the group's start
         const bool end = (i + 1 == g.size) || Kf[i + 1] != Kf[i];
-        if (end && i > hp) stagep_push(S, E, g.start + hp, i - hp + 1);
+        if (end && i > hp) stage_push_e(S, E, g.start + hp, i - hp + 1, MAXN / 2);
     }
-    stagep_flush(S, E, gl);
+    stage_flush_e(S, E, gl, MAXN / 2);
 }
 
 // ---- large groups (> 4096): block-segmented radix sort of key2, one -----
@@ -907,12 +968,12 @@ int tune_state(int which, int* choice, double* ns8, double* ns11)
 
 template <typename K>
 static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
-                         uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* alt,
+                         uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s, bool* alt,
                          bool iota_vals = false)
 {
     if (bits == 11)
-        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt, iota_vals);
-    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, tmp, s, alt, iota_vals);
+        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
+    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
 }
 
 static int bitlen(uint64_t v)
@@ -922,25 +983,23 @@ static int bitlen(uint64_t v)
     return b;
 }
 
-static uint32_t class_cap(uint64_t n, int c)
-{
-    // a class-c group has more than min_size - 1 members
-    static const uint64_t min_size[kClasses] = {2, 3, 5, 9, 17, 33, 65, (uint64_t)kWaveMax + 1,
-                                                (uint64_t)kMedMax + 1};
-    return (uint32_t)(n / min_size[c] + 1);
-}
+// Counter words in cnt_: set A classes [0, 9) and pool top 9, set B classes
+// [32, 41) and top 41, overflow flag 18, bucket fill counters [48, 57).
+constexpr int kCntB = 32, kTopOff = kClasses, kOverflow = 2 * kClasses, kFill = 48;
+
+static uint64_t pool_cap(uint64_t n) { return n / 2 + 1; }
 
 uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
-    const size_t tmpw =
-        std::max(scan_tmp_words(c), seg_words + scan_tmp_words(seg_words) + 64) + 1024;
-    uint64_t groups = 0;
-    for (int k = 0; k < kClasses; ++k) groups += class_cap(c, k);
+    const size_t tmpw = scan_tmp_words(seg_words) + 1024;
+    const uint64_t groups = pool_cap(c);
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
-    return 2 * (c + 64) + 8 * 4 * c + tmpw * 4 + (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
+    // text, out; sa, rank, keys0/1, vals0/1 (4 B each); bufB (key2 or radix histograms); bufA per tile
+    return 2 * (c + 64) + 6 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
+           (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
            2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
            sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
 }
@@ -955,14 +1014,14 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(std::max<uint32_t>(nblk, 1), cap_blk_ ? cap_blk_ + cap_blk_ / 2 : 0);
     release();
     const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
-    size_t tmpw = std::max(scan_tmp_words(c), seg_words + scan_tmp_words(seg_words) + 64) +
-                  1024;
+    const size_t tmpw = scan_tmp_words(seg_words) + 1024;  // scan scratch; the histograms live in bufB
+    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;  // round 0 or large groups
     QZ_HIP(hipMalloc(&text_, c + 64));
     QZ_HIP(hipMalloc(&out_, c + 64));
     QZ_HIP(hipMalloc(&sa_, c * 4));
     QZ_HIP(hipMalloc(&rank_, c * 4));
-    QZ_HIP(hipMalloc(&bufA_, c * 4));
-    QZ_HIP(hipMalloc(&bufB_, c * 4));
+    QZ_HIP(hipMalloc(&bufA_, tile_cap * 4));  // last head per radix tile
+    QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(c, seg_words) * 4));
     QZ_HIP(hipMalloc(&keys0_, c * 4));
     QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
@@ -972,13 +1031,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
     QZ_HIP(hipMalloc(&hist_, 256 * 4));
     QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
-    for (int set = 0; set < 2; ++set) {
-        size_t words = 0;
-        for (int k = 0; k < kClasses; ++k) words += class_cap(c, k);
-        QZ_HIP(hipMalloc(&groups_[set], words * sizeof(Grp)));
-    }
+    for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
     QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
-    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;  // round 0 or large groups
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
@@ -1012,17 +1066,13 @@ void BwtEngine::release()
     cap_blk_ = 0;
 }
 
-// Builds the per-class list pointers of group set `set` (device).
-static GroupLists make_lists(void* base, uint32_t* cnt, uint32_t* overflow, uint64_t cap)
+static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint64_t cap)
 {
     GroupLists gl;
-    Grp* p = (Grp*)base;
-    for (int k = 0; k < kClasses; ++k) {
-        gl.list[k] = p;
-        gl.cap[k] = class_cap(cap, k);
-        p += gl.cap[k];
-    }
+    gl.pool = (Grp*)pool;
     gl.cnt = cnt;
+    gl.top = cnt + kTopOff;
+    gl.cap = (uint32_t)pool_cap(cap);
     gl.overflow = overflow;
     return gl;
 }
@@ -1051,7 +1101,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
     QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
-                                   32, tmp_, s, &alt, true));
+                                   32, bufB_, tmp_, s, &alt, true));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;
@@ -1060,8 +1110,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
-    uint32_t* overflow = cnt_ + 2 * kClasses;
-    const GroupLists gl = make_lists(groups_[0], cnt_, overflow, cap_);
+    const GroupLists gl = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_);
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, (const TileDesc*)tiles0_, bufA_, sa_,
                        rank_, gl);
     QZ_HIP(hipGetLastError());
@@ -1075,9 +1124,11 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     rounds_ = 0;
     trace_ = getenv("QZBSC_TRACE") != nullptr;
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
-    uint32_t* overflow = cnt_ + 2 * kClasses;
-    GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_),
-                         make_lists(groups_[1], cnt_ + 32, overflow, cap_)};
+    uint32_t* overflow = cnt_ + kOverflow;
+    // emission targets: the pool with counter set A or B, alternating
+    const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_),
+                               make_lists(groups_[0], cnt_ + kCntB, overflow, cap_)};
+    Grp* const list = (Grp*)groups_[1];
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
     const int bits0 = tune_pick(0, n);
@@ -1112,7 +1163,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     uint64_t large_n = 0;
     uint32_t* hc = (uint32_t*)pinned_;
     for (;;) {
-        QZ_HIP(hipMemcpyAsync(hc, cnt_ + 32 * cur, kClasses * 4, hipMemcpyDeviceToHost, s));
+        QZ_HIP(hipMemcpyAsync(hc, gls[cur].cnt, kClasses * 4, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipMemcpyAsync(hc + 16, overflow, 4, hipMemcpyDeviceToHost, s));
         if (rounds_ == 1)
             QZ_HIP(hipMemcpyAsync(hc + 20, &((const PackParams*)ppd_)->k0, 4, hipMemcpyDeviceToHost, s));
@@ -1152,10 +1203,22 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         const GroupLists& C = gls[cur];
         const GroupLists& N = gls[cur ^ 1];
-        QZ_HIP(hipMemsetAsync(N.cnt, 0, kClasses * 4, s));
+        // the last round's groups: pool -> list ordered by class
         CurLists cl;
-        for (int k = 0; k < kClasses; ++k) cl.list[k] = C.list[k];
-        cl.cnt = C.cnt;
+        {
+            uint32_t o = 0;
+            for (int k = 0; k < kClasses; ++k) {
+                cl.list[k] = list + o;
+                o += counts[k];
+            }
+            cl.cnt = C.cnt;
+            QZ_HIP(hipMemsetAsync(cnt_ + kFill, 0, kClasses * 4, s));
+            hipLaunchKernelGGL(k_bucket, dim3(qz_div_up(o, kBucketRange)), dim3(256), 0, s, C.pool, o, C.cnt, list,
+                               cnt_ + kFill);
+            QZ_HIP(hipGetLastError());
+        }
+        // this round emits into the pool again, with the other counter set
+        QZ_HIP(hipMemsetAsync(N.cnt, 0, (kClasses + 1) * 4, s));
 
         SmallPlan plan;
         plan.wave_base[0] = 0;
@@ -1177,7 +1240,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         std::vector<uint32_t> lb;
         if (nlarge) {
             lg.resize(nlarge);
-            QZ_HIP(hipMemcpyAsync(lg.data(), C.list[kClsLarge], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipMemcpyAsync(lg.data(), cl.list[kClsLarge], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
             QZ_HIP(hipStreamSynchronize(s));
             lb.resize(nlarge + 1);
             for (uint32_t i = 0; i < nlarge; ++i) {
@@ -1213,17 +1276,17 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
-            hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave,
+            hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave,
                                sa_, rank_, bmap_, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
-            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, sa_,
+            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, sa_,
                                rank_, bmap_, h, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
-            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, sa_, rank_,
+            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, sa_, rank_,
                                bmap_, h, lk0, vals0_);
             QZ_HIP(hipGetLastError());
         }
@@ -1233,19 +1296,22 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
-            hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, C.list[kClsWave], C.cnt + kClsWave, bufB_,
+            hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave, bufB_,
                                sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
-            hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, C.list[kClsMed], C.cnt + kClsMed, bufB_,
+            hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, bufB_,
                                sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
             QZ_HIP(hipEventRecord(tev_[2], s));
             bool alt = false;
-            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, bitlen(n), tmp_, s, &alt));
+            // histograms in bufB: this round's keys there are consumed by the
+            // small/wave/medium sorts, which precede this sort on the stream
+            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, bitlen(n), bufB_, tmp_, s,
+                                           &alt));
             QZ_HIP(hipEventRecord(tev_[3], s));
             large_timed = true;
             large_bits = lbits;
@@ -1255,7 +1321,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             hipLaunchKernelGGL(k_tile_lasthead, dim3(lt), dim3(kThreads), 0, s, keys, ltiles, bufA_);
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
-            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, C.list[kClsLarge], lbase_, keys, vals,
+            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
                                bufA_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }

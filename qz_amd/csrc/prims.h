@@ -288,16 +288,14 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 }
 
 // Sorts by key bits [0, key_bits) with BITS-bit digits. `tiles` must have
-// been built with radix 1 << BITS (hist_words from build_tiles). tmp holds
-// hist_words + scan scratch. Result in (k0,v0) for even pass counts.
+// been built with radix 1 << BITS (hist_words from build_tiles). hist holds
+// hist_words, stmp the scan scratch (scan_tmp_words(hist_words)). Result in (k0,v0) for even pass counts.
 template <typename K, int BITS>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
-                          uint32_t hist_words, int key_bits, uint32_t* tmp, hipStream_t s, bool* in_alt,
-                          bool iota_vals = false)
+                          uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s,
+                          bool* in_alt, bool iota_vals = false)
 {
     *in_alt = false;
-    uint32_t* hist = tmp;
-    uint32_t* stmp = tmp + hist_words + 32;
     K *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     int passes = 0;
