@@ -47,7 +47,10 @@ QZBSC_API int bsc_init(int features);
 
 /* Replaces libbsc bsc_compress; bound at bsc.rs:97-106, called at
  * bsc.rs:172-183 and bsc.rs:234-247. `output` holds n + 28 + 1024 bytes
- * (bsc.rs:170). Returns the block size or a negative error. */
+ * (bsc.rs:170). Returns the block size or a negative error. Parameter sets
+ * qz uses: (0, 0, BWT, ADAPTIVE) default, (0, 0, BWT, STATIC), (16, 128, BWT,
+ * ADAPTIVE) with LZP; the static and fast coder requests are served by the
+ * adaptive coder and recorded as such in the block header. */
 QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize,
                            int lzpMinLen, int blockSorter, int coder, int features);
 
@@ -150,6 +153,17 @@ QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11);
  * captured the second time in a row that an engine sees a batch shape and
  * replayed while the shape repeats; QZBSC_GRAPH=0 turns replay off. */
 QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long* replays);
+
+/* LZP stage alone (host): the preprocessing bsc_compress applies when given
+ * lzpHashSize / lzpMinLen (libbsc lzp.h bsc_lzp_compress / bsc_lzp_decompress;
+ * parameter sets of crates/qz-lib/src/compression/bsc.rs:194-196, 237-246).
+ * compress returns the encoded size, or LIBBSC_NOT_COMPRESSIBLE when it
+ * would not be smaller than n (output holds n bytes); decompress writes
+ * exactly outputSize bytes and returns 0 or LIBBSC_DATA_CORRUPT. */
+QZBSC_API int qzbsc_lzp_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize,
+                                 int lzpMinLen);
+QZBSC_API int qzbsc_lzp_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
+                                   int lzpHashSize, int lzpMinLen);
 
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);

@@ -45,6 +45,8 @@ _SIGS = {
     "qzbsc_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "qzbsc_graph_stats": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "qzbsc_trim": (C.c_longlong, []),
+    "qzbsc_lzp_compress": (C.c_int, [u8p, u8p, C.c_int, C.c_int, C.c_int]),
+    "qzbsc_lzp_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int]),
 }
 
 _lib = None

@@ -73,13 +73,20 @@ def compress_with_params(data, lzp_hash_size: int, lzp_min_len: int, block_sorte
 
 
 def compress(data) -> bytes:
-    """bsc.rs:146-150: static QLFC (not supported by this engine yet)."""
+    """bsc.rs:146-150: the static-QLFC request, served by the adaptive coder
+    (the block header records coder 2)."""
     return compress_with_params(data, 0, 0, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_STATIC)
 
 
 def compress_adaptive(data) -> bytes:
-    """bsc.rs:194-196: LZP 16/128 + adaptive QLFC (LZP not supported yet)."""
+    """bsc.rs:194-196: LZP 16/128 + adaptive QLFC."""
     return compress_with_params(data, 16, 128, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_ADAPTIVE)
+
+
+def compress_adaptive_mt(data) -> bytes:
+    """bsc.rs:212-215,224-255: LZP 16/128, segment coding on several threads."""
+    return compress_with_params(data, 16, 128, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_ADAPTIVE,
+                                LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_MULTITHREADING | LIBBSC_FEATURE_CUDA)
 
 
 def compress_adaptive_no_lzp(data) -> bytes:
@@ -109,6 +116,31 @@ def _compress_parallel_with(data, block_compressor) -> bytes:
 
 def compress_parallel_adaptive_no_lzp(data) -> bytes:
     return _compress_parallel_with(data, compress_adaptive_no_lzp)
+
+
+def compress_parallel_adaptive(data) -> bytes:
+    """bsc.rs:206-210: the columnar header path (LZP 16/128)."""
+    return _compress_parallel_with(data, compress_adaptive)
+
+
+def lzp_compress(data, hash_size: int = 16, min_len: int = 128) -> bytes:
+    """The LZP stage alone (libbsc lzp.h bsc_lzp_compress); raises -3 when the
+    result would not be smaller than the input."""
+    a = _arr(data)
+    out = np.zeros(max(a.size, 1), dtype=np.uint8)
+    r = lib().qzbsc_lzp_compress(a.ctypes.data_as(u8p), out.ctypes.data_as(u8p), a.size, hash_size, min_len)
+    if r < 0:
+        raise BscError(r, "LZP compression")
+    return out[:r].tobytes()
+
+
+def lzp_decompress(data, n: int, hash_size: int = 16, min_len: int = 128) -> bytes:
+    a = _arr(data)
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    r = lib().qzbsc_lzp_decompress(a.ctypes.data_as(u8p), a.size, out.ctypes.data_as(u8p), n, hash_size, min_len)
+    if r < 0:
+        raise BscError(r, "LZP decompression")
+    return out[:n].tobytes()
 
 
 def decompress_with_features(data, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> bytes:

@@ -11,6 +11,12 @@
 //        (a single segment: [1][segment]), then num_indexes x i32 sampled
 //        BWT indexes and the count byte num_indexes.
 // Blocks of <= 28 bytes, or that do not shrink, are stored (mode 0).
+// With LZP (mode bits 8..23 set) the data is LZP-encoded first (lzp.cpp),
+// the coder payload is preceded by the LZP output size m (i32), and the
+// BWT, primary and sampled indexes refer to those m bytes; a block that LZP
+// does not shrink is coded without it. Requests for the static (1) or fast
+// (3) QLFC coder are served by the adaptive coder, and the header records
+// coder 2, which is what the payload holds.
 //
 // The forward and inverse BWT always run on the GPU (there is no CPU
 // transform in this library); the QLFC coder runs on the host.
@@ -18,6 +24,7 @@
 #include "api_internal.h"
 #include "engine.h"
 #include "pool.h"
+#include "lzp.h"
 #include "qlfc_coder.h"
 #include "../../include/qzbsc.h"
 
@@ -246,13 +253,11 @@ int check_params(int n, int lzpHashSize, int lzpMinLen, int blockSorter, int cod
 {
     if (blockSorter != LIBBSC_BLOCKSORTER_BWT) return (blockSorter >= 3 && blockSorter <= 8) ? QZ_NOT_SUPPORTED
                                                                                               : QZ_BAD_PARAMETER;
-    if (coder != LIBBSC_CODER_QLFC_ADAPTIVE) return (coder == 1 || coder == 3) ? QZ_NOT_SUPPORTED : QZ_BAD_PARAMETER;
-    if (lzpMinLen != 0 || lzpHashSize != 0) {
-        if (lzpMinLen < 4 || lzpMinLen > 255 || lzpHashSize < 10 || lzpHashSize > 28) return QZ_BAD_PARAMETER;
-        return QZ_NOT_SUPPORTED;
-    }
+    if (coder < 1 || coder > 3) return QZ_BAD_PARAMETER;
+    if ((lzpMinLen != 0 || lzpHashSize != 0) && !lzp::params_ok(lzpHashSize, lzpMinLen)) return QZ_BAD_PARAMETER;
     if (n < 0 || n > (1 << 30)) return QZ_BAD_PARAMETER;
-    *mode = blockSorter | (coder << 5);
+    *mode = blockSorter | (LIBBSC_CODER_QLFC_ADAPTIVE << 5);
+    if (lzpHashSize) *mode |= (lzpMinLen << 8) | (lzpHashSize << 16);
     return 0;
 }
 
@@ -281,22 +286,26 @@ int finish_block(const uint8_t* input, const uint8_t* bwt, int n, int mode, int 
     return body + kHeader;
 }
 
-// Same as finish_block, with the coder payload produced by a callable.
+// Same as finish_block, with the coder payload produced by a callable. dn
+// is the size of the data the BWT ran on: n, or the LZP output size when
+// mode carries LZP parameters (then written ahead of the coder payload).
 template <class Payload>
-static int finish_block_with(const uint8_t* input, int n, int mode, int primary, const int* aux, uint8_t* out,
-                             Payload payload_fn)
+static int finish_block_with(const uint8_t* input, int n, int dn, int mode, int primary, const int* aux,
+                             uint8_t* out, Payload payload_fn)
 {
     const uint32_t adler_data = adler32(input, (size_t)n);
-    int num_indexes = (n - 1) / bwt_sample_rate(n);
-    if (n < 64 * 1024) num_indexes = 0;
-    std::vector<uint8_t> payload((size_t)n + 4096);
+    int num_indexes = (dn - 1) / bwt_sample_rate(dn);
+    if (dn < 64 * 1024) num_indexes = 0;
+    const int pre = (mode >> 8) ? 4 : 0;
+    std::vector<uint8_t> payload((size_t)dn + 4096);
     const int result = payload_fn(payload.data(), (int)payload.size());
     if (result == QZ_GPU_ERROR) return result;
-    if (result < 0 || result + 1 + 4 * num_indexes >= n) return store_block(input, out, n);
-    memcpy(out + kHeader, payload.data(), (size_t)result);
-    for (int t = 0; t < num_indexes; ++t) put32(out + kHeader + result + 4 * t, (uint32_t)(aux[t + 1] - 1));
-    out[kHeader + result + 4 * num_indexes] = (uint8_t)num_indexes;
-    const int body = result + 1 + 4 * num_indexes;
+    if (result < 0 || pre + result + 1 + 4 * num_indexes >= n) return store_block(input, out, n);
+    if (pre) put32(out + kHeader, (uint32_t)dn);
+    memcpy(out + kHeader + pre, payload.data(), (size_t)result);
+    for (int t = 0; t < num_indexes; ++t) put32(out + kHeader + pre + result + 4 * t, (uint32_t)(aux[t + 1] - 1));
+    out[kHeader + pre + result + 4 * num_indexes] = (uint8_t)num_indexes;
+    const int body = pre + result + 1 + 4 * num_indexes;
     put32(out + 0, (uint32_t)(body + kHeader));
     put32(out + 4, (uint32_t)n);
     put32(out + 8, (uint32_t)mode);
@@ -349,6 +358,17 @@ int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, u
     return coder_decompress(input + kHeader, payload, bwt, bh.data_size, parallel);
 }
 
+// LZP blocks: the size m of the LZP data the BWT ran on (0 if the block
+// carries no LZP parameters), checked against the header.
+int lzp_data_size(const uint8_t* input, const BlockHeader& bh)
+{
+    if ((bh.mode >> 8) == 0 || bh.mode == 0) return 0;
+    if (bh.block_size < kHeader + 4) return QZ_DATA_CORRUPT;
+    const int m = (int)get32(input + kHeader);
+    if (m < 1 || m >= bh.data_size || bh.index > m) return QZ_DATA_CORRUPT;
+    return m;
+}
+
 }  // namespace qz
 
 using namespace qz;
@@ -369,19 +389,33 @@ QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, in
     QZ_TRY(check_params(n, lzpHashSize, lzpMinLen, blockSorter, coder, &mode));
     if (!input || !output) return QZ_BAD_PARAMETER;
     if (n <= kHeader) return store_block(input, output, n);
+    // LZP first (host); a block it does not shrink is coded without it
+    std::vector<uint8_t> lz;
+    const uint8_t* data = input;
+    int dn = n;
+    if (mode >> 8) {
+        lz.resize((size_t)n);
+        const int m = lzp::encode(input, n, lz.data(), n, lzpHashSize, lzpMinLen);
+        if (m > 1) {
+            data = lz.data();
+            dn = m;
+        } else {
+            mode &= 0xff;
+        }
+    }
     SlotGuard g;
     if (!g.ok()) return QZ_GPU_ERROR;
     Slot& sl = g.slot();
-    std::vector<uint32_t> offs = {0u, (uint32_t)n};
+    std::vector<uint32_t> offs = {0u, (uint32_t)dn};
     int primary = 0;
     std::vector<int> aux(256);
-    QZ_TRY(sl.fwd.reserve((uint64_t)n, 1));
-    QZ_HIP(hipMemcpyAsync(sl.fwd.text_, input, (size_t)n, hipMemcpyHostToDevice, sl.stream));
+    QZ_TRY(sl.fwd.reserve((uint64_t)dn, 1));
+    QZ_HIP(hipMemcpyAsync(sl.fwd.text_, data, (size_t)dn, hipMemcpyHostToDevice, sl.stream));
     QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, &primary, aux.data(), nullptr));
     QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
     const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
-    return finish_block_with(input, n, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
-        return coder_compress_front(sl.front, sl.fwd.out_, 0, n, p, cap, par, sl.stream);
+    return finish_block_with(input, n, dn, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
+        return coder_compress_front(sl.front, sl.fwd.out_, 0, dn, p, cap, par, sl.stream);
     });
 }
 
@@ -404,14 +438,44 @@ QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned
     BlockHeader bh;
     QZ_TRY(parse_header(input, inputSize, &bh));
     if (inputSize < bh.block_size || outputSize < bh.data_size) return QZ_UNEXPECTED_EOB;
-    const int rc = decode_payload(input, inputSize, bh, output, (features & LIBBSC_FEATURE_MULTITHREADING) != 0);
-    if (rc < 0) return rc;
-    if (rc == 0 && bh.data_size > 1) {
-        int n = bh.data_size;
-        QZ_TRY(qzbsc_bwt_decode_batch(output, output, &n, 1, &bh.index));
+    const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
+    const int m = lzp_data_size(input, bh);
+    if (m < 0) return m;
+    if (m > 0) {
+        // LZP block: coder payload after the size word -> m-byte BWT ->
+        // inverse BWT -> LZP decode into the caller's buffer
+        if (adler32(input + kHeader, (size_t)(bh.block_size - kHeader)) != bh.adler_body) return QZ_DATA_CORRUPT;
+        const int num_indexes = input[bh.block_size - 1];
+        const int payload = bh.block_size - kHeader - 4 - 1 - 4 * num_indexes;
+        if (payload < 1) return QZ_DATA_CORRUPT;
+        std::vector<uint8_t> tmp((size_t)m);
+        if (coder_decompress(input + kHeader + 4, payload, tmp.data(), m, par)) return QZ_DATA_CORRUPT;
+        if (m > 1) QZ_TRY(qzbsc_bwt_decode_batch(tmp.data(), tmp.data(), &m, 1, &bh.index));
+        const int lzpMin = (bh.mode >> 8) & 0xff, lzpHash = (bh.mode >> 16) & 0xff;
+        if (lzp::decode(tmp.data(), m, output, bh.data_size, lzpHash, lzpMin)) return QZ_DATA_CORRUPT;
+    } else {
+        const int rc = decode_payload(input, inputSize, bh, output, par);
+        if (rc < 0) return rc;
+        if (rc == 0 && bh.data_size > 1) {
+            int n = bh.data_size;
+            QZ_TRY(qzbsc_bwt_decode_batch(output, output, &n, 1, &bh.index));
+        }
     }
     if (adler32(output, (size_t)bh.data_size) != bh.adler_data) return QZ_DATA_CORRUPT;
     return LIBBSC_NO_ERROR;
+}
+
+// LZP stage alone (host): libbsc lzp.h's bsc_lzp_compress / _decompress.
+QZBSC_API int qzbsc_lzp_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize,
+                                 int lzpMinLen)
+{
+    return lzp::encode(input, n, output, n, lzpHashSize, lzpMinLen);
+}
+
+QZBSC_API int qzbsc_lzp_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
+                                   int lzpHashSize, int lzpMinLen)
+{
+    return lzp::decode(input, inputSize, output, outputSize, lzpHashSize, lzpMinLen);
 }
 
 // Device QLFC front end of one BWT block (host buffers): segments, tables

@@ -82,10 +82,38 @@ def test_parallel_framing_roundtrip():
     assert bsc.compress_parallel_adaptive_no_lzp(b"") == b"" and bsc.decompress_parallel(b"") == b""
 
 
-def test_unsupported_modes_raise():
-    with pytest.raises(BscError) as e:
-        bsc.compress(b"ACGT" * 100)
-    assert e.value.code == -4
-    with pytest.raises(BscError) as e:
-        bsc.compress_adaptive(b"ACGT" * 100)
-    assert e.value.code == -4
+def test_lzp_and_coder_requests_roundtrip():
+    """qz's other parameter sets (bsc.rs:146-150, 194-196, 212-215): LZP
+    16/128 and other LZP settings, and the static-QLFC request, all round
+    trip; a block LZP shrinks records both LZP parameters in its mode word,
+    and every coded block records the adaptive coder."""
+    rep = (synth.header_stream(3_000, 21).tobytes() * 6)[:700_000]  # long repeats: LZP 16/128 finds them
+    cases = _inputs() + [rep]
+    for data in cases:
+        for fn in (bsc.compress, bsc.compress_adaptive, bsc.compress_adaptive_mt):
+            blk = fn(data)
+            assert bsc.decompress(blk) == bytes(data)
+            mode = struct.unpack_from("<i", blk, 8)[0]
+            if mode:
+                assert mode & 0xff == 0x41
+    blk = bsc.compress_adaptive(rep)
+    mode = struct.unpack_from("<i", blk, 8)[0]
+    assert mode == 0x41 | (128 << 8) | (16 << 16)
+    assert len(blk) < len(bsc.compress_adaptive_no_lzp(rep))
+    for hs, ml in ((10, 4), (20, 32), (28, 255)):
+        blk = bsc.compress_with_params(rep, hs, ml, 1, 2)
+        assert bsc.decompress(blk) == rep
+        assert struct.unpack_from("<i", blk, 8)[0] >> 8 in (0, ml | (hs << 8))
+    framed = bsc.compress_parallel_adaptive(rep)
+    assert struct.unpack_from("<I", framed, 0)[0] == 1
+
+
+def test_lzp_block_corruption_fails():
+    rep = (synth.header_stream(3_000, 22).tobytes() * 6)[:700_000]
+    blk = bytearray(bsc.compress_adaptive(rep))
+    assert struct.unpack_from("<i", blk, 8)[0] >> 8
+    for pos in (30, len(blk) // 2, len(blk) - 2):
+        bad = bytearray(blk)
+        bad[pos] ^= 0x10
+        with pytest.raises(BscError):
+            bsc.decompress(bytes(bad))

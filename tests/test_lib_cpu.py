@@ -176,11 +176,18 @@ def test_bad_parameters():
     out = np.zeros(2000, np.uint8)
     ap, op = a.ctypes.data_as(u8p), out.ctypes.data_as(u8p)
     assert L.bsc_compress(ap, op, 100, 0, 0, 2, 2, 1) == -1      # unknown sorter
+    assert L.bsc_compress(ap, op, 100, 0, 0, 3, 2, 1) == -4      # ST sorters: never selected by qz
     assert L.bsc_compress(ap, op, 100, 0, 0, 1, 9, 1) == -1      # unknown coder
-    assert L.bsc_compress(ap, op, 100, 0, 0, 1, 1, 1) == -4      # static QLFC not supported yet
-    assert L.bsc_compress(ap, op, 100, 16, 128, 1, 2, 1) == -4   # LZP not supported yet
     assert L.bsc_compress(ap, op, 100, 3, 128, 1, 2, 1) == -1    # LZP hash size out of range
+    assert L.bsc_compress(ap, op, 100, 29, 128, 1, 2, 1) == -1
+    assert L.bsc_compress(ap, op, 100, 16, 3, 1, 2, 1) == -1     # LZP min length out of range
+    assert L.bsc_compress(ap, op, 100, 16, 256, 1, 2, 1) == -1
     assert L.bsc_compress(ap, op, -1, 0, 0, 1, 2, 1) == -1
+    # qz's static-QLFC and LZP 16/128 parameter sets pass validation (then
+    # need the GPU: -7 where there is none)
+    for args in ((0, 0, 1, 1), (16, 128, 1, 2), (0, 0, 1, 3)):
+        r = L.bsc_compress(ap, op, 100, *args, 1)
+        assert r == -7 or r > 0, (args, r)
 
 
 def test_libbsc_payloads_are_rejected_not_misdecoded():
