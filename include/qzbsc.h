@@ -165,6 +165,21 @@ QZBSC_API int qzbsc_lzp_compress(const unsigned char* input, unsigned char* outp
 QZBSC_API int qzbsc_lzp_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
                                    int lzpHashSize, int lzpMinLen);
 
+/* quality_ctx coder (host): qz's context-adaptive quality compressor,
+ * replacing compress_qualities_ctx / decompress_qualities_ctx
+ * (crates/qz-lib/src/compression/quality_ctx.rs:296-391, 393-505) with the
+ * same blob format and integer arithmetic. Reads are concatenated: read r
+ * has qual_lens[r] quality bytes (Phred+33) and seq_lens[r] sequence bytes.
+ * compress returns the blob size (0 for no reads), LIBBSC_BAD_PARAMETER for
+ * the reference's input errors, or LIBBSC_NOT_ENOUGH_MEMORY if cap is too
+ * small. decompress fills quals (room for the sum of seq_lens) and
+ * qual_lens; returns 0 or LIBBSC_DATA_CORRUPT. */
+QZBSC_API long long qzbsc_quality_ctx_compress(const unsigned char* quals, const int* qual_lens,
+                                               const unsigned char* seqs, const int* seq_lens, int num_reads,
+                                               unsigned char* out, long long cap);
+QZBSC_API int qzbsc_quality_ctx_decompress(const unsigned char* data, long long size, const unsigned char* seqs,
+                                           const int* seq_lens, int num_reads, unsigned char* quals, int* qual_lens);
+
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
 /* Releases the resident workspaces of every idle engine (no reference

@@ -47,6 +47,8 @@ _SIGS = {
     "qzbsc_trim": (C.c_longlong, []),
     "qzbsc_lzp_compress": (C.c_int, [u8p, u8p, C.c_int, C.c_int, C.c_int]),
     "qzbsc_lzp_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int]),
+    "qzbsc_quality_ctx_compress": (C.c_longlong, [u8p, i32p, u8p, i32p, C.c_int, u8p, C.c_longlong]),
+    "qzbsc_quality_ctx_decompress": (C.c_int, [u8p, C.c_longlong, u8p, i32p, C.c_int, u8p, i32p]),
 }
 
 _lib = None
