@@ -124,10 +124,10 @@ def tuning():
     """Radix digit-width tuner state for the round-0 and large-group sorts."""
     res = {}
     for which, name in ((0, "round0"), (1, "large_groups")):
-        ch, a, b = C.c_int(0), C.c_double(0), C.c_double(0)
-        trials = lib().qzbsc_tuning(which, C.byref(ch), C.byref(a), C.byref(b))
-        res[name] = {"digit_bits": ch.value or None, "trials": trials, "ns_per_elem_8": round(a.value, 4),
-                     "ns_per_elem_11": round(b.value, 4)}
+        ch, ns = C.c_int(0), (C.c_double * 3)()
+        trials = lib().qzbsc_tuning(which, C.byref(ch), ns)
+        res[name] = {"digit_bits": ch.value or None, "trials": trials,
+                     "ns_per_elem": {str(w): round(ns[k], 4) for k, w in enumerate((8, 9, 11)) if ns[k] > 0}}
     return res
 
 

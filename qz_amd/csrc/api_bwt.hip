@@ -179,12 +179,12 @@ QZBSC_API int qzbsc_bwt_decode(unsigned char* T, int n, int index, unsigned char
 }
 
 // Digit-width tuner state (which = 0: round-0 sort, 1: large-group sort):
-// chosen width (0 while undecided) and mean ns per element for 8 / 11 bits.
+// chosen width (0 while undecided) and mean ns per element for 8 / 9 / 11 bits.
 // Returns the number of timed trials.
-QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns8, double* ns11)
+QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns_per_elem)
 {
-    if ((which != 0 && which != 1) || !choice || !ns8 || !ns11) return QZ_BAD_PARAMETER;
-    return tune_state(which, choice, ns8, ns11);
+    if ((which != 0 && which != 1) || !choice || !ns_per_elem) return QZ_BAD_PARAMETER;
+    return tune_state(which, choice, ns_per_elem);
 }
 
 QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long* replays)

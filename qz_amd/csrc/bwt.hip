@@ -802,10 +802,15 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
 {
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
+    // a group lies in one block, whose text positions and SA slots are the
+    // same range: keys relative to its start need only bits(block size)
+    const uint32_t blk = bm.find(g0);
+    const uint32_t bstart = bm.off[blk], bend = bm.off[blk + 1];
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
         const uint32_t j = td.start + i;
         const uint32_t s = sa[g0 + (j - b0)];
-        keys[j] = round_key(rank, bm, s, h);
+        const uint64_t p = (uint64_t)s + h;
+        keys[j] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
         vals[j] = s;
     }
 }
@@ -910,59 +915,89 @@ __global__ void k_bwt_index(const uint32_t* __restrict__ rank, BlockMap bm,
 // alternates the two on the first sorts of at least kTuneMin elements, times
 // them with HIP events, and keeps the faster per element for the process.
 // QZBSC_RADIX_BITS=8|11 forces a width.
+// Candidate digit widths: 9 bits serves the large-group sort, whose block-
+// local keys need at most 27 bits for blocks of up to 128 MiB (3 passes
+// where 8-bit digits take 4); round 0's 32-bit keys gain nothing from it.
+constexpr int kWidths[3] = {8, 9, 11};
 struct RadixTuner {
     std::mutex mu;
     int choice = 0;
-    double ns[2] = {0, 0};
-    double elems[2] = {0, 0};
+    double ns[3] = {0, 0, 0};
+    double elems[3] = {0, 0, 0};
     int trials = 0;
+    int ncand = 2;
+    int cand[3] = {8, 11, 0};
 };
 RadixTuner g_tune[2];  // [0] round 0, [1] large groups
 constexpr uint64_t kTuneMin = 1u << 22;
-constexpr int kTuneTrials = 2;
+static std::once_flag g_tune_once;
+
+static void tune_init()
+{
+    std::call_once(g_tune_once, [] {
+        g_tune[1].ncand = 3;
+        for (int k = 0; k < 3; ++k) g_tune[1].cand[k] = kWidths[k];
+    });
+}
 
 static int tune_pick(int which, uint64_t n)
 {
+    tune_init();
     if (const char* e = getenv("QZBSC_RADIX_BITS")) {
         const int v = atoi(e);
-        if (v == 8 || v == 11) return v;
+        if (v == 8 || v == 9 || v == 11) return v;
     }
     RadixTuner& t = g_tune[which];
     std::lock_guard<std::mutex> lk(t.mu);
     if (t.choice) return t.choice;
     if (n < kTuneMin) return 8;
-    return (t.trials & 1) ? 11 : 8;
+    return t.cand[t.trials % t.ncand];
 }
 
-// True while the tuner still times sorts of n elements (its picks alternate).
+// True while the tuner still times sorts of n elements (its picks rotate).
 static bool tune_wants(int which, uint64_t n)
 {
+    tune_init();
     if (getenv("QZBSC_RADIX_BITS")) return false;
     RadixTuner& t = g_tune[which];
     std::lock_guard<std::mutex> lk(t.mu);
     return !t.choice && n >= kTuneMin;
 }
 
+static int width_index(int bits) { return bits == 9 ? 1 : bits == 11 ? 2 : 0; }
+
+// One timed sort of n elements with `bits`-bit digits; after one trial of
+// every candidate the fastest per element is kept for the process.
 static void tune_report(int which, int bits, uint64_t n, float ms)
 {
     if (n < kTuneMin || ms <= 0) return;
     RadixTuner& t = g_tune[which];
     std::lock_guard<std::mutex> lk(t.mu);
     if (t.choice) return;
-    const int k = bits == 11 ? 1 : 0;
+    const int k = width_index(bits);
     t.ns[k] += ms * 1e6;
     t.elems[k] += (double)n;
-    if (++t.trials >= kTuneTrials && t.elems[0] > 0 && t.elems[1] > 0)
-        t.choice = (t.ns[1] / t.elems[1] < t.ns[0] / t.elems[0]) ? 11 : 8;
+    ++t.trials;
+    int best = 0;
+    double best_ns = 0;
+    for (int c = 0; c < t.ncand; ++c) {
+        const int w = width_index(t.cand[c]);
+        if (t.elems[w] <= 0) return;  // a candidate is still untried
+        const double v = t.ns[w] / t.elems[w];
+        if (!best || v < best_ns) {
+            best = t.cand[c];
+            best_ns = v;
+        }
+    }
+    t.choice = best;
 }
 
-int tune_state(int which, int* choice, double* ns8, double* ns11)
+int tune_state(int which, int* choice, double* ns_per_elem)
 {
     RadixTuner& t = g_tune[which];
     std::lock_guard<std::mutex> lk(t.mu);
     *choice = t.choice;
-    *ns8 = t.elems[0] > 0 ? t.ns[0] / t.elems[0] : 0;
-    *ns11 = t.elems[1] > 0 ? t.ns[1] / t.elems[1] : 0;
+    for (int k = 0; k < 3; ++k) ns_per_elem[k] = t.elems[k] > 0 ? t.ns[k] / t.elems[k] : 0;
     return t.trials;
 }
 
@@ -973,6 +1008,8 @@ static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, con
 {
     if (bits == 11)
         return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
+    if (bits == 9)
+        return seg_radix_sort<K, 9>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
     return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
 }
 
@@ -1156,6 +1193,9 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     ++rounds_;
 
     // ---- doubling rounds ---------------------------------------------------
+    uint32_t maxb = 0;
+    for (uint32_t b = 0; b < nblk; ++b) maxb = std::max(maxb, offs_host_[b + 1] - offs_host_[b]);
+    const int lkey_bits = bitlen(maxb);  // block-local large-group keys are <= block size
     uint32_t h = 0;  // k0, read back with the first counts
     int cur = 0;
     bool large_timed = false;
@@ -1310,8 +1350,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             bool alt = false;
             // histograms in bufB: this round's keys there are consumed by the
             // small/wave/medium sorts, which precede this sort on the stream
-            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, bitlen(n), bufB_, tmp_, s,
-                                           &alt));
+            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, lkey_bits, bufB_, tmp_,
+                                           s, &alt));
             QZ_HIP(hipEventRecord(tev_[3], s));
             large_timed = true;
             large_bits = lbits;

@@ -61,7 +61,7 @@ private:
 
 // Digit-width tuner state: which = 0 round 0, 1 large groups. Returns the
 // number of timed trials; choice is 0 while undecided.
-int tune_state(int which, int* choice, double* ns8, double* ns11);
+int tune_state(int which, int* choice, double* ns_per_elem /* [3]: 8, 9, 11 bits */);
 
 class IbwtEngine {
 public:

@@ -117,3 +117,17 @@ def test_lzp_block_corruption_fails():
         bad[pos] ^= 0x10
         with pytest.raises(BscError):
             bsc.decompress(bytes(bad))
+
+
+def test_large_block_mt_roundtrip():
+    """qz's large-block MT mode (compress_adaptive_mt_no_lzp, ultra.rs:877-887
+    uses blocks up to 750 MB): one 400 MB block through the drop-in ABI."""
+    data = synth.config2_sequences(2_700_000, seed=31)[:400_000_000]
+    assert data.size == 400_000_000
+    blk = bsc.compress_adaptive_mt_no_lzp(data)
+    size, n, mode = struct.unpack_from("<iii", blk, 0)
+    assert size == len(blk) and n == data.size and mode == 0x41
+    assert blk[-1] == (n - 1) // O.sample_rate(n)
+    assert len(blk) < data.size // 3
+    back = bsc.decompress(blk)
+    assert back == data.tobytes()

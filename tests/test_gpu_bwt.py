@@ -74,11 +74,11 @@ def test_bwt_config1_block(gpu):
 
 
 def test_digit_widths_give_identical_order(gpu, monkeypatch):
-    """8- and 11-bit radix digits (the tuner's two choices) sort alike, on
+    """8-, 9- and 11-bit radix digits (the tuner's choices) sort alike, on
     round 0 and on groups larger than 4096 (headers give large groups)."""
     blocks = [synth.header_stream(60_000, 31), synth.quality_stream(20_000, 150, 32)]
     res = []
-    for bits in ("8", "11"):
+    for bits in ("8", "9", "11"):
         monkeypatch.setenv("QZBSC_RADIX_BITS", bits)
         out, prim, aux = gpu.bwt.encode_batch(blocks)
         res.append((out, prim))
