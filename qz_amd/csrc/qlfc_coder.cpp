@@ -51,11 +51,10 @@ struct Tables {
         for (int n = 0; n < 256; ++n) rate[n] = (uint16_t)(8192 / (2 * n + 3));
     }
 };
-const Tables& tab()
-{
-    static const Tables t;
-    return t;
-}
+// Built once at load time: the coding loops read it without a
+// first-use guard on every binary decision.
+const Tables g_tables;
+inline const Tables& tab() { return g_tables; }
 
 struct Pred {
     uint16_t p;  // P(bit == 0), 16-bit

@@ -42,9 +42,14 @@ public:
         range_ -= r;
         if (range_ < 0x10000u) { range_ <<= 16; shift_low(); }
     }
+    // branch-free on the (unpredictable) bit: a mask selects the update
     inline int code(int bit, int p)
     {
-        if (bit) bit1(p); else bit0(p);
+        const uint32_t r = (range_ >> 12) * (uint32_t)p;
+        const uint32_t m = 0u - (uint32_t)(bit != 0);
+        low_ += r & m;
+        range_ = (r & ~m) | ((range_ - r) & m);
+        if (range_ < 0x10000u) { range_ <<= 16; shift_low(); }
         return bit;
     }
     void put_bit(int bit) { code(bit, 2048); }
@@ -118,20 +123,15 @@ public:
     inline int code(int /*unused*/, int p)
     {
         const uint32_t r = (range_ >> 12) * (uint32_t)p;
-        int bit;
-        if (code_ >= r) {
-            code_ -= r;
-            range_ -= r;
-            bit = 1;
-        } else {
-            range_ = r;
-            bit = 0;
-        }
+        const uint32_t bit = code_ >= r ? 1u : 0u;
+        const uint32_t m = 0u - bit;  // branch-free, as in the encoder
+        code_ -= r & m;
+        range_ = (r & ~m) | ((range_ - r) & m);
         if (range_ < 0x10000u) {
             range_ <<= 16;
             code_ = (code_ << 16) | in16();
         }
-        return bit;
+        return (int)bit;
     }
     int get_bit() { return code(0, 2048); }
     uint32_t get_word()
