@@ -63,6 +63,20 @@ struct GroupLists {
     uint32_t* overflow;   // set if the pool would overflow (cannot happen by construction)
 };
 
+// Occupancy caps of the random-access kernels, as extra (unused) LDS per
+// workgroup: beyond a few workgroups per CU their scattered 4-byte loads
+// and stores only thrash the caches (swept on the MI355X; QZBSC_LDS_<name>
+// overrides one, in bytes).
+enum OccKernel { kOccSortSmall, kOccGatherSmall, kOccFinal0, kOccBwtOut, kOccFinalLarge, kOccCount };
+constexpr uint32_t kOccDefault[kOccCount] = {16384, 0, 0, 0, 0};
+static const char* const kOccEnv[kOccCount] = {"QZBSC_LDS_SORT_SMALL", "QZBSC_LDS_GATHER_SMALL", "QZBSC_LDS_FINAL0",
+                                               "QZBSC_LDS_BWT_OUT", "QZBSC_LDS_FINAL_LARGE"};
+static uint32_t occ_lds(OccKernel k)
+{
+    if (const char* e = getenv(kOccEnv[k])) return (uint32_t)std::min(32768L, std::max(0L, atol(e)));
+    return kOccDefault[k];
+}
+
 // Block of a batch position: off[] holds the block offsets; tab[k] is the
 // block holding position k << kTabShift (tab has one entry more than there
 // are buckets, the last one nblk - 1), so a lookup searches only between the
@@ -909,15 +923,16 @@ __global__ void k_bwt_index(const uint32_t* __restrict__ rank, BlockMap bm,
 }
 
 // ---- digit-width autotuner -------------------------------------------------
-// The two radix sorts (round 0 and large groups) run with 8- or 11-bit
-// digits: 11 bits saves a pass on 32-bit keys, 8 bits keeps digit runs long
-// enough for coalesced stores. Both orders are identical, so the tuner
-// alternates the two on the first sorts of at least kTuneMin elements, times
-// them with HIP events, and keeps the faster per element for the process.
-// QZBSC_RADIX_BITS=8|11 forces a width.
-// Candidate digit widths: 9 bits serves the large-group sort, whose block-
-// local keys need at most 27 bits for blocks of up to 128 MiB (3 passes
-// where 8-bit digits take 4); round 0's 32-bit keys gain nothing from it.
+// The two radix sorts (round 0 and large groups) choose their digit width:
+// 11 bits saves a pass on 32-bit keys, 8 bits keeps digit runs long enough
+// for coalesced stores, and 9 bits serves the large-group sort, whose
+// block-local keys need at most 27 bits for blocks of up to 128 MiB (3
+// passes where 8-bit digits take 4; round 0's 32-bit keys gain nothing from
+// it). Every width gives the same order, so the tuner rotates the candidates
+// ({8, 11} for round 0, {8, 9, 11} for large groups) on the first sorts of at
+// least kTuneMin elements, times them with HIP events, and keeps the fastest
+// per element for the process. QZBSC_RADIX_BITS=8|9|11 forces one width for
+// both sorts.
 constexpr int kWidths[3] = {8, 9, 11};
 struct RadixTuner {
     std::mutex mu;
@@ -1148,7 +1163,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
     const GroupLists gl = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_);
-    hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, (const TileDesc*)tiles0_, bufA_, sa_,
+    hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals, (const TileDesc*)tiles0_, bufA_, sa_,
                        rank_, gl);
     QZ_HIP(hipGetLastError());
     return 0;
@@ -1160,6 +1175,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     const BlockMap bmap_{off_, btab_, nblk};
     rounds_ = 0;
     trace_ = getenv("QZBSC_TRACE") != nullptr;
+
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
     uint32_t* overflow = cnt_ + kOverflow;
     // emission targets: the pool with counter set A or B, alternating
@@ -1311,7 +1327,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
 
         // phase A: gather every key of the round before any rank changes
         if (small_waves) {
-            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), 0, s, cl, plan, sa_, rank_, bmap_,
+            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), occ_lds(kOccGatherSmall), s, cl, plan, sa_, rank_, bmap_,
                                 h, bufB_);
             QZ_HIP(hipGetLastError());
         }
@@ -1332,7 +1348,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
         if (small_waves) {
-            hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), 0, s, cl, plan, bufB_, sa_, rank_, N);
+            hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), occ_lds(kOccSortSmall), s, cl, plan,
+                               bufB_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
@@ -1361,7 +1378,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             hipLaunchKernelGGL(k_tile_lasthead, dim3(lt), dim3(kThreads), 0, s, keys, ltiles, bufA_);
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
-            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
+            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), occ_lds(kOccFinalLarge), s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
                                bufA_, sa_, rank_, N);
             QZ_HIP(hipGetLastError());
         }
@@ -1379,7 +1396,7 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
     build_block_tab(offs_host_, n, btab_host_);
     QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
     QZ_TRY(sort_suffixes(text, n, nblk, s));
-    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), 0, s, text, sa_, rank_, bmap_, n,
+    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text, sa_, rank_, bmap_, n,
                        out);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
