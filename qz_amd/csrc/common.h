@@ -58,6 +58,18 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t mask)
 }
 
 namespace qz {
+// Workgroups go to the 8 XCDs of the MI355X round-robin by workgroup id, and
+// each XCD has its own L2. xcd_item maps workgroup `bid` of a grid of n to an
+// item index so that every XCD takes one contiguous range of items:
+// neighbouring tiles, which share histogram and output lines, then meet in
+// one L2 instead of eight. A bijection on [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_item(uint32_t bid, uint32_t n)
+{
+    const uint32_t x = bid % kXcds, j = bid / kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // QZBSC_TRACE_MEM=1 prints every workspace reallocation and its duration.
 struct MemTrace {
     const char* what;

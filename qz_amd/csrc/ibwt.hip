@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict
                                                         uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t cnt[4][256];
-    const TileDesc td = tiles[blockIdx.x];
+    const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
     for (int w = 0; w < 4; ++w) cnt[w][tid] = 0;
     __syncthreads();
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
     constexpr int W = kIThreads / QZ_WAVE;
     __shared__ uint32_t run[256];
     __shared__ uint32_t wh[W][256];
-    const TileDesc td = tiles[blockIdx.x];
+    const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
     const uint32_t ob = off[td.block];
     const uint32_t p = prim[td.block];

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
     constexpr uint32_t D = 1u << BITS;
     constexpr int W = kSortThreads / QZ_WAVE;
     __shared__ uint32_t cnt[W][D];
-    const TileDesc td = tiles[blockIdx.x];
+    const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
     for (uint32_t d = tid; d < D; d += kSortThreads)
 #pragma unroll
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     __shared__ uint32_t lstart[D];
     __shared__ uint32_t gbase[D];
     __shared__ uint32_t wtot[W];
-    const TileDesc td = tiles[blockIdx.x];
+    const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
     for (uint32_t d = tid; d < D; d += kSortThreads) {
 #pragma unroll
