@@ -64,6 +64,24 @@ QZBSC_API int bsc_block_info(const unsigned char* blockHeader, int headerSize, i
 QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
                              int features);
 
+/* Batch forms of bsc_compress / bsc_decompress over the same block format.
+ * Blocks lie end to end: block b of bsc_compress_batch starts at input +
+ * sum of the earlier sizes and writes at most sizes[b] + 28 + 1024 bytes at
+ * output + sum over c < b of (sizes[c] + 28 + 1024), its size (or a negative
+ * code) in out_sizes[b]; block b of qzbsc_decompress_batch (in_sizes[b]
+ * bytes) decodes into output + sum of the earlier out_sizes, out_sizes[b]
+ * >= its data size, its code in rcs[b]. One device batch (up to 1 GiB)
+ * serves all blocks; the host coder runs on one thread per block. Both
+ * return 0 or the first block's error. No counterpart in the reference:
+ * they replace qz's rayon fan-out of per-block calls over a stream's
+ * blocks (mod.rs:388-413 compress_stream_to_bsc_blocks; the batches of 8
+ * blocks of decompress_impl.rs:97-121). */
+QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes, int nblocks, unsigned char* output,
+                                   int* out_sizes, int lzpHashSize, int lzpMinLen, int blockSorter, int coder,
+                                   int features);
+QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_sizes, int nblocks,
+                                     unsigned char* output, const int* out_sizes, int features, int* rcs);
+
 /* ---- Part 2: BWT stages ------------------------------------------------ */
 
 /* libbsc bwt.h bsc_bwt_encode (called inside libbsc's bsc_compress, which

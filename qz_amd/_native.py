@@ -23,6 +23,9 @@ _SIGS = {
     "bsc_compress": (C.c_int, [u8p, u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bsc_block_info": (C.c_int, [u8p, C.c_int, i32p, i32p, C.c_int]),
     "bsc_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
+    "qzbsc_compress_batch": (C.c_int, [u8p, i32p, C.c_int, u8p, i32p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int]),
+    "qzbsc_decompress_batch": (C.c_int, [u8p, i32p, C.c_int, u8p, i32p, C.c_int, i32p]),
     "qzbsc_bwt_encode": (C.c_int, [u8p, C.c_int, u8p, i32p, C.c_int]),
     "qzbsc_bwt_decode": (C.c_int, [u8p, C.c_int, C.c_int, C.c_uint8, i32p, C.c_int]),
     "qzbsc_bwt_encode_batch": (C.c_int, [u8p, u8p, i32p, C.c_int, i32p, i32p]),

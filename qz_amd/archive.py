@@ -1,0 +1,466 @@
+"""The .qz v2 archive around the BSC streams: qz's default container,
+written and read with this library's block coder.
+
+An archive is an 8-byte prefix (magic "QZ", version 2, a reserved byte, the
+u32 header size), a 52-byte header body (+8 when constant lengths are
+recorded), then the header, sequence, N-mask and quality streams, each in
+the BSC multi-block framing [u32 num_blocks][u32 len, block]... (an empty
+stream takes no bytes at all).
+
+Writer, following crates/qz-lib/src/compression:
+- records_to_streams (mod.rs:280-320): header stream = varint(len) + id per
+  record (the id keeps its '@'); sequence stream = the bases, each prefixed
+  by varint(len) unless all reads share one length; quality stream = each
+  record's qualities as 7-bit Phred codes packed LSB first and flushed to a
+  byte boundary per record (columnar.rs:103-134, QualityBinning::None),
+  varint(len)-prefixed unless all share one length;
+- detect_const_lengths (mod.rs:329-354);
+- compress_stream_to_bsc_blocks (mod.rs:388-413): 25 MiB pieces, each one
+  adaptive-QLFC BSC block without LZP;
+- write_archive_header (mod.rs:430-482) and the stream lengths
+  (compress_impl.rs:757-760: 4 + the framed blocks, 0 when no block).
+
+Reader, following decompress_impl.rs: the header parse of
+decompress_streaming_bsc (275-335), the stream walk of
+stream_decompressor_inner (72-135: blocks in batches of 8, a block claiming
+more than 64 MB is rejected), the offset scans (444-496) and the record
+assembly of reconstruct_records_parallel (504-636: "id\\nseq\\n+\\nqual\\n").
+
+This build covers the container qz writes by default for the BSC path:
+encoding 0, lossless 7-bit qualities (binning code 0), and BSC for the
+quality, sequence and header streams (compressor codes 1/1/1). Archives
+using other encodings or codecs raise ArchiveError.
+
+Blocks go through the batch C-ABI (bsc.compress_batch /
+bsc.decompress_batch): all blocks of a stream share one device batch for
+their BWTs, where qz fans out per-block libbsc calls over rayon. The three
+streams decode concurrently, as qz's three decompressor threads do.
+"""
+from __future__ import annotations
+
+import struct
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import bsc
+
+ARCHIVE_MAGIC = b"QZ"
+ARCHIVE_VERSION = 2
+V2_PREFIX_SIZE = 8
+BODY_SIZE = 52                # 9 + 2 + 1 + 40 (mod.rs:446)
+CONST_LEN_SIZE = 8
+DECOMPRESS_BATCH_SIZE = 8     # mod.rs:30
+MAX_BLOCK_BYTES = 64 * 1024 * 1024  # decompress_impl.rs:105
+BITS_PER_QUAL = 7             # QualityBinning::None (columnar.rs:34)
+
+ENCODING_PLAIN = 0
+BINNING_LOSSLESS = 0
+COMPRESSOR_BSC = 1
+
+
+class ArchiveError(ValueError):
+    """A malformed archive, or one using a feature this build does not read."""
+
+
+@dataclass
+class FastqRecord:
+    id: bytes          # header line including the leading '@'
+    sequence: bytes
+    quality: bytes
+
+
+@dataclass
+class ArchiveHeader:
+    version: int
+    header_size: int
+    encoding_type: int
+    flags: int
+    quality_binning: int
+    quality_compressor: int
+    sequence_compressor: int
+    header_compressor: int
+    num_reads: int
+    headers_len: int
+    sequences_len: int
+    nmasks_len: int
+    qualities_len: int
+    const_seq_len: int
+    const_qual_len: int
+
+    @property
+    def data_offset(self) -> int:
+        return self.header_size
+
+
+# ---- varints and quality packing -------------------------------------------
+
+def write_varint(out: bytearray, value: int) -> None:
+    """LEB128, 7 bits per byte, least significant first (dna_utils.rs:285-291)."""
+    while value >= 0x80:
+        out.append((value & 0x7F) | 0x80)
+        value >>= 7
+    out.append(value)
+
+
+def read_varint(buf, pos: int):
+    """Returns (value, next position); raises ArchiveError when truncated."""
+    value = shift = 0
+    while True:
+        if pos >= len(buf):
+            raise ArchiveError("truncated varint")
+        b = buf[pos]
+        pos += 1
+        value |= (b & 0x7F) << shift
+        if not b & 0x80:
+            return value, pos
+        shift += 7
+        if shift > 63:
+            raise ArchiveError("varint too long")
+
+
+def packed_len(nqual: int) -> int:
+    return (nqual * BITS_PER_QUAL + 7) // 8
+
+
+def _codes_to_packed(codes: np.ndarray) -> np.ndarray:
+    """(N, L) 7-bit codes -> (N, packed_len(L)) bytes, LSB first per record."""
+    n, length = codes.shape
+    bits = ((codes[:, :, None] >> np.arange(BITS_PER_QUAL, dtype=np.uint8)) & 1).reshape(n, length * BITS_PER_QUAL)
+    pad = packed_len(length) * 8 - length * BITS_PER_QUAL
+    if pad:
+        bits = np.concatenate([bits, np.zeros((n, pad), np.uint8)], axis=1)
+    return np.packbits(bits, axis=1, bitorder="little")
+
+
+def _packed_to_codes(packed: np.ndarray, length: int) -> np.ndarray:
+    """(N, packed_len(L)) bytes -> (N, L) 7-bit codes."""
+    n = packed.shape[0]
+    bits = np.unpackbits(packed, axis=1, bitorder="little")[:, :length * BITS_PER_QUAL]
+    bits = bits.reshape(n, length, BITS_PER_QUAL)
+    return (bits << np.arange(BITS_PER_QUAL, dtype=np.uint8)).sum(axis=2, dtype=np.uint16).astype(np.uint8)
+
+
+def pack_qualities(qual: bytes) -> bytes:
+    """One record: Phred+33 ASCII -> min(q - 33, 127) (0 below '!') as 7-bit
+    codes, packed LSB first (columnar.rs:103-134 with QualityBinning::None,
+    encode = phred.min(127))."""
+    if not qual:
+        return b""
+    q = np.frombuffer(qual, dtype=np.uint8).astype(np.int16)
+    codes = np.clip(q - 33, 0, 127).astype(np.uint8)
+    return _codes_to_packed(codes[None, :])[0].tobytes()
+
+
+def unpack_qualities(packed: bytes, nqual: int) -> bytes:
+    """Inverse of pack_qualities: codes + 33 (columnar.rs:167-207)."""
+    if nqual == 0:
+        return b""
+    a = np.frombuffer(packed, dtype=np.uint8)
+    if a.size < packed_len(nqual):
+        raise ArchiveError("truncated quality record")
+    return (_packed_to_codes(a[None, :packed_len(nqual)], nqual)[0] + 33).tobytes()
+
+
+# ---- FASTQ text ---------------------------------------------------------------
+
+def parse_fastq(data: bytes) -> list:
+    """Four-line FASTQ records (id line starting with '@', '+' separator)."""
+    lines = data.split(b"\n")
+    if lines and lines[-1] == b"":
+        lines.pop()
+    if len(lines) % 4:
+        raise ArchiveError("FASTQ: line count is not a multiple of 4")
+    recs = []
+    for i in range(0, len(lines), 4):
+        h, s, p, q = lines[i:i + 4]
+        if not h.startswith(b"@") or not p.startswith(b"+"):
+            raise ArchiveError(f"FASTQ: malformed record {i // 4}")
+        recs.append(FastqRecord(h, s, q))
+    return recs
+
+
+def format_fastq(records) -> bytes:
+    """decompress_impl.rs:726-748: id, sequence, '+', quality, one per line."""
+    parts = []
+    for r in records:
+        parts += [r.id, b"\n", r.sequence, b"\n+\n", r.quality, b"\n"]
+    return b"".join(parts)
+
+
+# ---- writer -----------------------------------------------------------------
+
+def detect_const_lengths(records):
+    """mod.rs:329-354: (const_seq_len, const_qual_len), 0 meaning variable."""
+    if not records:
+        return 0, 0
+    sl = len(records[0].sequence)
+    ql = len(records[0].quality)
+    cs = sl if all(len(r.sequence) == sl for r in records) else 0
+    cq = ql if ql and all(len(r.quality) == ql for r in records) else 0
+    return cs, cq
+
+
+def records_to_streams(records, const_seq_len: int, const_qual_len: int):
+    """mod.rs:280-320 for encoding 0 with lossless qualities: returns the
+    header, sequence and quality streams."""
+    h = bytearray()
+    for r in records:
+        write_varint(h, len(r.id))
+        h += r.id
+    if const_seq_len:
+        s = b"".join(r.sequence for r in records)
+    else:
+        sb = bytearray()
+        for r in records:
+            write_varint(sb, len(r.sequence))
+            sb += r.sequence
+        s = bytes(sb)
+    if const_qual_len:
+        if records:
+            q = np.frombuffer(b"".join(r.quality for r in records), dtype=np.uint8)
+            codes = np.clip(q.astype(np.int16) - 33, 0, 127).astype(np.uint8).reshape(len(records), const_qual_len)
+            qs = _codes_to_packed(codes).tobytes()
+        else:
+            qs = b""
+    else:
+        qb = bytearray()
+        for r in records:
+            write_varint(qb, len(r.quality))
+            qb += pack_qualities(r.quality)
+        qs = bytes(qb)
+    return bytes(h), s, qs
+
+
+def archive_header(num_reads: int, headers_len: int, sequences_len: int, qualities_len: int,
+                   const_seq_len: int = 0, const_qual_len: int = 0) -> bytes:
+    """mod.rs:430-482 for encoding 0, lossless qualities and BSC for all three
+    streams: prefix, body, and the constant lengths when either is set."""
+    has_const = const_seq_len > 0 or const_qual_len > 0
+    header_size = V2_PREFIX_SIZE + BODY_SIZE + (CONST_LEN_SIZE if has_const else 0)
+    out = bytearray(ARCHIVE_MAGIC)
+    out += bytes([ARCHIVE_VERSION, 0])
+    out += struct.pack("<I", header_size)
+    out += bytes([ENCODING_PLAIN, 0x02 if has_const else 0x00, BINNING_LOSSLESS, COMPRESSOR_BSC, COMPRESSOR_BSC,
+                  COMPRESSOR_BSC, 0, 0, 0])  # quality model / delta / dictionary disabled
+    out += struct.pack("<H", 0)              # template prefix length
+    out += bytes([0])                        # no comment
+    out += struct.pack("<QQQQQ", num_reads, headers_len, sequences_len, 0, qualities_len)
+    if has_const:
+        out += struct.pack("<II", const_seq_len, const_qual_len)
+    return bytes(out)
+
+
+def compress_stream(data: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
+    """One stream in the multi-block framing; b"" for an empty stream."""
+    if not data:
+        return b""
+    a = np.frombuffer(data, dtype=np.uint8)
+    pieces = [a[i:i + block_size] for i in range(0, a.size, block_size)]
+    blocks = bsc.compress_batch(pieces)
+    parts = [struct.pack("<I", len(blocks))]
+    for b in blocks:
+        parts += [struct.pack("<I", len(b)), b]
+    return b"".join(parts)
+
+
+def compress_records(records, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
+    cs, cq = detect_const_lengths(records)
+    h, s, q = records_to_streams(records, cs, cq)
+    hz, sz, qz = (compress_stream(x, block_size) for x in (h, s, q))
+    return archive_header(len(records), len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
+
+
+def compress_fastq(fastq: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
+    """FASTQ text -> .qz v2 archive bytes."""
+    return compress_records(parse_fastq(fastq), block_size)
+
+
+# ---- reader -----------------------------------------------------------------
+
+def parse_archive_header(buf) -> ArchiveHeader:
+    """decompress_impl.rs:285-335, plus the checks that the header and the
+    streams fit the buffer."""
+    mv = memoryview(buf)
+    if len(mv) < V2_PREFIX_SIZE + BODY_SIZE:
+        raise ArchiveError("archive shorter than its header")
+    if bytes(mv[0:2]) != ARCHIVE_MAGIC:
+        raise ArchiveError("Not a QZ archive (missing magic bytes)")
+    version = mv[2]
+    if version > ARCHIVE_VERSION:
+        raise ArchiveError(f"Archive version {version} is newer than this build supports (max {ARCHIVE_VERSION})")
+    (header_size,) = struct.unpack_from("<I", mv, 4)
+    b = V2_PREFIX_SIZE
+    flags = mv[b + 1]
+    num_reads, hl, sl, nl, ql = struct.unpack_from("<QQQQQ", mv, b + 12)
+    cs = cq = 0
+    if flags & 0x02:
+        if len(mv) < b + BODY_SIZE + CONST_LEN_SIZE:
+            raise ArchiveError("archive shorter than its constant-length fields")
+        cs, cq = struct.unpack_from("<II", mv, b + BODY_SIZE)
+    h = ArchiveHeader(version=version, header_size=header_size, encoding_type=mv[b], flags=flags,
+                      quality_binning=mv[b + 2], quality_compressor=mv[b + 3], sequence_compressor=mv[b + 4],
+                      header_compressor=mv[b + 5], num_reads=num_reads, headers_len=hl, sequences_len=sl,
+                      nmasks_len=nl, qualities_len=ql, const_seq_len=cs, const_qual_len=cq)
+    if header_size < b + BODY_SIZE or header_size > len(mv):
+        raise ArchiveError(f"bad header size {header_size}")
+    if header_size + hl + sl + nl + ql > len(mv):
+        raise ArchiveError("archive truncated: streams extend past the end")
+    return h
+
+
+def stream_offsets(h: ArchiveHeader):
+    """(offset, length) of the header, sequence and quality streams
+    (decompress_impl.rs:355-359: N-masks lie between sequences and qualities)."""
+    ho = h.data_offset
+    so = ho + h.headers_len
+    qo = so + h.sequences_len + h.nmasks_len
+    return (ho, h.headers_len), (so, h.sequences_len), (qo, h.qualities_len)
+
+
+def stream_blocks(buf, offset: int, length: int) -> list:
+    """The compressed blocks of one framed stream (stream_decompressor_inner,
+    decompress_impl.rs:81-113), checked to fill exactly `length` bytes."""
+    if length == 0:
+        return []
+    mv = memoryview(buf)
+    end = offset + length
+    if length < 4 or end > len(mv):
+        raise ArchiveError("truncated stream")
+    (nb,) = struct.unpack_from("<I", mv, offset)
+    p = offset + 4
+    blocks = []
+    for _ in range(nb):
+        if p + 4 > end:
+            raise ArchiveError("truncated block length")
+        (ln,) = struct.unpack_from("<I", mv, p)
+        p += 4
+        if ln > MAX_BLOCK_BYTES:
+            raise ArchiveError(f"BSC block claims {ln} bytes (max 64 MB) - archive may be corrupt")
+        if p + ln > end:
+            raise ArchiveError("truncated block data")
+        blocks.append(mv[p:p + ln])
+        p += ln
+    if p != end:
+        raise ArchiveError("stream length does not match its blocks")
+    return blocks
+
+
+def decompress_stream(buf, offset: int, length: int, batch: int = DECOMPRESS_BATCH_SIZE) -> bytes:
+    """All blocks of one stream, decoded `batch` blocks per device batch."""
+    blocks = stream_blocks(buf, offset, length)
+    out = []
+    for i in range(0, len(blocks), batch):
+        out += bsc.decompress_batch(blocks[i:i + batch])
+    return b"".join(out)
+
+
+def _check_supported(h: ArchiveHeader) -> None:
+    if h.encoding_type != ENCODING_PLAIN:
+        raise ArchiveError(f"encoding type {h.encoding_type} is not read by this build")
+    if h.quality_binning != BINNING_LOSSLESS:
+        raise ArchiveError(f"quality binning {h.quality_binning} is not read by this build")
+    if h.qualities_len and h.quality_compressor != COMPRESSOR_BSC:
+        raise ArchiveError(f"quality compressor {h.quality_compressor} is not read by this build")
+    if h.sequence_compressor != COMPRESSOR_BSC or h.header_compressor != COMPRESSOR_BSC:
+        raise ArchiveError("only BSC sequence and header streams are read by this build")
+    if h.nmasks_len:
+        raise ArchiveError("N-mask streams are not read by this build")
+
+
+def decompress_streams(buf, batch: int = DECOMPRESS_BATCH_SIZE):
+    """(header, raw streams h, s, q): the three streams decode concurrently."""
+    h = parse_archive_header(buf)
+    _check_supported(h)
+    spans = stream_offsets(h)
+    res = [None, None, None]
+    err = [None]
+
+    def work(k):
+        try:
+            res[k] = decompress_stream(buf, spans[k][0], spans[k][1], batch)
+        except Exception as e:  # re-raised on the caller's thread
+            err[0] = e
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    if err[0] is not None:
+        raise err[0]
+    return h, res[0], res[1], res[2]
+
+
+def _scan_lengths(data: bytes, count: int, what: str):
+    """Varint-prefixed records: (starts, lengths) (decompress_impl.rs:444-471)."""
+    starts = np.empty(count, np.int64)
+    lens = np.empty(count, np.int64)
+    pos = 0
+    for i in range(count):
+        try:
+            ln, pos = read_varint(data, pos)
+        except ArchiveError:
+            raise ArchiveError(f"truncated {what} stream at record {i}") from None
+        starts[i] = pos
+        lens[i] = ln
+        pos += ln if what != "quality" else packed_len(ln)
+    if pos > len(data):
+        raise ArchiveError(f"truncated {what} stream")
+    return starts, lens
+
+
+def streams_to_records(h: ArchiveHeader, hs: bytes, ss: bytes, qs: bytes) -> list:
+    """reconstruct_records_parallel (decompress_impl.rs:504-636) for
+    encoding 0: records from the decoded streams."""
+    n = h.num_reads
+    hst, hln = _scan_lengths(hs, n, "header")
+    if h.const_seq_len:
+        L = h.const_seq_len
+        if len(ss) < n * L:
+            raise ArchiveError("truncated sequence stream")
+        seqs = [ss[i * L:(i + 1) * L] for i in range(n)]
+    else:
+        sst, sln = _scan_lengths(ss, n, "sequence")
+        seqs = [ss[int(a):int(a + b)] for a, b in zip(sst, sln)]
+    if h.qualities_len == 0:
+        quals = [b""] * n
+    elif h.const_qual_len:
+        L = h.const_qual_len
+        P = packed_len(L)
+        if len(qs) < n * P:
+            raise ArchiveError("truncated quality stream")
+        if n:
+            codes = _packed_to_codes(np.frombuffer(qs, np.uint8)[:n * P].reshape(n, P), L) + 33
+            flat = codes.tobytes()
+            quals = [flat[i * L:(i + 1) * L] for i in range(n)]
+        else:
+            quals = []
+    else:
+        qst, qln = _scan_lengths(qs, n, "quality")
+        quals = [unpack_qualities(qs[int(a):int(a) + packed_len(int(b))], int(b)) for a, b in zip(qst, qln)]
+    return [FastqRecord(hs[int(a):int(a + b)], seqs[i], quals[i]) for i, (a, b) in enumerate(zip(hst, hln))]
+
+
+def decompress_archive(buf, batch: int = DECOMPRESS_BATCH_SIZE) -> bytes:
+    """.qz v2 archive bytes -> FASTQ text."""
+    h, hs, ss, qs = decompress_streams(buf, batch)
+    return format_fastq(streams_to_records(h, hs, ss, qs))
+
+
+def compress_file(fastq_path: str, archive_path: str) -> int:
+    with open(fastq_path, "rb") as f:
+        data = compress_fastq(f.read())
+    with open(archive_path, "wb") as f:
+        f.write(data)
+    return len(data)
+
+
+def decompress_file(archive_path: str, fastq_path: str) -> int:
+    """Returns the FASTQ size."""
+    with open(archive_path, "rb") as f:
+        text = decompress_archive(f.read())
+    with open(fastq_path, "wb") as f:
+        f.write(text)
+    return len(text)

@@ -100,6 +100,77 @@ def compress_adaptive_mt_no_lzp(data) -> bytes:
                                 LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_MULTITHREADING | LIBBSC_FEATURE_CUDA)
 
 
+def compress_batch(blocks, lzp_hash_size: int = 0, lzp_min_len: int = 0,
+                   block_sorter: int = LIBBSC_BLOCKSORTER_BWT, coder: int = LIBBSC_CODER_QLFC_ADAPTIVE,
+                   features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> list:
+    """Many blocks through one qzbsc_compress_batch call (one device batch,
+    host coder on one thread per block); the same bytes per block as
+    compress_with_params. Replaces the per-block par_iter of
+    compress_stream_to_bsc_blocks (mod.rs:388-413)."""
+    _ensure_initialized()
+    arrs = [_arr(b) for b in blocks]
+    if not arrs:
+        return []
+    for a in arrs:
+        if a.size > _I32_MAX:
+            raise BscError(LIBBSC_BAD_PARAMETER, f"BSC: input too large ({a.size} bytes, max {_I32_MAX})")
+    sizes = np.array([a.size for a in arrs], dtype=np.int32)
+    inp = np.concatenate(arrs) if len(arrs) > 1 else arrs[0]
+    caps = sizes.astype(np.int64) + LIBBSC_HEADER_SIZE + 1024
+    out = np.empty(int(caps.sum()), dtype=np.uint8)
+    osz = np.zeros(len(arrs), dtype=np.int32)
+    rc = lib().qzbsc_compress_batch(inp.ctypes.data_as(u8p), sizes.ctypes.data_as(i32p), len(arrs),
+                                    out.ctypes.data_as(u8p), osz.ctypes.data_as(i32p), lzp_hash_size, lzp_min_len,
+                                    block_sorter, coder, features)
+    if rc < 0:
+        raise BscError(rc, "BSC batch compression")
+    res, off = [], 0
+    for k in range(len(arrs)):
+        res.append(out[off:off + int(osz[k])].tobytes())
+        off += int(caps[k])
+    return res
+
+
+def decompress_batch(blocks, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> list:
+    """Many blocks through one qzbsc_decompress_batch call: host coder
+    decode per block in parallel, one device batch of inverse BWTs. Each
+    block is checked as bsc.rs:384-436 checks one (size, header, all three
+    checksums). Replaces the batches of 8 per-block decompress calls of
+    decompress_impl.rs:97-121."""
+    _ensure_initialized()
+    arrs = [_arr(b) for b in blocks]
+    if not arrs:
+        return []
+    dsz = np.zeros(len(arrs), dtype=np.int32)
+    for k, a in enumerate(arrs):
+        if a.size < LIBBSC_HEADER_SIZE:
+            raise BscError(LIBBSC_UNEXPECTED_EOB,
+                           f"BSC: compressed data too small (need at least {LIBBSC_HEADER_SIZE} bytes for header)")
+        bs = np.zeros(1, np.int32)
+        ds = np.zeros(1, np.int32)
+        rc = lib().bsc_block_info(a.ctypes.data_as(u8p), a.size, bs.ctypes.data_as(i32p), ds.ctypes.data_as(i32p),
+                                  features)
+        if rc != LIBBSC_NO_ERROR:
+            raise BscError(rc, "BSC block_info")
+        if ds[0] <= 0:
+            raise BscError(LIBBSC_DATA_CORRUPT, f"BSC: invalid decompressed size: {int(ds[0])}")
+        dsz[k] = ds[0]
+    isz = np.array([a.size for a in arrs], dtype=np.int32)
+    inp = np.concatenate(arrs) if len(arrs) > 1 else arrs[0]
+    out = np.empty(int(dsz.astype(np.int64).sum()), dtype=np.uint8)
+    rcs = np.zeros(len(arrs), dtype=np.int32)
+    rc = lib().qzbsc_decompress_batch(inp.ctypes.data_as(u8p), isz.ctypes.data_as(i32p), len(arrs),
+                                      out.ctypes.data_as(u8p), dsz.ctypes.data_as(i32p), features,
+                                      rcs.ctypes.data_as(i32p))
+    if rc != LIBBSC_NO_ERROR:
+        raise BscError(rc, "BSC batch decompression")
+    res, off = [], 0
+    for k in range(len(arrs)):
+        res.append(out[off:off + int(dsz[k])].tobytes())
+        off += int(dsz[k])
+    return res
+
+
 def _compress_parallel_with(data, block_compressor) -> bytes:
     """bsc.rs:257-301: [num_blocks u32][len u32, block]... even for one block."""
     a = _arr(data)

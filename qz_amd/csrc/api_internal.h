@@ -8,6 +8,4 @@ int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::v
                      int* aux, float* gpu_ms);
 int inv_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::vector<uint32_t>& offs,
                      const int* primary, float* gpu_ms);
-int coder_compress_front(QlfcFront& front, const uint8_t* dbwt, uint32_t block, int n, uint8_t* out, int cap,
-                         bool parallel, hipStream_t stream);
 }  // namespace qz

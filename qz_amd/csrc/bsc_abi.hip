@@ -29,6 +29,7 @@
 #include "../../include/qzbsc.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -164,27 +165,49 @@ int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parall
         out, cap);
 }
 
-// Coder payload from device front-end results (front.run() done on `dbwt`).
-int coder_compress_front(QlfcFront& front, const uint8_t* dbwt, uint32_t block, int n, uint8_t* out, int cap,
-                         bool parallel, hipStream_t stream)
+// Host copy of the front end's results for a whole batch (front.run() done on
+// the batch's BWT `dbwt` of n bytes): per run symbol, rank and length, per
+// segment symbol table and symbol count, and the BWT itself (stored
+// segments copy it).
+struct FrontHost {
+    std::vector<uint8_t> sym, rank, tables, bwt;
+    std::vector<uint32_t> len;
+    std::vector<uint16_t> nsym;
+};
+
+static int front_to_host(const QlfcFront& front, const uint8_t* dbwt, uint32_t n, FrontHost& fh, hipStream_t stream)
+{
+    const uint32_t R = front.runs_;  // runs of every segment, laid out in segment order
+    const size_t S = front.segs_.size();
+    fh.sym.resize(R);
+    fh.rank.resize(R);
+    fh.len.resize(R);
+    fh.tables.resize(S * 256);
+    fh.nsym.resize(S);
+    fh.bwt.resize(n);
+    if (R) {
+        QZ_HIP(hipMemcpyAsync(fh.sym.data(), front.rsym_, R, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(fh.rank.data(), front.rrank_, R, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(fh.len.data(), front.rlen_, (size_t)R * 4, hipMemcpyDeviceToHost, stream));
+    }
+    if (S) {
+        QZ_HIP(hipMemcpyAsync(fh.tables.data(), front.table_, S * 256, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(fh.nsym.data(), front.nsym_, S * 2, hipMemcpyDeviceToHost, stream));
+    }
+    QZ_HIP(hipMemcpyAsync(fh.bwt.data(), dbwt, n, hipMemcpyDeviceToHost, stream));
+    QZ_HIP(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// Coder payload of batch block `block` from the host copy of the front end.
+static int coder_compress_host(const QlfcFront& front, const FrontHost& fh, uint32_t block, int n, uint8_t* out,
+                               int cap, bool parallel)
 {
     std::vector<int> sidx;
     for (int i = 0; i < (int)front.segs_.size(); ++i)
         if (front.segs_[i].block == block) sidx.push_back(i);
     const int nb = (int)sidx.size();
     if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_GPU_ERROR;
-    const uint32_t r0 = front.segs_[sidx[0]].r0, r1 = front.segs_[sidx[nb - 1]].r1;
-    const uint32_t R = r1 - r0;
-    std::vector<uint8_t> sym(R), rank(R), tables((size_t)nb * 256);
-    std::vector<uint32_t> len(R);
-    std::vector<uint16_t> nsym(nb);
-    QZ_HIP(hipMemcpyAsync(sym.data(), front.rsym_ + r0, R, hipMemcpyDeviceToHost, stream));
-    QZ_HIP(hipMemcpyAsync(rank.data(), front.rrank_ + r0, R, hipMemcpyDeviceToHost, stream));
-    QZ_HIP(hipMemcpyAsync(len.data(), front.rlen_ + r0, (size_t)R * 4, hipMemcpyDeviceToHost, stream));
-    QZ_HIP(hipMemcpyAsync(tables.data(), front.table_ + (size_t)sidx[0] * 256, (size_t)nb * 256,
-                          hipMemcpyDeviceToHost, stream));
-    QZ_HIP(hipMemcpyAsync(nsym.data(), front.nsym_ + sidx[0], (size_t)nb * 2, hipMemcpyDeviceToHost, stream));
-    QZ_HIP(hipStreamSynchronize(stream));
     int size[qlfc::kMaxSegments], start[qlfc::kMaxSegments];
     for (int k = 0; k < nb; ++k) {
         size[k] = (int)front.segs_[sidx[k]].size;
@@ -195,19 +218,20 @@ int coder_compress_front(QlfcFront& front, const uint8_t* dbwt, uint32_t block, 
     for_segments(nb, parallel, [&](int k) {
         const auto& sg = front.segs_[sidx[k]];
         qlfc::SegRuns Rn;
-        Rn.sym = sym.data() + (sg.r0 - r0);
-        Rn.rank = rank.data() + (sg.r0 - r0);
-        Rn.len = len.data() + (sg.r0 - r0);
+        Rn.sym = fh.sym.data() + sg.r0;
+        Rn.rank = fh.rank.data() + sg.r0;
+        Rn.len = fh.len.data() + sg.r0;
         Rn.count = sg.r1 - sg.r0;
-        Rn.nsym = nsym[k];
-        memcpy(Rn.table, tables.data() + (size_t)k * 256, 256);
+        Rn.nsym = fh.nsym[sidx[k]];
+        memcpy(Rn.table, fh.tables.data() + (size_t)sidx[k] * 256, 256);
         segout[k].resize((size_t)sg.size + 64);
         res[k] = qlfc::encode_segment(Rn, (int)sg.size, segout[k].data(), (int)sg.size);
     });
     return assemble_payload(
         n, nb, size, segout, res,
         [&](int sg, uint8_t* dst) {
-            return hipMemcpy(dst, dbwt + start[sg], (size_t)size[sg], hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+            memcpy(dst, fh.bwt.data() + start[sg], (size_t)size[sg]);
+            return 0;
         },
         out, cap);
 }
@@ -358,6 +382,27 @@ int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, u
     return coder_decompress(input + kHeader, payload, bwt, bh.data_size, parallel);
 }
 
+// Runs fn(b) for b in [0, n) on up to hardware_concurrency host threads.
+template <class Fn>
+static void parallel_blocks(int n, Fn fn)
+{
+    const int T = std::min<int>(n, (int)std::max(1u, std::thread::hardware_concurrency()));
+    if (T <= 1) {
+        for (int b = 0; b < n; ++b) fn(b);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&] {
+            for (int b; (b = next.fetch_add(1)) < n;) fn(b);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Device batches of the batch entry points hold at most this many bytes.
+constexpr uint64_t kBatchBytes = 1ull << 30;
+
 // LZP blocks: the size m of the LZP data the BWT ran on (0 if the block
 // carries no LZP parameters), checked against the header.
 int lzp_data_size(const uint8_t* input, const BlockHeader& bh)
@@ -413,9 +458,11 @@ QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, in
     QZ_HIP(hipMemcpyAsync(sl.fwd.text_, data, (size_t)dn, hipMemcpyHostToDevice, sl.stream));
     QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, &primary, aux.data(), nullptr));
     QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
+    FrontHost fh;
+    QZ_TRY(front_to_host(sl.front, sl.fwd.out_, (uint32_t)dn, fh, sl.stream));
     const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
     return finish_block_with(input, n, dn, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
-        return coder_compress_front(sl.front, sl.fwd.out_, 0, dn, p, cap, par, sl.stream);
+        return coder_compress_host(sl.front, fh, 0, dn, p, cap, par);
     });
 }
 
@@ -462,6 +509,204 @@ QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned
         }
     }
     if (adler32(output, (size_t)bh.data_size) != bh.adler_data) return QZ_DATA_CORRUPT;
+    return LIBBSC_NO_ERROR;
+}
+
+// Batch form of bsc_compress. Block b (sizes[b] bytes) starts at input +
+// sum of the earlier sizes; its output goes to output + sum over c < b of
+// (sizes[c] + 28 + 1024), libbsc's per-block buffer contract, and its size
+// (or a negative libbsc code) to out_sizes[b]. One device batch (up to
+// kBatchBytes) sorts the suffixes of all its blocks, the front end runs on
+// the device over the batch, and the host codes the blocks in parallel.
+// Returns 0, or the first block's error.
+QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes, int nblocks, unsigned char* output,
+                                   int* out_sizes, int lzpHashSize, int lzpMinLen, int blockSorter, int coder,
+                                   int features)
+{
+    if (!input || !sizes || !output || !out_sizes || nblocks < 1) return QZ_BAD_PARAMETER;
+    std::vector<int> mode(nblocks), dn(nblocks);
+    std::vector<size_t> ioff(nblocks + 1, 0), ooff(nblocks + 1, 0);
+    for (int b = 0; b < nblocks; ++b) {
+        QZ_TRY(check_params(sizes[b], lzpHashSize, lzpMinLen, blockSorter, coder, &mode[b]));
+        ioff[b + 1] = ioff[b] + (size_t)sizes[b];
+        ooff[b + 1] = ooff[b] + (size_t)sizes[b] + kHeader + 1024;
+    }
+    (void)features;
+    // LZP first (host, per block); a block it does not shrink is coded without it
+    std::vector<std::vector<uint8_t>> lz(nblocks);
+    parallel_blocks(nblocks, [&](int b) {
+        const int n = sizes[b];
+        dn[b] = n;
+        if (n > kHeader && (mode[b] >> 8)) {
+            lz[b].resize((size_t)n);
+            const int m = lzp::encode(input + ioff[b], n, lz[b].data(), n, lzpHashSize, lzpMinLen);
+            if (m > 1) {
+                dn[b] = m;
+            } else {
+                lz[b].clear();
+                mode[b] &= 0xff;
+            }
+        }
+    });
+    bool device = false;
+    for (int b = 0; b < nblocks; ++b) {
+        if (sizes[b] <= kHeader) out_sizes[b] = store_block(input + ioff[b], output + ooff[b], sizes[b]);
+        else device = true;
+    }
+    if (!device) return LIBBSC_NO_ERROR;  // only stored blocks: no device needed
+    SlotGuard g;
+    if (!g.ok()) return QZ_GPU_ERROR;
+    Slot& sl = g.slot();
+    auto flush = [&](const std::vector<int>& list) -> int {
+        const int k = (int)list.size();
+        std::vector<uint32_t> offs(k + 1, 0);
+        for (int i = 0; i < k; ++i) offs[i + 1] = offs[i] + (uint32_t)dn[list[i]];
+        QZ_TRY(sl.fwd.reserve(offs[k], (uint32_t)k));
+        for (int i = 0; i < k; ++i) {
+            const int b = list[i];
+            const uint8_t* data = lz[b].empty() ? input + ioff[b] : lz[b].data();
+            QZ_HIP(hipMemcpyAsync(sl.fwd.text_ + offs[i], data, (size_t)dn[b], hipMemcpyHostToDevice, sl.stream));
+        }
+        std::vector<int> prim(k), aux((size_t)k * 256);
+        QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, prim.data(), aux.data(), nullptr));
+        QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
+        FrontHost fh;
+        QZ_TRY(front_to_host(sl.front, sl.fwd.out_, offs[k], fh, sl.stream));
+        parallel_blocks(k, [&](int i) {
+            const int b = list[i];
+            out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], prim[i],
+                                             aux.data() + (size_t)i * 256, output + ooff[b],
+                                             [&](uint8_t* p, int cap) {
+                                                 return coder_compress_host(sl.front, fh, (uint32_t)i, dn[b], p,
+                                                                            cap, false);
+                                             });
+        });
+        return 0;
+    };
+    std::vector<int> list;
+    uint64_t bytes = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        if (sizes[b] <= kHeader) continue;
+        if (!list.empty() && bytes + (uint64_t)dn[b] > kBatchBytes) {
+            QZ_TRY(flush(list));
+            list.clear();
+            bytes = 0;
+        }
+        list.push_back(b);
+        bytes += (uint64_t)dn[b];
+    }
+    if (!list.empty()) QZ_TRY(flush(list));
+    for (int b = 0; b < nblocks; ++b)
+        if (out_sizes[b] < 0) return out_sizes[b];
+    return LIBBSC_NO_ERROR;
+}
+
+// Batch form of bsc_decompress. Block b (in_sizes[b] bytes) starts at input
+// + sum of the earlier in_sizes and decodes into output + sum of the earlier
+// out_sizes, out_sizes[b] being at least its data size (bsc_block_info).
+// The host decodes every coder payload in parallel, then one device batch
+// (up to kBatchBytes) inverts all the BWTs; each block's result (0 or a
+// negative libbsc code) goes to rcs[b]. Returns 0, or the first block's
+// error.
+QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_sizes, int nblocks,
+                                     unsigned char* output, const int* out_sizes, int features, int* rcs)
+{
+    if (!input || !in_sizes || !output || !out_sizes || !rcs || nblocks < 1) return QZ_BAD_PARAMETER;
+    (void)features;
+    std::vector<size_t> ioff(nblocks + 1, 0), ooff(nblocks + 1, 0);
+    for (int b = 0; b < nblocks; ++b) {
+        if (in_sizes[b] < 0 || out_sizes[b] < 0) return QZ_BAD_PARAMETER;
+        ioff[b + 1] = ioff[b] + (size_t)in_sizes[b];
+        ooff[b + 1] = ooff[b] + (size_t)out_sizes[b];
+    }
+    std::vector<BlockHeader> bh(nblocks);
+    std::vector<std::vector<uint8_t>> tmp(nblocks);  // LZP blocks: the m-byte BWT domain
+    std::vector<int> need(nblocks, 0);               // BWT bytes to invert
+    std::vector<uint8_t*> bw(nblocks, nullptr);
+    parallel_blocks(nblocks, [&](int b) {
+        const uint8_t* in = input + ioff[b];
+        uint8_t* out = output + ooff[b];
+        int rc = parse_header(in, in_sizes[b], &bh[b]);
+        if (!rc && (in_sizes[b] < bh[b].block_size || out_sizes[b] < bh[b].data_size)) rc = QZ_UNEXPECTED_EOB;
+        const int m = rc ? 0 : lzp_data_size(in, bh[b]);
+        if (!rc && m < 0) rc = m;
+        if (!rc && m > 0) {
+            if (adler32(in + kHeader, (size_t)(bh[b].block_size - kHeader)) != bh[b].adler_body) {
+                rc = QZ_DATA_CORRUPT;
+            } else {
+                const int num_indexes = in[bh[b].block_size - 1];
+                const int payload = bh[b].block_size - kHeader - 4 - 1 - 4 * num_indexes;
+                tmp[b].resize((size_t)m);
+                if (payload < 1 || coder_decompress(in + kHeader + 4, payload, tmp[b].data(), m, false))
+                    rc = QZ_DATA_CORRUPT;
+                else if (m > 1) {
+                    bw[b] = tmp[b].data();
+                    need[b] = m;
+                }
+            }
+        } else if (!rc) {
+            const int r = decode_payload(in, in_sizes[b], bh[b], out, false);
+            if (r < 0) rc = r;
+            else if (r == 0 && bh[b].data_size > 1) {
+                bw[b] = out;
+                need[b] = bh[b].data_size;
+            }
+        }
+        rcs[b] = rc;
+    });
+    std::vector<std::vector<int>> batches(1);
+    uint64_t bytes = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        if (rcs[b] || !need[b]) continue;
+        if (bh[b].index < 1 || bh[b].index > need[b]) {  // a primary the inverse cannot take
+            rcs[b] = QZ_DATA_CORRUPT;
+            continue;
+        }
+        if (!batches.back().empty() && bytes + (uint64_t)need[b] > kBatchBytes) {
+            batches.emplace_back();
+            bytes = 0;
+        }
+        batches.back().push_back(b);
+        bytes += (uint64_t)need[b];
+    }
+    auto flush = [&](Slot& sl, const std::vector<int>& list) -> int {
+        const int k = (int)list.size();
+        std::vector<uint32_t> offs(k + 1, 0);
+        std::vector<int> prim(k);
+        for (int i = 0; i < k; ++i) {
+            offs[i + 1] = offs[i] + (uint32_t)need[list[i]];
+            prim[i] = bh[list[i]].index;
+        }
+        QZ_TRY(sl.inv.reserve(offs[k], (uint32_t)k));
+        for (int i = 0; i < k; ++i)
+            QZ_HIP(hipMemcpyAsync(sl.inv.U_ + offs[i], bw[list[i]], (size_t)need[list[i]], hipMemcpyHostToDevice,
+                                  sl.stream));
+        QZ_TRY(inv_batch_device(sl, sl.inv.U_, sl.inv.T_, offs, prim.data(), nullptr));
+        for (int i = 0; i < k; ++i)
+            QZ_HIP(hipMemcpyAsync(bw[list[i]], sl.inv.T_ + offs[i], (size_t)need[list[i]], hipMemcpyDeviceToHost,
+                                  sl.stream));
+        QZ_HIP(hipStreamSynchronize(sl.stream));
+        return 0;
+    };
+    if (!batches[0].empty()) {  // stored and single-byte blocks need no device
+        SlotGuard g;
+        if (!g.ok()) return QZ_GPU_ERROR;
+        for (const auto& list : batches) QZ_TRY(flush(g.slot(), list));
+    }
+    parallel_blocks(nblocks, [&](int b) {
+        if (rcs[b]) return;
+        uint8_t* out = output + ooff[b];
+        if (!tmp[b].empty()) {
+            const int lzpMin = (bh[b].mode >> 8) & 0xff, lzpHash = (bh[b].mode >> 16) & 0xff;
+            if (lzp::decode(tmp[b].data(), (int)tmp[b].size(), out, bh[b].data_size, lzpHash, lzpMin)) {
+                rcs[b] = QZ_DATA_CORRUPT;
+                return;
+            }
+        }
+        if (adler32(out, (size_t)bh[b].data_size) != bh[b].adler_data) rcs[b] = QZ_DATA_CORRUPT;
+    });
+    for (int b = 0; b < nblocks; ++b)
+        if (rcs[b]) return rcs[b];
     return LIBBSC_NO_ERROR;
 }
 

@@ -145,6 +145,25 @@ def config2_sequences(n_reads: int, seed: int = 2) -> np.ndarray:
     return sequence_stream(50_000_000, n_reads, 150, seed, 0.002, 0.0001)
 
 
+def fastq(n_reads: int, read_len: int, seed: int, genome_len: int = 1_000_000, sub_rate: float = 0.001,
+          variable: bool = False) -> bytes:
+    """FASTQ text from the same generators: Illumina-style names, reads of
+    read_len bases (0.01% N), binned qualities. variable=True trims each read
+    and its qualities to a random length in [read_len // 2, read_len]."""
+    seq = sequence_stream(genome_len, n_reads, read_len, seed, sub_rate, 0.0001).reshape(n_reads, read_len)
+    qual = quality_stream(n_reads, read_len, seed + 7).reshape(n_reads, read_len)
+    hs = header_stream(n_reads, seed + 3).tobytes()
+    rng = np.random.default_rng(seed + 11)
+    lens = rng.integers(read_len // 2, read_len + 1, n_reads) if variable else np.full(n_reads, read_len)
+    parts, p = [], 0
+    for i in range(n_reads):
+        ln = hs[p]  # one varint byte: names are < 128 bytes
+        L = int(lens[i])
+        parts += [hs[p + 1:p + 1 + ln], b"\n", seq[i, :L].tobytes(), b"\n+\n", qual[i, :L].tobytes(), b"\n"]
+        p += 1 + ln
+    return b"".join(parts)
+
+
 def cut_blocks(stream: np.ndarray, block: int = BLOCK):
     return [stream[i:i + block] for i in range(0, stream.size, block)]
 

@@ -1,0 +1,160 @@
+"""The .qz v2 container (qz_amd.archive) on the host, no GPU needed.
+
+Pins: the libbsc archive under tests/golden (a real qz archive, 50,000 reads
+of 101 bp): its 68 header bytes must be what archive_header writes for its
+field values, its three streams must frame as one block each, and each
+block's data size must be what records_to_streams gives for 50,000 records
+of 101 bases and 101 qualities (5,050,000 and 50,000 x 89 bytes: the 7-bit
+packing length). Varints and the bit packing are checked on hand-derived
+vectors; the stream layout on hand-assembled bytes.
+"""
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from qz_amd import archive as A
+from qz_amd import bsc
+
+GOLD = Path(__file__).resolve().parent / "golden" / "ref_ne50k.qz"
+
+
+def gold():
+    return GOLD.read_bytes()
+
+
+def test_fixture_header_fields_and_bytes():
+    g = gold()
+    h = A.parse_archive_header(g)
+    assert (h.version, h.header_size, h.encoding_type, h.flags) == (2, 68, 0, 0x02)
+    assert (h.quality_binning, h.quality_compressor, h.sequence_compressor, h.header_compressor) == (0, 1, 1, 1)
+    assert h.num_reads == 50_000
+    assert (h.headers_len, h.sequences_len, h.nmasks_len, h.qualities_len) == (203_692, 1_199_090, 0, 1_222_078)
+    assert (h.const_seq_len, h.const_qual_len) == (101, 101)
+    assert h.header_size + h.headers_len + h.sequences_len + h.qualities_len == len(g)
+    mine = A.archive_header(h.num_reads, h.headers_len, h.sequences_len, h.qualities_len, h.const_seq_len,
+                            h.const_qual_len)
+    assert mine == g[:68]
+
+
+def test_fixture_stream_framing_and_sizes():
+    g = gold()
+    h = A.parse_archive_header(g)
+    spans = A.stream_offsets(h)
+    blocks = [A.stream_blocks(g, o, n) for o, n in spans]
+    assert [len(b) for b in blocks] == [1, 1, 1]
+    assert [len(b[0]) for b in blocks] == [203_684, 1_199_082, 1_222_070]
+    data_sizes = [struct.unpack_from("<i", bytes(b[0]), 4)[0] for b in blocks]
+    # header stream: not derivable without decoding; the other two are
+    assert data_sizes[0] == 2_904_234
+    assert data_sizes[1] == h.num_reads * h.const_seq_len
+    assert data_sizes[2] == h.num_reads * A.packed_len(h.const_qual_len) == 4_450_000
+
+
+def test_varint_vectors():
+    for v, enc in [(0, b"\x00"), (1, b"\x01"), (127, b"\x7f"), (128, b"\x80\x01"), (300, b"\xac\x02"),
+                   (16384, b"\x80\x80\x01"), (2**32, b"\x80\x80\x80\x80\x10")]:
+        out = bytearray()
+        A.write_varint(out, v)
+        assert bytes(out) == enc
+        assert A.read_varint(enc + b"zz", 0) == (v, len(enc))
+    with pytest.raises(A.ArchiveError):
+        A.read_varint(b"\x80\x80", 0)
+
+
+def test_quality_packing_vectors():
+    # phred 40 ('I') then 2 ('#'): 40 | 2 << 7 = 296 -> bytes 0x28, 0x01
+    assert A.pack_qualities(b"I#") == b"\x28\x01"
+    # eight codes of 127 fill exactly 7 bytes of ones
+    assert A.pack_qualities(bytes([160] * 8)) == b"\xff" * 7
+    # one code: 7 bits in one byte; phred 0 -> 0
+    assert A.pack_qualities(b"!") == b"\x00"
+    assert A.pack_qualities(b"\"") == b"\x01"
+    assert A.packed_len(101) == 89 and A.packed_len(8) == 7 and A.packed_len(9) == 8
+    rng = np.random.default_rng(5)
+    for n in (1, 7, 8, 9, 101, 150, 251):
+        q = bytes(rng.integers(33, 161, n, dtype=np.uint8))
+        p = A.pack_qualities(q)
+        assert len(p) == A.packed_len(n)
+        assert A.unpack_qualities(p, n) == q
+    # out of the 7-bit range: clamped (columnar.rs encode = phred.min(127); ASCII below '!' saturates at 0)
+    assert A.unpack_qualities(A.pack_qualities(bytes([200, 20])), 2) == bytes([160, 33])
+
+
+def _records(n, rng, const=True):
+    recs = []
+    for i in range(n):
+        L = 12 if const else int(rng.integers(0, 20))
+        seq = bytes(rng.choice(list(b"ACGTN"), L).astype(np.uint8))
+        qual = bytes(rng.integers(35, 75, L, dtype=np.uint8))
+        recs.append(A.FastqRecord(b"@r%d/%d" % (i, L), seq, qual))
+    return recs
+
+
+def test_stream_layout_by_hand():
+    recs = [A.FastqRecord(b"@a", b"ACGT", b"II#!"), A.FastqRecord(b"@bb", b"GG", b"#I")]
+    cs, cq = A.detect_const_lengths(recs)
+    assert (cs, cq) == (0, 0)
+    h, s, q = A.records_to_streams(recs, cs, cq)
+    assert h == b"\x02@a\x03@bb"
+    assert s == b"\x04ACGT\x02GG"
+    assert q == b"\x04" + A.pack_qualities(b"II#!") + b"\x02" + A.pack_qualities(b"#I")
+    recs2 = [A.FastqRecord(b"@a", b"ACGT", b"II#!"), A.FastqRecord(b"@b", b"TTTT", b"!!!!")]
+    assert A.detect_const_lengths(recs2) == (4, 4)
+    h, s, q = A.records_to_streams(recs2, 4, 4)
+    assert s == b"ACGTTTTT"
+    assert q == A.pack_qualities(b"II#!") + A.pack_qualities(b"!!!!")
+
+
+@pytest.mark.parametrize("const", [True, False])
+def test_streams_to_records_roundtrip(const):
+    rng = np.random.default_rng(9)
+    recs = _records(300, rng, const)
+    cs, cq = A.detect_const_lengths(recs)
+    hs, ss, qs = A.records_to_streams(recs, cs, cq)
+    hdr = A.parse_archive_header(A.archive_header(len(recs), len(hs), len(ss), len(qs), cs, cq) +
+                                 hs + ss + qs)
+    back = A.streams_to_records(hdr, hs, ss, qs)
+    assert back == recs
+    text = A.format_fastq(recs)
+    assert A.parse_fastq(text) == recs
+
+
+def test_header_and_framing_errors():
+    g = gold()
+    with pytest.raises(A.ArchiveError):
+        A.parse_archive_header(b"XZ" + g[2:])
+    with pytest.raises(A.ArchiveError):
+        A.parse_archive_header(g[:2] + b"\x03" + g[3:])
+    with pytest.raises(A.ArchiveError):
+        A.parse_archive_header(g[:-1])  # streams run past the end
+    h = A.parse_archive_header(g)
+    o, n = A.stream_offsets(h)[1]
+    with pytest.raises(A.ArchiveError):
+        A.stream_blocks(g, o, n - 1)    # framing does not fill the stated length
+    bad = bytearray(g)
+    struct.pack_into("<I", bad, o + 4, 65 * 1024 * 1024)
+    with pytest.raises(A.ArchiveError):
+        A.stream_blocks(bytes(bad), o, n)
+    with pytest.raises(A.ArchiveError):
+        A.parse_fastq(b"@a\nAC\n+\n")
+
+
+def test_unsupported_features_are_rejected():
+    g = bytearray(gold())
+    g[8] = 4  # encoding type 4 (sequence hints)
+    with pytest.raises(A.ArchiveError):
+        A.decompress_streams(bytes(g))
+    g = bytearray(gold())
+    g[8 + 3] = 4  # quality_ctx quality stream
+    with pytest.raises(A.ArchiveError):
+        A.decompress_streams(bytes(g))
+
+
+def test_stored_blocks_batch_without_device():
+    """Blocks of <= 28 bytes are stored: the batch forms need no device."""
+    blocks = [b"", b"A", b"ACGT" * 7]
+    comp = bsc.compress_batch([np.frombuffer(b, np.uint8) for b in blocks])
+    assert [len(c) for c in comp] == [28 + len(b) for b in blocks]
+    assert bsc.decompress_batch(comp[1:]) == blocks[1:]

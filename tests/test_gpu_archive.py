@@ -1,0 +1,98 @@
+"""Batch block forms and the .qz v2 archive on the GPU.
+
+* qzbsc_compress_batch gives, block for block, the bytes bsc_compress gives
+  (default and LZP parameter sets, stored and tiny blocks included), and
+  qzbsc_decompress_batch returns what per-block bsc_decompress returns;
+* FASTQ -> archive -> FASTQ is the identity for constant and variable read
+  lengths, with streams of many blocks (several device batches of 8);
+* every block of an archive stream decodes alone to its part of the stream;
+* a corrupted block, a truncated archive and the libbsc archive under
+  tests/golden (whose QLFC payloads this library's coder does not read)
+  raise instead of returning data.
+
+No torch here: the library must initialise the device itself.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+from qz_amd import archive as A
+from qz_amd import bsc, synth
+from qz_amd.bwt import BscError
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks():
+    rng = np.random.default_rng(4)
+    return [
+        synth.sequence_stream(200_000, 2_000, 150, 7, 0.002),            # 300 KB of reads
+        synth.quality_stream(2_000, 150, 8),                               # qualities
+        synth.header_stream(5_000, 9),                                     # names
+        rng.integers(0, 256, 70_000, dtype=np.uint8),                      # incompressible: stored
+        np.frombuffer(b"ACGTACGTAC", np.uint8),                            # <= 28 bytes: stored
+        np.frombuffer(b"G", np.uint8),
+        np.frombuffer(b"ACGT" * 40_000, np.uint8),                         # long repeats
+    ]
+
+
+def test_batch_forms_match_per_block_calls():
+    blocks = _blocks()
+    print("batch forms:", [b.size for b in blocks], flush=True)
+    one = [bsc.compress_adaptive_no_lzp(b) for b in blocks]
+    many = bsc.compress_batch(blocks)
+    assert many == one
+    lz_one = [bsc.compress_adaptive(b) for b in blocks]
+    lz_many = bsc.compress_batch(blocks, 16, 128)
+    assert lz_many == lz_one
+    for comp in (many, lz_many):
+        back = bsc.decompress_batch(comp)
+        assert back == [b.tobytes() for b in blocks]
+        assert back == [bsc.decompress(c) for c in comp]
+
+
+@pytest.mark.parametrize("variable", [False, True])
+def test_archive_roundtrip(variable):
+    text = synth.fastq(30_000, 150, 21, variable=variable)
+    print("archive round trip: variable", variable, len(text), "bytes", flush=True)
+    arc = A.compress_records(A.parse_fastq(text), block_size=256 * 1024)
+    h = A.parse_archive_header(arc)
+    assert h.num_reads == 30_000
+    assert (h.const_seq_len, h.const_qual_len) == ((0, 0) if variable else (150, 150))
+    spans = A.stream_offsets(h)
+    nblocks = [len(A.stream_blocks(arc, o, n)) for o, n in spans]
+    assert max(nblocks) > A.DECOMPRESS_BATCH_SIZE  # several device batches per stream
+    assert A.decompress_archive(arc) == text
+    # each block alone gives its 256 KiB piece of the stream
+    cs, cq = A.detect_const_lengths(A.parse_fastq(text))
+    streams = A.records_to_streams(A.parse_fastq(text), cs, cq)
+    for (o, n), raw in zip(spans, streams):
+        blocks = A.stream_blocks(arc, o, n)
+        pieces = [bsc.decompress(bytes(b)) for b in blocks]
+        assert b"".join(pieces) == raw
+        assert all(len(p) == 256 * 1024 for p in pieces[:-1])
+
+
+def test_archive_corruption_and_truncation_fail():
+    text = synth.fastq(5_000, 150, 22)
+    arc = bytearray(A.compress_fastq(text))
+    h = A.parse_archive_header(bytes(arc))
+    o, n = A.stream_offsets(h)[1]
+    bad = bytearray(arc)
+    bad[o + 8 + 28 + 100] ^= 0x10  # inside the first sequence block's payload
+    with pytest.raises(BscError):
+        A.decompress_archive(bytes(bad))
+    with pytest.raises(A.ArchiveError):
+        A.decompress_archive(bytes(arc[:-10]))
+    bad = bytearray(arc)
+    struct.pack_into("<Q", bad, 8 + 12, h.num_reads + 1)  # more reads than the streams hold
+    with pytest.raises(A.ArchiveError):
+        A.decompress_archive(bytes(bad))
+
+
+def test_libbsc_archive_is_rejected():
+    from pathlib import Path
+    g = (Path(__file__).resolve().parent / "golden" / "ref_ne50k.qz").read_bytes()
+    with pytest.raises(BscError):
+        A.decompress_archive(g)
