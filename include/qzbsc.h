@@ -82,6 +82,24 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
 QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_sizes, int nblocks,
                                      unsigned char* output, const int* out_sizes, int features, int* rcs);
 
+/* FASTQ text <-> the header / sequence / quality streams of a .qz archive
+ * (host). qzbsc_fastq_to_streams first with hdr, seq, qual NULL: info[0..5]
+ * = records, constant sequence length, constant quality length (0 =
+ * variable), and the three stream sizes; then with buffers of those sizes
+ * and info unchanged, to write them. Replaces records_to_streams
+ * (crates/qz-lib/src/compression/mod.rs:280-320, encoding 0, lossless
+ * qualities) and detect_const_lengths (mod.rs:329-354). -1 for malformed
+ * FASTQ (a record needs four lines, '@' and '+').
+ * qzbsc_streams_to_fastq returns the FASTQ size (out may be NULL to size it)
+ * or -5 when a stream ends early. Replaces the offset scans and record
+ * assembly of decompress_impl.rs:444-636. */
+QZBSC_API long long qzbsc_fastq_to_streams(const unsigned char* fastq, long long size, unsigned char* hdr,
+                                           unsigned char* seq, unsigned char* qual, long long* info);
+QZBSC_API long long qzbsc_streams_to_fastq(const unsigned char* hs, long long hlen, const unsigned char* ss,
+                                           long long slen, const unsigned char* qs, long long qlen, long long n,
+                                           long long const_seq, long long const_qual, unsigned char* out,
+                                           long long cap);
+
 /* ---- Part 2: BWT stages ------------------------------------------------ */
 
 /* libbsc bwt.h bsc_bwt_encode (called inside libbsc's bsc_compress, which

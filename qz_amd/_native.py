@@ -26,6 +26,9 @@ _SIGS = {
     "qzbsc_compress_batch": (C.c_int, [u8p, i32p, C.c_int, u8p, i32p, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int]),
     "qzbsc_decompress_batch": (C.c_int, [u8p, i32p, C.c_int, u8p, i32p, C.c_int, i32p]),
+    "qzbsc_fastq_to_streams": (C.c_longlong, [u8p, C.c_longlong, u8p, u8p, u8p, C.POINTER(C.c_longlong)]),
+    "qzbsc_streams_to_fastq": (C.c_longlong, [u8p, C.c_longlong, u8p, C.c_longlong, u8p, C.c_longlong,
+                                              C.c_longlong, C.c_longlong, C.c_longlong, u8p, C.c_longlong]),
     "qzbsc_bwt_encode": (C.c_int, [u8p, C.c_int, u8p, i32p, C.c_int]),
     "qzbsc_bwt_decode": (C.c_int, [u8p, C.c_int, C.c_int, C.c_uint8, i32p, C.c_int]),
     "qzbsc_bwt_encode_batch": (C.c_int, [u8p, u8p, i32p, C.c_int, i32p, i32p]),

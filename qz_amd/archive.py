@@ -32,9 +32,12 @@ quality, sequence and header streams (compressor codes 1/1/1). Archives
 using other encodings or codecs raise ArchiveError.
 
 Blocks go through the batch C-ABI (bsc.compress_batch /
-bsc.decompress_batch): all blocks of a stream share one device batch for
-their BWTs, where qz fans out per-block libbsc calls over rayon. The three
-streams decode concurrently, as qz's three decompressor threads do.
+bsc.decompress_batch), where qz fans out per-block libbsc calls over rayon:
+the blocks of all three streams share one device batch when compressing;
+the three streams decode concurrently, as qz's three decompressor threads
+do, each 8 blocks per device batch. The FASTQ <-> streams conversions run
+natively (qzbsc_fastq_to_streams / qzbsc_streams_to_fastq); the per-record
+Python functions below state the same layout and check them in the tests.
 """
 from __future__ import annotations
 
@@ -45,6 +48,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import bsc
+from ._native import lib, u8p
 
 ARCHIVE_MAGIC = b"QZ"
 ARCHIVE_VERSION = 2
@@ -252,29 +256,68 @@ def archive_header(num_reads: int, headers_len: int, sequences_len: int, qualiti
     return bytes(out)
 
 
+def compress_streams(streams, block_size: int = bsc.BSC_BLOCK_SIZE) -> list:
+    """Each stream in the multi-block framing (b"" for an empty stream). The
+    blocks of all streams go through one batch call: one device batch, and
+    the host coder on all of them at once."""
+    pieces, owner = [], []
+    for k, data in enumerate(streams):
+        a = np.frombuffer(data, dtype=np.uint8)
+        for i in range(0, a.size, block_size):
+            pieces.append(a[i:i + block_size])
+            owner.append(k)
+    blocks = bsc.compress_batch(pieces) if pieces else []
+    out = []
+    for k in range(len(streams)):
+        mine = [b for b, o in zip(blocks, owner) if o == k]
+        if not mine:
+            out.append(b"")
+            continue
+        parts = [struct.pack("<I", len(mine))]
+        for b in mine:
+            parts += [struct.pack("<I", len(b)), b]
+        out.append(b"".join(parts))
+    return out
+
+
 def compress_stream(data: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
     """One stream in the multi-block framing; b"" for an empty stream."""
-    if not data:
-        return b""
-    a = np.frombuffer(data, dtype=np.uint8)
-    pieces = [a[i:i + block_size] for i in range(0, a.size, block_size)]
-    blocks = bsc.compress_batch(pieces)
-    parts = [struct.pack("<I", len(blocks))]
-    for b in blocks:
-        parts += [struct.pack("<I", len(b)), b]
-    return b"".join(parts)
+    return compress_streams([data], block_size)[0]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(u8p)
+
+
+def fastq_to_streams(fastq: bytes):
+    """FASTQ text -> (num_reads, const_seq_len, const_qual_len, header,
+    sequence, quality streams) by the native scan (qzbsc_fastq_to_streams);
+    the same streams as records_to_streams(parse_fastq(...))."""
+    import ctypes as C
+    src = np.frombuffer(fastq, dtype=np.uint8)
+    info = (C.c_longlong * 6)()
+    rc = lib().qzbsc_fastq_to_streams(_ptr(src), src.size, None, None, None, info)
+    if rc < 0:
+        raise ArchiveError("malformed FASTQ (a record needs four lines, '@' and '+')")
+    n, cs, cq, hl, sl, ql = (int(v) for v in info)
+    h, s, q = (np.empty(x, np.uint8) for x in (hl, sl, ql))
+    rc = lib().qzbsc_fastq_to_streams(_ptr(src), src.size, _ptr(h), _ptr(s), _ptr(q), info)
+    if rc < 0:
+        raise ArchiveError("malformed FASTQ")
+    return n, cs, cq, h.tobytes(), s.tobytes(), q.tobytes()
 
 
 def compress_records(records, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
     cs, cq = detect_const_lengths(records)
-    h, s, q = records_to_streams(records, cs, cq)
-    hz, sz, qz = (compress_stream(x, block_size) for x in (h, s, q))
+    hz, sz, qz = compress_streams(records_to_streams(records, cs, cq), block_size)
     return archive_header(len(records), len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
 
 
 def compress_fastq(fastq: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
     """FASTQ text -> .qz v2 archive bytes."""
-    return compress_records(parse_fastq(fastq), block_size)
+    n, cs, cq, h, s, q = fastq_to_streams(fastq)
+    hz, sz, qz = compress_streams([h, s, q], block_size)
+    return archive_header(n, len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
 
 
 # ---- reader -----------------------------------------------------------------
@@ -443,10 +486,27 @@ def streams_to_records(h: ArchiveHeader, hs: bytes, ss: bytes, qs: bytes) -> lis
     return [FastqRecord(hs[int(a):int(a + b)], seqs[i], quals[i]) for i, (a, b) in enumerate(zip(hst, hln))]
 
 
+def streams_to_fastq(h: ArchiveHeader, hs: bytes, ss: bytes, qs: bytes) -> bytes:
+    """The decoded streams -> FASTQ text by the native assembly
+    (qzbsc_streams_to_fastq); the same text as
+    format_fastq(streams_to_records(...))."""
+    a = [np.frombuffer(x, dtype=np.uint8) for x in (hs, ss, qs)]
+    args = (_ptr(a[0]), a[0].size, _ptr(a[1]), a[1].size, _ptr(a[2]), a[2].size, h.num_reads, h.const_seq_len,
+            h.const_qual_len)
+    size = lib().qzbsc_streams_to_fastq(*args, None, 0)
+    if size < 0:
+        raise ArchiveError("truncated stream: the streams hold fewer records than the header states")
+    out = np.empty(size, np.uint8)
+    rc = lib().qzbsc_streams_to_fastq(*args, _ptr(out), size)
+    if rc != size:
+        raise ArchiveError("record assembly failed")
+    return out.tobytes()
+
+
 def decompress_archive(buf, batch: int = DECOMPRESS_BATCH_SIZE) -> bytes:
     """.qz v2 archive bytes -> FASTQ text."""
     h, hs, ss, qs = decompress_streams(buf, batch)
-    return format_fastq(streams_to_records(h, hs, ss, qs))
+    return streams_to_fastq(h, hs, ss, qs)
 
 
 def compress_file(fastq_path: str, archive_path: str) -> int:

@@ -129,6 +129,24 @@ static void for_segments(int nb, bool parallel, Fn fn)
     }
 }
 
+// Runs fn(b) for b in [0, n) on up to hardware_concurrency host threads.
+template <class Fn>
+static void parallel_blocks(int n, Fn fn)
+{
+    const int T = std::min<int>(n, (int)std::max(1u, std::thread::hardware_concurrency()));
+    if (T <= 1) {
+        for (int b = 0; b < n; ++b) fn(b);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&] {
+            for (int b; (b = next.fetch_add(1)) < n;) fn(b);
+        });
+    for (auto& x : th) x.join();
+}
+
 // Coder payload for one block from its BWT (host front end). cap: bytes
 // available.
 int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parallel)
@@ -199,6 +217,59 @@ static int front_to_host(const QlfcFront& front, const uint8_t* dbwt, uint32_t n
     return 0;
 }
 
+// One coded segment of a batch.
+struct SegCode {
+    std::vector<uint8_t> out;
+    int res = -1;
+};
+
+// Codes every segment of the batch from the host copy of the front end, all
+// segments of all blocks in parallel.
+static void code_segments_host(const QlfcFront& front, const FrontHost& fh, std::vector<SegCode>& sc)
+{
+    sc.assign(front.segs_.size(), SegCode{});
+    parallel_blocks((int)front.segs_.size(), [&](int i) {
+        const auto& sg = front.segs_[i];
+        qlfc::SegRuns Rn;
+        Rn.sym = fh.sym.data() + sg.r0;
+        Rn.rank = fh.rank.data() + sg.r0;
+        Rn.len = fh.len.data() + sg.r0;
+        Rn.count = sg.r1 - sg.r0;
+        Rn.nsym = fh.nsym[i];
+        memcpy(Rn.table, fh.tables.data() + (size_t)i * 256, 256);
+        sc[i].out.resize((size_t)sg.size + 64);
+        sc[i].res = qlfc::encode_segment(Rn, (int)sg.size, sc[i].out.data(), (int)sg.size);
+    });
+}
+
+// Coder payload of batch block `block` from its coded segments (each block's
+// segments are taken once).
+static int payload_from_segments(const QlfcFront& front, const FrontHost& fh, std::vector<SegCode>& sc,
+                                 uint32_t block, int n, uint8_t* out, int cap)
+{
+    std::vector<int> sidx;
+    for (int i = 0; i < (int)front.segs_.size(); ++i)
+        if (front.segs_[i].block == block) sidx.push_back(i);
+    const int nb = (int)sidx.size();
+    if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_GPU_ERROR;
+    int size[qlfc::kMaxSegments], start[qlfc::kMaxSegments];
+    std::vector<std::vector<uint8_t>> segout(nb);
+    std::vector<int> res(nb);
+    for (int k = 0; k < nb; ++k) {
+        size[k] = (int)front.segs_[sidx[k]].size;
+        start[k] = (int)front.segs_[sidx[k]].start;
+        segout[k] = std::move(sc[sidx[k]].out);
+        res[k] = sc[sidx[k]].res;
+    }
+    return assemble_payload(
+        n, nb, size, segout, res,
+        [&](int sg, uint8_t* dst) {
+            memcpy(dst, fh.bwt.data() + start[sg], (size_t)size[sg]);
+            return 0;
+        },
+        out, cap);
+}
+
 // Coder payload of batch block `block` from the host copy of the front end.
 static int coder_compress_host(const QlfcFront& front, const FrontHost& fh, uint32_t block, int n, uint8_t* out,
                                int cap, bool parallel)
@@ -236,40 +307,61 @@ static int coder_compress_host(const QlfcFront& front, const FrontHost& fh, uint
         out, cap);
 }
 
-int coder_decompress(const uint8_t* in, int in_size, uint8_t* out, int n, bool parallel)
+// One segment of a coder payload to decode: in[0..isz) -> out[0..osz). A
+// segment of a multi-segment payload whose coded size equals its size is
+// stored raw.
+struct SegTask {
+    const uint8_t* in;
+    int isz;
+    uint8_t* out;
+    int osz;
+    bool raw_ok;
+};
+
+// Parses the segment table of a coder payload that decodes into out (n
+// bytes) and appends its segments to t.
+static int coder_tasks(const uint8_t* in, int in_size, uint8_t* out, int n, std::vector<SegTask>& t)
 {
     if (in_size < 1) return QZ_UNEXPECTED_EOB;
     const int nb = in[0];
     if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_DATA_CORRUPT;
-    if (nb == 1) return qlfc::decode_segment(in + 1, in_size - 1, out, n) ? QZ_DATA_CORRUPT : 0;
+    if (nb == 1) {
+        t.push_back({in + 1, in_size - 1, out, n, false});
+        return 0;
+    }
     if (1 + 8 * nb > in_size) return QZ_UNEXPECTED_EOB;
-    int istart[qlfc::kMaxSegments], isz[qlfc::kMaxSegments], ostart[qlfc::kMaxSegments], osz[qlfc::kMaxSegments];
     int64_t ip = 1 + 8 * nb, op = 0;
+    SegTask seg[qlfc::kMaxSegments];
     for (int s = 0; s < nb; ++s) {
-        osz[s] = (int)get32(in + 1 + 8 * s);
-        isz[s] = (int)get32(in + 5 + 8 * s);
-        if (osz[s] < 1 || isz[s] < 1) return QZ_DATA_CORRUPT;
-        istart[s] = (int)ip;
-        ostart[s] = (int)op;
-        ip += isz[s];
-        op += osz[s];
-        if (ip > in_size || op > n) return QZ_DATA_CORRUPT;
+        const int osz = (int)get32(in + 1 + 8 * s), isz = (int)get32(in + 5 + 8 * s);
+        if (osz < 1 || isz < 1) return QZ_DATA_CORRUPT;
+        if (ip + isz > in_size || op + osz > n) return QZ_DATA_CORRUPT;
+        seg[s] = {in + ip, isz, out + op, osz, true};
+        ip += isz;
+        op += osz;
     }
     if (op != n) return QZ_DATA_CORRUPT;
-    std::vector<int> rc(nb, 0);
-    auto work = [&](int s) {
-        if (isz[s] == osz[s]) memcpy(out + ostart[s], in + istart[s], (size_t)isz[s]);
-        else rc[s] = qlfc::decode_segment(in + istart[s], isz[s], out + ostart[s], osz[s]);
-    };
-    if (parallel) {
-        std::vector<std::thread> th;
-        for (int s = 0; s < nb; ++s) th.emplace_back(work, s);
-        for (auto& t : th) t.join();
-    } else {
-        for (int s = 0; s < nb; ++s) work(s);
+    t.insert(t.end(), seg, seg + nb);
+    return 0;
+}
+
+static int run_seg_task(const SegTask& x)
+{
+    if (x.raw_ok && x.isz == x.osz) {
+        memcpy(x.out, x.in, (size_t)x.isz);
+        return 0;
     }
-    for (int s = 0; s < nb; ++s)
-        if (rc[s]) return QZ_DATA_CORRUPT;
+    return qlfc::decode_segment(x.in, x.isz, x.out, x.osz) ? QZ_DATA_CORRUPT : 0;
+}
+
+int coder_decompress(const uint8_t* in, int in_size, uint8_t* out, int n, bool parallel)
+{
+    std::vector<SegTask> t;
+    QZ_TRY(coder_tasks(in, in_size, out, n, t));
+    std::vector<int> rc(t.size(), 0);
+    for_segments((int)t.size(), parallel, [&](int s) { rc[s] = run_seg_task(t[s]); });
+    for (int r : rc)
+        if (r) return QZ_DATA_CORRUPT;
     return 0;
 }
 
@@ -362,9 +454,11 @@ int parse_header(const uint8_t* h, int size, BlockHeader* bh)
     return ok ? 0 : QZ_DATA_CORRUPT;
 }
 
-// Checks the body checksum and decodes the coder payload into `bwt`
-// (n = data size). For a stored block copies the data and returns 1.
-int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, uint8_t* bwt, bool parallel)
+// Checks the body checksum and the mode of a block and appends its coder
+// segments (decoding into `bwt`, n = data size) to t. For a stored block
+// copies the data and returns 1.
+static int payload_tasks(const uint8_t* input, int inputSize, const BlockHeader& bh, uint8_t* bwt,
+                         std::vector<SegTask>& t)
 {
     if (inputSize < bh.block_size) return QZ_UNEXPECTED_EOB;
     if (adler32(input + kHeader, (size_t)(bh.block_size - kHeader)) != bh.adler_body) return QZ_DATA_CORRUPT;
@@ -379,25 +473,21 @@ int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, u
     const int num_indexes = input[bh.block_size - 1];
     const int payload = bh.block_size - kHeader - 1 - 4 * num_indexes;
     if (payload < 1) return QZ_DATA_CORRUPT;
-    return coder_decompress(input + kHeader, payload, bwt, bh.data_size, parallel);
+    return coder_tasks(input + kHeader, payload, bwt, bh.data_size, t);
 }
 
-// Runs fn(b) for b in [0, n) on up to hardware_concurrency host threads.
-template <class Fn>
-static void parallel_blocks(int n, Fn fn)
+// Checks the body checksum and decodes the coder payload into `bwt`
+// (n = data size). For a stored block copies the data and returns 1.
+int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, uint8_t* bwt, bool parallel)
 {
-    const int T = std::min<int>(n, (int)std::max(1u, std::thread::hardware_concurrency()));
-    if (T <= 1) {
-        for (int b = 0; b < n; ++b) fn(b);
-        return;
-    }
-    std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t)
-        th.emplace_back([&] {
-            for (int b; (b = next.fetch_add(1)) < n;) fn(b);
-        });
-    for (auto& x : th) x.join();
+    std::vector<SegTask> t;
+    const int r = payload_tasks(input, inputSize, bh, bwt, t);
+    if (r != 0) return r;
+    std::vector<int> rc(t.size(), 0);
+    for_segments((int)t.size(), parallel, [&](int k) { rc[k] = run_seg_task(t[k]); });
+    for (int x : rc)
+        if (x) return QZ_DATA_CORRUPT;
+    return 0;
 }
 
 // Device batches of the batch entry points hold at most this many bytes.
@@ -572,13 +662,15 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
         QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
         FrontHost fh;
         QZ_TRY(front_to_host(sl.front, sl.fwd.out_, offs[k], fh, sl.stream));
+        std::vector<SegCode> sc;
+        code_segments_host(sl.front, fh, sc);  // every segment of every block at once
         parallel_blocks(k, [&](int i) {
             const int b = list[i];
             out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], prim[i],
                                              aux.data() + (size_t)i * 256, output + ooff[b],
                                              [&](uint8_t* p, int cap) {
-                                                 return coder_compress_host(sl.front, fh, (uint32_t)i, dn[b], p,
-                                                                            cap, false);
+                                                 return payload_from_segments(sl.front, fh, sc, (uint32_t)i, dn[b],
+                                                                              p, cap);
                                              });
         });
         return 0;
@@ -623,6 +715,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
     std::vector<std::vector<uint8_t>> tmp(nblocks);  // LZP blocks: the m-byte BWT domain
     std::vector<int> need(nblocks, 0);               // BWT bytes to invert
     std::vector<uint8_t*> bw(nblocks, nullptr);
+    std::vector<std::vector<SegTask>> tasks(nblocks);  // coder segments per block
     parallel_blocks(nblocks, [&](int b) {
         const uint8_t* in = input + ioff[b];
         uint8_t* out = output + ooff[b];
@@ -637,7 +730,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
                 const int num_indexes = in[bh[b].block_size - 1];
                 const int payload = bh[b].block_size - kHeader - 4 - 1 - 4 * num_indexes;
                 tmp[b].resize((size_t)m);
-                if (payload < 1 || coder_decompress(in + kHeader + 4, payload, tmp[b].data(), m, false))
+                if (payload < 1 || coder_tasks(in + kHeader + 4, payload, tmp[b].data(), m, tasks[b]))
                     rc = QZ_DATA_CORRUPT;
                 else if (m > 1) {
                     bw[b] = tmp[b].data();
@@ -645,7 +738,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
                 }
             }
         } else if (!rc) {
-            const int r = decode_payload(in, in_sizes[b], bh[b], out, false);
+            const int r = payload_tasks(in, in_sizes[b], bh[b], out, tasks[b]);
             if (r < 0) rc = r;
             else if (r == 0 && bh[b].data_size > 1) {
                 bw[b] = out;
@@ -654,6 +747,16 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
         }
         rcs[b] = rc;
     });
+    {
+        // every coder segment of every block at once
+        std::vector<std::pair<int, const SegTask*>> all;
+        for (int b = 0; b < nblocks; ++b)
+            for (const auto& t : tasks[b]) all.push_back({b, &t});
+        std::vector<int> src(all.size(), 0);
+        parallel_blocks((int)all.size(), [&](int i) { src[i] = run_seg_task(*all[i].second); });
+        for (size_t i = 0; i < all.size(); ++i)
+            if (src[i] && !rcs[all[i].first]) rcs[all[i].first] = QZ_DATA_CORRUPT;
+    }
     std::vector<std::vector<int>> batches(1);
     uint64_t bytes = 0;
     for (int b = 0; b < nblocks; ++b) {

@@ -158,3 +158,40 @@ def test_stored_blocks_batch_without_device():
     comp = bsc.compress_batch([np.frombuffer(b, np.uint8) for b in blocks])
     assert [len(c) for c in comp] == [28 + len(b) for b in blocks]
     assert bsc.decompress_batch(comp[1:]) == blocks[1:]
+
+
+def _odd_records(rng, n):
+    """Long names (2-byte varints), empty reads, out-of-range quality bytes."""
+    recs = []
+    for i in range(n):
+        name = b"@" + b"x" * int(rng.integers(0, 300)) + b"%d" % i
+        L = int(rng.integers(0, 200))
+        seq = bytes(rng.choice(list(b"ACGTN"), L).astype(np.uint8))
+        qual = bytes(rng.integers(33, 161, L, dtype=np.uint8))
+        recs.append(A.FastqRecord(name, seq, qual))
+    return recs
+
+
+@pytest.mark.parametrize("kind", ["const", "variable", "odd"])
+def test_native_conversions_match_record_functions(kind):
+    rng = np.random.default_rng(13)
+    recs = _odd_records(rng, 400) if kind == "odd" else _records(500, rng, kind == "const")
+    text = A.format_fastq(recs)
+    cs, cq = A.detect_const_lengths(recs)
+    h, s, q = A.records_to_streams(recs, cs, cq)
+    assert A.fastq_to_streams(text) == (len(recs), cs, cq, h, s, q)
+    hdr = A.parse_archive_header(A.archive_header(len(recs), len(h), len(s), len(q), cs, cq) + h + s + q)
+    assert A.streams_to_fastq(hdr, h, s, q) == text
+    # one record more than the streams hold
+    hdr.num_reads += 1
+    with pytest.raises(A.ArchiveError):
+        A.streams_to_fastq(hdr, h, s, q)
+
+
+def test_native_fastq_edge_cases():
+    assert A.fastq_to_streams(b"") == (0, 0, 0, b"", b"", b"")
+    # no trailing newline
+    assert A.fastq_to_streams(b"@a\nAC\n+\nII")[:3] == (1, 2, 2)
+    for bad in (b"@a\nAC\n+\n", b"a\nAC\n+\nII\n", b"@a\nAC\n-\nII\n"):
+        with pytest.raises(A.ArchiveError):
+            A.fastq_to_streams(bad)
