@@ -197,6 +197,64 @@ def graph_section(torch, bwt, synth, sizes=(100_000, 1_000_000, 4_000_000), jobs
             "rows": rows, "captures": c, "replays": r}
 
 
+def archive_section(synth, reads: int, threads: int):
+    """FASTQ -> .qz v2 archive -> FASTQ, end to end (host stream building,
+    batched GPU BWTs, host coder, container), against the per-block pattern
+    of qz (each 25 MiB block its own bsc_compress / bsc_decompress call from
+    a pool of host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from qz_amd import archive as A, bsc
+    text = synth.fastq(reads, 150, 5, genome_len=5_000_000)
+    mb = len(text) / 1e6
+    arc = A.compress_fastq(text)  # warm up: workspaces and coder tables
+    assert A.decompress_archive(arc) == text, "archive round trip mismatch"
+    tc, td = [], []
+    for _ in range(2):
+        t = time.perf_counter()
+        arc = A.compress_fastq(text)
+        tc.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        back = A.decompress_archive(arc)
+        td.append(time.perf_counter() - t)
+    assert back == text
+    # the block calls alone: batched (all streams per call) ...
+    _, cs, cq, h, s, q = A.fastq_to_streams(text)
+    bc, bd = [], []
+    for _ in range(2):
+        t = time.perf_counter()
+        zs = A.compress_streams([h, s, q])
+        bc.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        _, hs, ss, qs = A.decompress_streams(A.archive_header(reads, *map(len, zs), cs, cq) + b"".join(zs))
+        bd.append(time.perf_counter() - t)
+    assert (hs, ss, qs) == (h, s, q)
+    # ... and the per-block pattern over the same streams
+    with ThreadPoolExecutor(threads) as pool:
+        def pieces(x):
+            a = np.frombuffer(x, np.uint8)
+            return [a[i:i + bsc.BSC_BLOCK_SIZE] for i in range(0, a.size, bsc.BSC_BLOCK_SIZE)]
+        allp = pieces(h) + pieces(s) + pieces(q)
+        t = time.perf_counter()
+        comp = list(pool.map(bsc.compress_adaptive_no_lzp, allp))
+        pc = time.perf_counter() - t
+        t = time.perf_counter()
+        dec = list(pool.map(bsc.decompress, comp))
+        pd = time.perf_counter() - t
+    assert dec == [p.tobytes() for p in allp]
+    return {"workload": f"{reads} FASTQ records (Illumina names, 150 bp reads from a 5 Mbp genome, binned "
+                        f"qualities) = {len(text)} B; default BSC pipeline (encoding 0, 7-bit lossless qualities)",
+            "fastq_mb": round(mb, 1), "archive_bytes": len(arc), "ratio": round(len(arc) / len(text), 4),
+            "blocks": len(allp),
+            "compress_mb_s": round(mb / min(tc), 1), "decompress_mb_s": round(mb / min(td), 1),
+            "batched_blocks_compress_mb_s": round(mb / min(bc), 1),
+            "batched_blocks_decompress_mb_s": round(mb / min(bd), 1),
+            "per_block_compress_mb_s": round(mb / pc, 1), "per_block_decompress_mb_s": round(mb / pd, 1),
+            "host_threads": threads,
+            "note": "MB/s of FASTQ text. compress/decompress: end to end (FASTQ<->streams, container, blocks). "
+                    "batched_blocks / per_block: the block calls alone on the same streams, batched entry "
+                    "points vs one bsc_compress / bsc_decompress call per block from a pool of host threads"}
+
+
 def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int):
     """Poisson arrivals of compress (forward BWT) and decompress (inverse
     BWT) jobs, 70/30, each 1..40 blocks of 100 KB..25 MiB from the mixed
@@ -220,13 +278,18 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
            "rates": []}
     budget = 200 << 30
     with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers) as srv:
-        # single-job latency, jobs alone
-        alone = []
-        for t in templates[:6]:
-            j = srv.submit_raw(0, t["inp"], t["sizes"], t["out"])
-            srv.wait(j)
-            alone.append(j.latency_ms)
-        res["single_job_ms"] = round(float(np.median(alone)), 2)
+        # each job alone (both kinds), twice; the second run counts
+        solo = {}
+        for rep in range(2):
+            for k, t in enumerate(templates):
+                for kind in (0, 1):
+                    if kind == 0:
+                        j = srv.submit_raw(0, t["inp"], t["sizes"], t["out"])
+                    else:
+                        j = srv.submit_raw(1, t["bwt"], t["sizes"], t["out"], t["prim"])
+                    srv.wait(j)
+                    solo[(k, kind)] = j.latency_ms
+        res["single_job_ms"] = round(float(np.median([solo[(k, 0)] for k in range(len(templates))])), 2)
         for rate in rates:
             arrivals = np.cumsum(rng.exponential(1.0 / rate, size=jobs_per_rate))
             picks = rng.integers(0, len(templates), size=jobs_per_rate)
@@ -242,17 +305,24 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
                     sub.append((srv.submit_raw(0, t["inp"], t["sizes"], t["out"]), t["inp"].size))
                 else:
                     sub.append((srv.submit_raw(1, t["bwt"], t["sizes"], t["out"], t["prim"]), t["inp"].size))
-            lat, nbat = [], []
-            for j, _ in sub:
+            lat, nbat, slow, alone = [], [], [], []
+            for (j, _), p, k in zip(sub, picks, kinds):
                 srv.wait(j)
                 lat.append(j.latency_ms)
                 nbat.append(j.batch_jobs)
+                a = solo[(int(p), int(k))]
+                alone.append(a)
+                slow.append(j.latency_ms / a)
             wall = time.perf_counter() - t0
             nbytes = sum(b for _, b in sub)
             res["rates"].append({"offered_jobs_per_s": rate, "jobs_per_s": round(len(sub) / wall, 2),
                                  "mb_s": round(nbytes / wall / 1e6, 1),
                                  "p50_ms": round(float(np.percentile(lat, 50)), 1),
                                  "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                                 "alone_p99_ms": round(float(np.percentile(alone, 99)), 1),
+                                 "p99_vs_alone_p99": round(float(np.percentile(lat, 99) / np.percentile(alone, 99)), 2),
+                                 "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
+                                 "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
                                  "mean_jobs_per_batch": round(float(np.mean(nbat)), 2)})
             log(f"serving rate {rate}: {res['rates'][-1]}")
         res["peak_inflight_bytes"] = srv.peak_bytes()
@@ -279,6 +349,8 @@ def main():
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
+    ap.add_argument("--archive-reads", type=int, default=1_000_000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -413,6 +485,11 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers)
 
+    archive = None
+    if not args.no_archive:
+        archive = archive_section(synth, args.archive_reads, max(1, min(args.cpu_threads, os.cpu_count() or 1)))
+        log(f"archive: {archive}")
+
     cpu = None
     if not args.no_cpu:
         thr = max(1, min(args.cpu_threads, os.cpu_count() or 1, len(blocks)))
@@ -450,6 +527,7 @@ def main():
         "graph_replay": graphs,
         "uneven_batch": uneven,
         "serving": serving,
+        "archive": archive,
         "tuning": bwt.tuning(),
     }
     if per_block:

@@ -166,6 +166,12 @@ QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, u
  * wait: blocks until the job is done; returns its status and, optionally,
  * its latency, its queueing time and the number of jobs in its batch. */
 QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_bytes, int workers);
+/* As qzbsc_server_create (which uses a 32 MiB lane), with the small-job lane
+ * set explicitly: one more worker takes only jobs of at most
+ * small_lane_bytes input bytes, so small requests do not queue behind
+ * batches of large ones (0 = no lane). No counterpart in the reference. */
+QZBSC_API void* qzbsc_server_create_lanes(long long hbm_budget, long long max_batch_bytes, int workers,
+                                          long long small_lane_bytes);
 QZBSC_API void qzbsc_server_destroy(void* srv);
 QZBSC_API int qzbsc_server_submit(void* srv, int kind, const unsigned char* in, unsigned char* out, const int* sizes,
                                   int nblocks, const int* primary_in, int* primary_out, int* aux_out);

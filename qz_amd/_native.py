@@ -42,6 +42,7 @@ _SIGS = {
     "qzbsc_coder_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int]),
     "qzbsc_gpu_available": (C.c_int, []),
     "qzbsc_server_create": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int]),
+    "qzbsc_server_create_lanes": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int, C.c_longlong]),
     "qzbsc_server_destroy": (None, [C.c_void_p]),
     "qzbsc_server_submit": (C.c_int, [C.c_void_p, C.c_int, u8p, u8p, i32p, C.c_int, i32p, i32p, i32p]),
     "qzbsc_server_wait": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -33,16 +33,15 @@ using other encodings or codecs raise ArchiveError.
 
 Blocks go through the batch C-ABI (bsc.compress_batch /
 bsc.decompress_batch), where qz fans out per-block libbsc calls over rayon:
-the blocks of all three streams share one device batch when compressing;
-the three streams decode concurrently, as qz's three decompressor threads
-do, each 8 blocks per device batch. The FASTQ <-> streams conversions run
+the blocks of all three streams share one device batch when compressing,
+and when decoding each batch takes 8 blocks of every stream (qz's three
+decompressor threads, 8 blocks each). The FASTQ <-> streams conversions run
 natively (qzbsc_fastq_to_streams / qzbsc_streams_to_fastq); the per-record
 Python functions below state the same layout and check them in the tests.
 """
 from __future__ import annotations
 
 import struct
-import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -413,27 +412,23 @@ def _check_supported(h: ArchiveHeader) -> None:
 
 
 def decompress_streams(buf, batch: int = DECOMPRESS_BATCH_SIZE):
-    """(header, raw streams h, s, q): the three streams decode concurrently."""
+    """(header, raw streams h, s, q). The three streams advance together:
+    each batch call takes up to `batch` blocks of every stream (qz runs
+    three decompressor threads, each 8 blocks at a time), so one device
+    batch and one host coder pool serve all three."""
     h = parse_archive_header(buf)
     _check_supported(h)
-    spans = stream_offsets(h)
-    res = [None, None, None]
-    err = [None]
-
-    def work(k):
-        try:
-            res[k] = decompress_stream(buf, spans[k][0], spans[k][1], batch)
-        except Exception as e:  # re-raised on the caller's thread
-            err[0] = e
-
-    th = [threading.Thread(target=work, args=(k,)) for k in range(3)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    if err[0] is not None:
-        raise err[0]
-    return h, res[0], res[1], res[2]
+    per = [stream_blocks(buf, o, n) for o, n in stream_offsets(h)]
+    out = [[], [], []]
+    for i in range(0, max(len(b) for b in per), batch):
+        chunk, owner = [], []
+        for k in range(3):
+            part = per[k][i:i + batch]
+            chunk += part
+            owner += [k] * len(part)
+        for k, d in zip(owner, bsc.decompress_batch(chunk)):
+            out[k].append(d)
+    return h, b"".join(out[0]), b"".join(out[1]), b"".join(out[2])
 
 
 def _scan_lengths(data: bytes, count: int, what: str):

@@ -10,6 +10,11 @@
 // that arrive while others run join the next batch as soon as a worker and
 // memory free up. Every job records submit / start / finish times.
 //
+// Small-job lane: one more worker takes only jobs of at most lane_max bytes
+// (oldest such job first, batched with later ones up to lane_max), so a
+// small request does not queue behind batches of large ones; its kernels run
+// on the lane's own stream beside theirs.
+//
 // No counterpart in the reference: qz issues one libbsc call per block from
 // rayon workers (crates/qz-lib/src/compression/mod.rs:402-404,
 // decompress_impl.rs:117-120); this replaces that fan-out for a service.
@@ -64,12 +69,15 @@ public:
     // Starts the workers and returns once every one holds its resident
     // forward and inverse workspaces for a full batch (init_rc() != 0 if
     // any could not).
-    Server(uint64_t budget, uint64_t max_batch, int workers) : budget_(budget), max_batch_(max_batch)
+    Server(uint64_t budget, uint64_t max_batch, int workers, uint64_t lane_max)
+        : budget_(budget), max_batch_(max_batch)
     {
         trace_ = getenv("QZBSC_TRACE_SERVE") != nullptr;
-        for (int w = 0; w < workers; ++w) threads_.emplace_back([this] { work(); });
+        for (int w = 0; w < workers; ++w) threads_.emplace_back([this] { work(0); });
+        if (lane_max) threads_.emplace_back([this, lane_max] { work(lane_max); });
+        const int all = (int)threads_.size();
         std::unique_lock<std::mutex> lk(mu_);
-        cv_done_.wait(lk, [&] { return ready_ == workers; });
+        cv_done_.wait(lk, [&] { return ready_ == all; });
     }
     int init_rc() const { return init_rc_; }
     static uint32_t batch_blocks(uint64_t bytes) { return (uint32_t)(bytes / 65536 + 64); }
@@ -91,7 +99,7 @@ public:
         j->t_submit = now_ms();
         pending_.push_back(j.get());
         jobs_[id] = std::move(j);
-        cv_work_.notify_one();
+        cv_work_.notify_all();  // the small-job lane may not take it: wake every idle worker
         return id;
     }
 
@@ -113,13 +121,16 @@ public:
     uint64_t peak_inflight() const { return peak_; }
 
 private:
-    void work()
+    // lane_max == 0: a regular worker (any job, oldest first); otherwise the
+    // small-job lane (jobs of at most lane_max bytes).
+    void work(uint64_t lane_max)
     {
+        const uint64_t cap = lane_max ? lane_max : max_batch_;
         SlotGuard g;
         {
             int rc = g.ok() ? 0 : QZ_GPU_ERROR;
-            if (rc == 0) rc = g.slot().fwd.reserve(max_batch_, batch_blocks(max_batch_));
-            if (rc == 0) rc = g.slot().inv.reserve(max_batch_, batch_blocks(max_batch_));
+            if (rc == 0) rc = g.slot().fwd.reserve(cap, batch_blocks(cap));
+            if (rc == 0) rc = g.slot().inv.reserve(cap, batch_blocks(cap));
             std::lock_guard<std::mutex> lk(mu_);
             if (rc) init_rc_ = rc;
             ++ready_;
@@ -130,32 +141,36 @@ private:
             uint64_t fp = 0;
             {
                 std::unique_lock<std::mutex> lk(mu_);
+                auto eligible = [&](const Job* j) { return !lane_max || j->bytes <= lane_max; };
+                std::deque<Job*>::iterator first;
                 for (;;) {
                     if (stop_ && pending_.empty()) return;
-                    if (!pending_.empty()) {
-                        Job* first = pending_.front();
-                        const uint64_t need = job_footprint(first->kind, first->bytes, (uint32_t)first->sizes.size());
+                    first = std::find_if(pending_.begin(), pending_.end(), eligible);
+                    if (first != pending_.end()) {
+                        const uint64_t need = job_footprint((*first)->kind, (*first)->bytes,
+                                                            (uint32_t)(*first)->sizes.size());
                         // admit when it fits next to what is in flight (a job bigger than
                         // the whole budget runs alone)
                         if (inflight_ == 0 || inflight_ + need <= budget_) break;
                     }
+                    if (stop_ && first == pending_.end()) return;  // what is left is not this lane's
                     cv_work_.wait(lk);
                 }
-                // continuous batching: oldest job, then later jobs of the same
-                // direction while cap and budget allow
-                const int kind = pending_.front()->kind;
+                // continuous batching: the oldest job this worker may take, then
+                // later jobs of the same direction while cap and budget allow
+                const int kind = (*first)->kind;
                 uint64_t bytes = 0;
                 uint32_t blocks = 0;
-                for (auto it = pending_.begin(); it != pending_.end();) {
+                for (auto it = first; it != pending_.end();) {
                     Job* j = *it;
-                    if (j->kind != kind) {
+                    if (j->kind != kind || !eligible(j)) {
                         ++it;
                         continue;
                     }
                     const uint64_t nb = bytes + j->bytes;
                     const uint32_t nbl = blocks + (uint32_t)j->sizes.size();
                     const uint64_t nfp = job_footprint(kind, nb, nbl);
-                    if (!batch.empty() && (nb > max_batch_ || inflight_ + nfp > budget_ || nb >= 0x7fffffffull))
+                    if (!batch.empty() && (nb > cap || inflight_ + nfp > budget_ || nb >= 0x7fffffffull))
                         break;
                     batch.push_back(j);
                     bytes = nb;
@@ -258,11 +273,14 @@ using namespace qz;
 
 extern "C" {
 
-QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_bytes, int workers)
+QZBSC_API void* qzbsc_server_create_lanes(long long hbm_budget, long long max_batch_bytes, int workers,
+                                          long long small_lane_bytes)
 {
-    if (hbm_budget <= 0 || max_batch_bytes <= 0 || workers < 1 || workers > 16) return nullptr;
+    if (hbm_budget <= 0 || max_batch_bytes <= 0 || workers < 1 || workers > 16 || small_lane_bytes < 0)
+        return nullptr;
     if (!pool_device_ok()) return nullptr;
-    pool_set_limit(std::max(4, workers + 2));
+    const uint64_t lane = (uint64_t)std::min(small_lane_bytes, max_batch_bytes);
+    pool_set_limit(std::max(4, workers + 3));
     // every worker keeps a forward and an inverse workspace for a full batch
     // resident from the start: cap the batch so that all of them together
     // stay inside the budget
@@ -275,15 +293,26 @@ QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_by
     uint64_t cap = (uint64_t)max_batch_bytes;
     auto resident = [&](uint64_t c) {
         const uint32_t nb = Server::batch_blocks(c);
-        return (uint64_t)workers * (BwtEngine::footprint(c, nb) + IbwtEngine::footprint(c, nb));
+        uint64_t r = (uint64_t)workers * (BwtEngine::footprint(c, nb) + IbwtEngine::footprint(c, nb));
+        if (lane) {
+            const uint64_t l = std::min<uint64_t>(lane, c);
+            const uint32_t lb = Server::batch_blocks(l);
+            r += BwtEngine::footprint(l, lb) + IbwtEngine::footprint(l, lb);
+        }
+        return r;
     };
     while (cap > (1u << 20) && resident(cap) > budget) cap -= cap / 8;
-    Server* s = new Server(budget, cap, workers);
+    Server* s = new Server(budget, cap, workers, std::min<uint64_t>(lane, cap));
     if (s->init_rc() != 0) {
         delete s;
         return nullptr;
     }
     return s;
+}
+
+QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_bytes, int workers)
+{
+    return qzbsc_server_create_lanes(hbm_budget, max_batch_bytes, workers, 32ll << 20);
 }
 
 QZBSC_API void qzbsc_server_destroy(void* srv) { delete (Server*)srv; }

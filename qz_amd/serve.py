@@ -28,8 +28,11 @@ class Job:
 
 
 class Server:
-    def __init__(self, hbm_budget: int, max_batch_bytes: int = 1 << 30, workers: int = 2):
-        self._h = lib().qzbsc_server_create(hbm_budget, max_batch_bytes, workers)
+    def __init__(self, hbm_budget: int, max_batch_bytes: int = 1 << 30, workers: int = 2,
+                 small_lane_bytes: int = 32 << 20):
+        """workers regular workers plus, when small_lane_bytes > 0, one
+        worker that takes only jobs of at most that many bytes."""
+        self._h = lib().qzbsc_server_create_lanes(hbm_budget, max_batch_bytes, workers, small_lane_bytes)
         if not self._h:
             raise BscError(-7, "qzbsc_server_create")
 

@@ -66,3 +66,25 @@ def test_tight_budget_admits_one_batch_at_a_time():
         out, prim, _ = bwt.encode_batch(jb)
         for b, blk in enumerate(j.blocks()):
             assert np.array_equal(blk, out[b])
+
+
+def test_small_job_lane_skips_the_queue():
+    """One regular worker busy with a 100 MB job: a small job submitted right
+    after goes to the small-job lane and finishes first, with the same result
+    as alone."""
+    big = synth.cut_blocks(synth.config2_sequences(700_000, seed=8))[:4]
+    small = [synth.sequence_stream(100_000, 1_000, 150, 9, 0.002)]
+    print("lane test:", [b.size for b in big], small[0].size, flush=True)
+    with serve.Server(hbm_budget=64 << 30, max_batch_bytes=1 << 30, workers=1, small_lane_bytes=32 << 20) as srv:
+        srv.wait(srv.submit_forward(big))  # workspaces warm
+        jb = srv.submit_forward(big)
+        js = srv.submit_forward(small)
+        srv.wait(js)
+        srv.wait(jb)
+        assert js.latency_ms < jb.latency_ms
+        assert js.batch_jobs == 1 and jb.batch_jobs == 1
+    for j, blocks in ((jb, big), (js, small)):
+        out, prim, _ = bwt.encode_batch(blocks)
+        assert j.prim_out.tolist() == list(prim)
+        for b, blk in enumerate(j.blocks()):
+            assert np.array_equal(blk, out[b])
