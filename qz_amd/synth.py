@@ -121,18 +121,23 @@ def quality_stream(n_reads: int, read_len: int, seed: int, p_repeat: float = 0.8
     prof = (1 - t)[:, None] * head[None, :] + t[:, None] * tail[None, :]
     cdf = np.round(np.cumsum(prof, axis=1)[:, :3] * 65536).astype(np.uint32)  # (L, 3), 16-bit fixed point
     rep_t = int(round(p_repeat * 65536))
-    pos = np.arange(read_len, dtype=np.int32)
-    out = np.empty(n_reads * read_len, dtype=np.uint8)
+    out = np.empty((n_reads, read_len), dtype=np.uint8)
+    cdf_t = np.ascontiguousarray(cdf.T[:, :, None])  # (3, L, 1)
+    sub = 4096  # reads per cache-sized piece
     for s in range(0, n_reads, chunk):
         k = min(chunk, n_reads - s)
         u = rng.integers(0, 65536, size=(k, read_len), dtype=np.uint32)
-        fresh = ((u > cdf[None, :, 0]).astype(np.uint8) + (u > cdf[None, :, 1]) + (u > cdf[None, :, 2]))
         rep = rng.integers(0, 65536, size=(k, read_len), dtype=np.uint32) < rep_t
-        rep[:, 0] = False
-        src = np.maximum.accumulate(np.where(rep, 0, pos[None, :]), axis=1)
-        q = np.take_along_axis(fresh, src, axis=1)
-        out[s * read_len:(s + k) * read_len] = _QV[q].reshape(-1)
-    return out
+        for r0 in range(0, k, sub):
+            ut = u[r0:r0 + sub].T  # cycle-major: the chain runs one cycle at a time over all reads
+            rt = np.ascontiguousarray(rep[r0:r0 + sub].T)
+            ft = (ut > cdf_t[0]).astype(np.uint8)
+            ft += ut > cdf_t[1]
+            ft += ut > cdf_t[2]
+            for j in range(1, read_len):
+                np.copyto(ft[j], ft[j - 1], where=rt[j])
+            out[s + r0:s + r0 + ft.shape[1]] = _QV[ft.T]
+    return out.reshape(-1)
 
 
 def config1_block(seed: int = 1) -> np.ndarray:
