@@ -197,6 +197,42 @@ def graph_section(torch, bwt, synth, sizes=(100_000, 1_000_000, 4_000_000), jobs
             "rows": rows, "captures": c, "replays": r}
 
 
+def streams_section(torch, bwt, synth, reads: int, steps: int):
+    """The other streams of the configurations, each as one batch of 25 MiB
+    blocks from HBM, forward + inverse BWT like the headline: config 3
+    (binned qualities, 150 per read) and the config-2 header stream
+    (Illumina names)."""
+    dev = torch.device("cuda:0")
+    rows = []
+    for name, gen in (("config 3 qualities", lambda: synth.quality_stream(reads, 150, 3)),
+                      ("config 2 headers", lambda: synth.header_stream(reads, 4))):
+        stream = gen()
+        sizes = np.array([b.size for b in synth.cut_blocks(stream)], dtype=np.int32)
+        d_in = torch.from_numpy(stream).to(dev)
+        d_b, d_o = torch.empty_like(d_in), torch.empty_like(d_in)
+        torch.cuda.synchronize()
+        fw, iv = [], []
+        for k in range(steps + 1):  # the first is warmup
+            t = time.perf_counter()
+            prim, _, f = bwt.dev_encode_batch(d_in.data_ptr(), d_b.data_ptr(), sizes)
+            i = bwt.dev_decode_batch(d_b.data_ptr(), d_o.data_ptr(), sizes, prim)
+            wall = time.perf_counter() - t
+            if k:
+                fw.append(f)
+                iv.append(i)
+                walls = wall if k == 1 else min(walls, wall)
+        torch.cuda.synchronize()
+        ok = torch.equal(d_o, d_in)
+        rows.append({"stream": name, "bytes": int(stream.size), "blocks": int(sizes.size),
+                     "mb_s": round(stream.size / walls / 1e6, 1), "forward_ms": round(min(fw), 2),
+                     "inverse_ms": round(min(iv), 2), "round_trip_ok": bool(ok)})
+        del d_in, d_b, d_o
+        if not ok:
+            print(f"bench.py: {name} round trip failed", file=sys.stderr)
+            sys.exit(1)
+    return rows
+
+
 def archive_section(synth, reads: int, threads: int):
     """FASTQ -> .qz v2 archive -> FASTQ, end to end (host stream building,
     batched GPU BWTs, host coder, container), against the per-block pattern
@@ -350,6 +386,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
+    ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
     ap.add_argument("--archive-reads", type=int, default=1_000_000)
     args = ap.parse_args()
 
@@ -485,6 +522,11 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers)
 
+    streams = None
+    if not args.no_streams:
+        streams = streams_section(torch, bwt, synth, args.reads, 2)
+        log(f"streams: {streams}")
+
     archive = None
     if not args.no_archive:
         archive = archive_section(synth, args.archive_reads, max(1, min(args.cpu_threads, os.cpu_count() or 1)))
@@ -527,6 +569,7 @@ def main():
         "graph_replay": graphs,
         "uneven_batch": uneven,
         "serving": serving,
+        "streams": streams,
         "archive": archive,
         "tuning": bwt.tuning(),
     }
