@@ -262,33 +262,116 @@ __device__ __forceinline__ void stage_push(Stage& S, uint32_t start, uint32_t si
 // Whole workgroup; ends with the staged groups in the device lists.
 __device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { stage_flush_e(S.h, S.e, gl, kStage); }
 
+// The doubling key of suffix s: 1 + rank of s + d (d = the depth of s's
+// group, the symbols all its members share), 0 past the block end.
 __device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const BlockMap& bm, uint32_t s,
-                                              uint32_t h)
+                                              uint32_t d)
 {
     const uint32_t end = bm.off[bm.find(s) + 1];
-    const uint64_t p = (uint64_t)s + h;
+    const uint64_t p = (uint64_t)s + d;
     return (p < end) ? rank[(uint32_t)p] + 1u : 0u;
 }
 
+// Depth of a new subgroup (>= 2 members) of the group starting at slot
+// `start`: the group's depth plus that of the group its members' keys point
+// into (key = 1 + that group's start slot). Members with equal keys share
+// the group's depth of symbols and then the target group's; a key of 0
+// (past the block end) is unique within a group and never forms a subgroup.
+__device__ __forceinline__ uint32_t sub_depth(const uint32_t* __restrict__ dep_in, uint32_t start, uint32_t key)
+{
+    const uint32_t d = dep_in[start];
+    const uint32_t t = key ? dep_in[key - 1u] : 0u;
+    const uint32_t s = d + t;
+    return s < d ? 0xffffffffu : s;  // saturate (never reached: depth < 2^31)
+}
+
+// Byte histogram of the batch in hist[0..256) and, in hist[256], the number
+// of positions whose successor differs (run boundaries, block ends ignored).
 __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restrict__ text, uint32_t n,
                                                         uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[4][256];
+    __shared__ uint32_t bnd;
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
     for (int w = 0; w < 4; ++w) h[w][tid] = 0;
+    if (tid == 0) bnd = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
-        atomicAdd(&h[wave][text[i]], 1u);
+    uint32_t b = 0;
+    for (uint32_t i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) {
+        const uint8_t c = text[i];
+        atomicAdd(&h[wave][c], 1u);
+        b += (i + 1 < n && text[i + 1] != c) ? 1u : 0u;
+    }
+    for (int d = 1; d < QZ_WAVE; d <<= 1) b += __shfl_xor(b, d, QZ_WAVE);
+    if (lane_id() == 0 && b) atomicAdd(&bnd, b);
     __syncthreads();
     uint32_t t = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
     if (t) atomicAdd(&hist[tid], t);
+    if (tid == 0 && bnd) atomicAdd(&hist[256], bnd);
 }
 
+// Round-0 keys. Packed: the next k0 symbol codes in radix sigma+1. Run
+// length (rle): nseg segments of segbits bits, most significant first, each
+// one run (or its first cap symbols) as (code, class, value) in (sb, 2, lb)
+// bits: class 0 = the run is followed by a smaller symbol (or the block
+// end), value = length; 1 = the run goes on past cap symbols, value = cap;
+// 2 = followed by a larger symbol, value = cap + 1 - length. A run of c
+// followed by a smaller symbol sorts below any longer run of c, a run
+// followed by a larger one above it, so these keys order suffixes as their
+// text does; equal keys share the sum of their lengths (the key's depth).
 struct PackParams {
     uint16_t code[256];  // symbol -> 1..sigma (sigma may be 256)
     uint32_t radix;      // sigma + 1
-    uint32_t k0;         // symbols per 32-bit key
+    uint32_t k0;         // symbols per 32-bit key (packed keys)
+    uint32_t rle;        // 1: run-length keys, per-group depths
+    uint32_t sb, lb, nseg, segbits, cap;
 };
+
+// Symbols a round-0 key covers: k0 for packed keys, the sum of the run
+// lengths for run-length keys.
+__device__ __forceinline__ uint32_t key_depth(uint32_t key, const PackParams& pp)
+{
+    if (!pp.rle) return pp.k0;
+    uint32_t d = 0;
+    const uint32_t vmask = (1u << pp.lb) - 1u;
+    for (uint32_t q = 0; q < pp.nseg; ++q) {
+        const uint32_t part = (key >> (32u - (q + 1u) * pp.segbits)) & ((1u << pp.segbits) - 1u);
+        if ((part >> (pp.lb + 2u)) == 0u) break;  // past the block end
+        const uint32_t cls = (part >> pp.lb) & 3u, v = part & vmask;
+        d += (cls == 2u) ? pp.cap + 1u - v : v;
+    }
+    return d;
+}
+
+// Run-length key of position i (block end `end`), see PackParams.
+__device__ __forceinline__ uint32_t rle_key(const uint8_t* __restrict__ text, const uint16_t* code, uint32_t i,
+                                            uint32_t end, const PackParams& pp)
+{
+    uint32_t key = 0, p = i;
+    for (uint32_t q = 0; q < pp.nseg; ++q) {
+        uint32_t part = 0;
+        if (p < end) {
+            const uint8_t b = text[p];
+            uint32_t r = 1;
+            while (r <= pp.cap && p + r < end && text[p + r] == b) ++r;
+            const uint32_t c = code[b];
+            uint32_t cls, v, len;
+            if (r > pp.cap) {
+                cls = 1u;
+                v = len = pp.cap;
+            } else {
+                const uint32_t nx = (p + r < end) ? code[text[p + r]] : 0u;
+                cls = nx < c ? 0u : 2u;
+                v = nx < c ? r : pp.cap + 1u - r;
+                len = r;
+            }
+            part = (c << (pp.lb + 2u)) | (cls << pp.lb) | v;
+            p += len;
+        }
+        key = (key << pp.segbits) | part;
+    }
+    return key << (32u - pp.nseg * pp.segbits);
+}
 
 // Packed prefix key of every position: the next k0 symbol codes in radix
 // sigma+1, code 0 past the block end. Each thread builds 16 consecutive keys
@@ -306,6 +389,16 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
     __syncthreads();
     const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kKeyItems;
     if (i0 >= n) return;
+    if (ppd->rle) {
+        const PackParams& pp = *ppd;
+        uint32_t end = 0;
+        for (int q = 0; q < kKeyItems && i0 + q < n; ++q) {
+            const uint32_t i = i0 + q;
+            if (i >= end) end = bm.off[bm.find(i) + 1];
+            keys[i] = rle_key(text, code, i, end, pp);
+        }
+        return;
+    }
     uint32_t top = 1;  // weight of the leading code
     for (uint32_t j = 1; j < k0; ++j) top *= radix;
     uint32_t end = 0, key = 0;
@@ -344,7 +437,11 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
 // Symbol packing from the byte histogram (one workgroup of 256): present
 // symbols get codes 1..sigma in byte order, 0 means past the block end, and
 // k0 is the largest k with (sigma+1)^k <= 2^32.
-__global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp)
+// Run-length keys when the mean run is at least kRleMinRun symbols (force:
+// -1 decide, 0 never, 1 always).
+constexpr uint32_t kRleMinRun = 3;
+__global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp,
+                                                        int force)
 {
     __shared__ uint32_t wsum[kThreads / QZ_WAVE];
     const uint32_t c = threadIdx.x, lane = lane_id(), w = c / QZ_WAVE;
@@ -368,6 +465,21 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         }
         pp->radix = (uint32_t)radix;
         pp->k0 = k ? k : 1;
+        uint64_t n = 0;
+        for (int x = 0; x < 256; ++x) n += hist[x];
+        const uint64_t runs = (uint64_t)hist[256] + 1;
+        const bool rle = force >= 0 ? force != 0 : n >= (uint64_t)kRleMinRun * runs;
+        uint32_t sb = 0;
+        while ((1ull << sb) < radix) ++sb;  // codes 0..sigma
+        const uint32_t nseg = sb <= 5 ? 3u : 2u;
+        const uint32_t segbits = 32u / nseg;  // 10 or 16
+        const uint32_t lb = segbits - 2u - sb;
+        pp->rle = rle ? 1u : 0u;
+        pp->sb = sb;
+        pp->lb = lb;
+        pp->nseg = nseg;
+        pp->segbits = segbits;
+        pp->cap = (1u << lb) - 1u;
     }
 }
 
@@ -417,7 +529,8 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ vals,
                                                      const TileDesc* __restrict__ tiles,
                                                      const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
-                                                     uint32_t* __restrict__ rank, GroupLists gl)
+                                                     uint32_t* __restrict__ rank, GroupLists gl,
+                                                     const PackParams* __restrict__ ppd, uint32_t* __restrict__ dep)
 {
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);  // 1024 slots per wave
     __shared__ Stage S;
@@ -456,7 +569,10 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
             sa[j] = s;
             rank[s] = g;
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
-            if (end && j > g) stage_push(S, g, j - g + 1);
+            if (end && j > g) {
+                stage_push(S, g, j - g + 1);
+                if (ppd->rle) dep[g] = key_depth(k, *ppd);
+            }
         }
     }
     stage_flush(S, gl);
@@ -493,8 +609,9 @@ struct CurLists {
 __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPlan plan,
                                                            const uint32_t* __restrict__ sa,
                                                            const uint32_t* __restrict__ rank,
-                                                           BlockMap bm,
-                                                           uint32_t h, uint32_t* __restrict__ key2)
+                                                           BlockMap bm, uint32_t h,
+                                                           const uint32_t* __restrict__ dep,
+                                                           uint32_t* __restrict__ key2)
 {
     const uint32_t gwave = blockIdx.x * (kThreads / QZ_WAVE) + threadIdx.x / QZ_WAVE;
     if (gwave >= plan.wave_base[kSmallClasses]) return;  // gather has no barriers
@@ -503,7 +620,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPla
     const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
     if (valid) {
         const uint32_t slot = start + e;
-        key2[slot] = round_key(rank, bm, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], dep ? dep[start] : h);
     }
 }
 
@@ -511,7 +628,8 @@ constexpr int kSmallTasks = 16;  // wave tasks per wave in k_sort_small (<= 4096
 
 __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan plan, const uint32_t* __restrict__ key2,
                                                          uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
-                                                         GroupLists gl)
+                                                         GroupLists gl, const uint32_t* __restrict__ dep_in,
+                                                         uint32_t* __restrict__ dep_out)
 {
     __shared__ Stage S;
     stage_init(S);
@@ -556,7 +674,10 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
             sa[start + e] = val;
             if (hp) rank[val] = start + hp;  // hp == 0: still the group's start
             const bool end = (e + 1 == size) || knext != key;
-            if (end && e > hp) stage_push(S, start + hp, e - hp + 1);
+            if (end && e > hp) {
+                stage_push(S, start + hp, e - hp + 1);
+                if (dep_out) dep_out[start + hp] = sub_depth(dep_in, start, key);
+            }
         }
         (void)c;
     }
@@ -568,28 +689,30 @@ __global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__
                                                          const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h,
+                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
                                                          uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
+    const uint32_t d = dep ? dep[g.start] : h;
     for (uint32_t i = threadIdx.x; i < g.size; i += QZ_WAVE) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, bm, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], d);
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h,
+                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
                                                          uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
+    const uint32_t d = dep ? dep[g.start] : h;
     for (uint32_t i = threadIdx.x; i < g.size; i += kThreads) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, bm, sa[slot], h);
+        key2[slot] = round_key(rank, bm, sa[slot], d);
     }
 }
 
@@ -731,7 +854,8 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
                                                               const uint32_t* __restrict__ cnt,
                                                               const uint32_t* __restrict__ key2,
                                                               uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
-                                                              GroupLists gl)
+                                                              GroupLists gl, const uint32_t* __restrict__ dep_in,
+                                                              uint32_t* __restrict__ dep_out)
 {
     constexpr uint32_t T = WAVES * QZ_WAVE;
     constexpr uint32_t MAXN = WAVES == 1 ? kWaveMax : kMedMax;
@@ -798,7 +922,10 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
         sa[g.start + i] = v;
         if (hp) rank[v] = g.start + hp;  // hp == 0: still the group's start
         const bool end = (i + 1 == g.size) || Kf[i + 1] != Kf[i];
-        if (end && i > hp) stage_push_e(S, E, g.start + hp, i - hp + 1, MAXN / 2);
+        if (end && i > hp) {
+            stage_push_e(S, E, g.start + hp, i - hp + 1, MAXN / 2);
+            if (dep_out) dep_out[g.start + hp] = sub_depth(dep_in, g.start, Kf[i]);
+        }
     }
     stage_flush_e(S, E, gl, MAXN / 2);
 }
@@ -811,7 +938,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
                                                          const uint32_t* __restrict__ lbase,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h,
+                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     const TileDesc td = tiles[blockIdx.x];
@@ -820,10 +947,11 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
     // same range: keys relative to its start need only bits(block size)
     const uint32_t blk = bm.find(g0);
     const uint32_t bstart = bm.off[blk], bend = bm.off[blk + 1];
+    const uint32_t d = dep ? dep[g0] : h;
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
         const uint32_t j = td.start + i;
         const uint32_t s = sa[g0 + (j - b0)];
-        const uint64_t p = (uint64_t)s + h;
+        const uint64_t p = (uint64_t)s + d;
         keys[j] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
         vals[j] = s;
     }
@@ -839,7 +967,9 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
                                                           const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ vals,
                                                           const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
-                                                          uint32_t* __restrict__ rank, GroupLists gl)
+                                                          uint32_t* __restrict__ rank, GroupLists gl, BlockMap bm,
+                                                          const uint32_t* __restrict__ dep_in,
+                                                          uint32_t* __restrict__ dep_out)
 {
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);
     __shared__ Stage S;
@@ -847,6 +977,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
     stage_init(S);
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
+    const uint32_t bstart = dep_out ? bm.off[bm.find(g0)] : 0u;  // keys are block-local: rank + 1 - bstart
     const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
     const uint32_t qb = w * Q;
     uint32_t m = 0;
@@ -881,7 +1012,10 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
             sa[slot] = s;
             if (hs != g0) rank[s] = hs;  // the first subgroup keeps the group's start
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
-            if (end && slot > hs) stage_push(S, hs, slot - hs + 1);
+            if (end && slot > hs) {
+                stage_push(S, hs, slot - hs + 1);
+                if (dep_out) dep_out[hs] = sub_depth(dep_in, g0, k ? k - 1u + bstart + 1u : 0u);
+            }
         }
     }
     stage_flush(S, gl);
@@ -1049,8 +1183,8 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     const size_t tmpw = scan_tmp_words(seg_words) + 1024;
     const uint64_t groups = pool_cap(c);
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
-    // text, out; sa, rank, keys0/1, vals0/1 (4 B each); bufB (key2 or radix histograms); bufA per tile
-    return 2 * (c + 64) + 6 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
+    // text, out; sa, rank, keys0/1, vals0/1, dep0/1 (4 B each); bufB (key2 or radix histograms); bufA per tile
+    return 2 * (c + 64) + 8 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
            (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
            2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
            sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
@@ -1078,10 +1212,12 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
     QZ_HIP(hipMalloc(&vals1_, c * 4));
+    QZ_HIP(hipMalloc(&dep_[0], c * 4));
+    QZ_HIP(hipMalloc(&dep_[1], c * 4));
     QZ_HIP(hipMalloc(&tmp_, tmpw * 4));
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
-    QZ_HIP(hipMalloc(&hist_, 256 * 4));
+    QZ_HIP(hipMalloc(&hist_, 264 * 4));
     QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
     for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
     QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
@@ -1100,8 +1236,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 
 void BwtEngine::release()
 {
-    void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, tmp_, off_, idx_, hist_,
-                  cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
+    void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, dep_[0], dep_[1], tmp_, off_,
+                  idx_, hist_, cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1109,6 +1245,7 @@ void BwtEngine::release()
     text_ = out_ = nullptr;
     sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = cnt_ = lbase_ = nullptr;
     keys0_ = keys1_ = nullptr;
+    dep_[0] = dep_[1] = nullptr;
     groups_[0] = groups_[1] = nullptr;
     tiles_ = tiles0_ = ppd_ = nullptr;
     btab_ = nullptr;
@@ -1129,7 +1266,14 @@ static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint
     return gl;
 }
 
-// Sorts the suffixes of the batch in text (device); off_ holds the offsets.
+// QZBSC_RLE=0|1 forces packed or run-length round-0 keys (default: by the
+// batch's mean run length).
+static int rle_force()
+{
+    const char* e = getenv("QZBSC_RLE");
+    return e ? (atoi(e) ? 1 : 0) : -1;
+}
+
 // Round 0 as one launch sequence with no host round trip, so it can be
 // replayed as a graph: clear the counters, count symbols, derive the packing
 // on the device, build the packed prefix keys, sort them (32 key bits),
@@ -1138,13 +1282,13 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
                               uint32_t hist_words, bool timed, hipStream_t s)
 {
     const BlockMap bmap_{off_, btab_, nblk};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 264u);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, 64u);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
                        text, n, hist_);
     QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_);
+    hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force());
     QZ_HIP(hipGetLastError());
     const uint32_t g = qz_div_up(n, kThreads);
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
@@ -1163,8 +1307,8 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
     const GroupLists gl = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_);
-    hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals, (const TileDesc*)tiles0_, bufA_, sa_,
-                       rank_, gl);
+    hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
+                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, gl, (const PackParams*)ppd_, dep_[0]);
     QZ_HIP(hipGetLastError());
     return 0;
 }
@@ -1202,7 +1346,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         QZ_TRY(prepare0(s));
         QZ_TRY(launch0(s));
     } else {
-        std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_};
+        std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_,
+                                     (uint64_t)(rle_force() + 1)};
         key.insert(key.end(), offs_host_.begin(), offs_host_.end());
         QZ_TRY(g0_.run(key, s, prepare0, launch0));
     }
@@ -1212,7 +1357,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     uint32_t maxb = 0;
     for (uint32_t b = 0; b < nblk; ++b) maxb = std::max(maxb, offs_host_[b + 1] - offs_host_[b]);
     const int lkey_bits = bitlen(maxb);  // block-local large-group keys are <= block size
-    uint32_t h = 0;  // k0, read back with the first counts
+    uint32_t h = 0;  // the doubling depth of packed keys: k0, read back with the first counts
+    bool rle = false;  // run-length keys: every group carries its own depth (dep_)
     int cur = 0;
     bool large_timed = false;
     int large_bits = 8;
@@ -1222,10 +1368,11 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         QZ_HIP(hipMemcpyAsync(hc, gls[cur].cnt, kClasses * 4, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipMemcpyAsync(hc + 16, overflow, 4, hipMemcpyDeviceToHost, s));
         if (rounds_ == 1)
-            QZ_HIP(hipMemcpyAsync(hc + 20, &((const PackParams*)ppd_)->k0, 4, hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipMemcpyAsync(hc + 20, &((const PackParams*)ppd_)->k0, 8, hipMemcpyDeviceToHost, s));  // k0, rle
         QZ_HIP(hipStreamSynchronize(s));
         if (rounds_ == 1) {
             h = ((volatile uint32_t*)hc)[20];
+            rle = ((volatile uint32_t*)hc)[21] != 0;
             k0_ = h;
         }
         uint32_t counts[kClasses];
@@ -1324,42 +1471,46 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         const TileDesc* ltiles = (const TileDesc*)tiles_;
         uint32_t* lk0 = keys0_;
         uint32_t* lk1 = keys1_;
+        // group depths: read from this round's buffer (written when the
+        // groups were emitted), new subgroups' depths go to the other one
+        const uint32_t* dep_in = rle ? dep_[cur] : nullptr;
+        uint32_t* dep_out = rle ? dep_[cur ^ 1] : nullptr;
 
         // phase A: gather every key of the round before any rank changes
         if (small_waves) {
-            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), occ_lds(kOccGatherSmall), s, cl, plan, sa_, rank_, bmap_,
-                                h, bufB_);
+            hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), occ_lds(kOccGatherSmall), s, cl,
+                               plan, sa_, rank_, bmap_, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
             hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave,
-                               sa_, rank_, bmap_, h, bufB_);
+                               sa_, rank_, bmap_, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
             hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, sa_,
-                               rank_, bmap_, h, bufB_);
+                               rank_, bmap_, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
             hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, sa_, rank_,
-                               bmap_, h, lk0, vals0_);
+                               bmap_, h, dep_in, lk0, vals0_);
             QZ_HIP(hipGetLastError());
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
         if (small_waves) {
             hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), occ_lds(kOccSortSmall), s, cl, plan,
-                               bufB_, sa_, rank_, N);
+                               bufB_, sa_, rank_, N, dep_in, dep_out);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
             hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave, bufB_,
-                               sa_, rank_, N);
+                               sa_, rank_, N, dep_in, dep_out);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
             hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, bufB_,
-                               sa_, rank_, N);
+                               sa_, rank_, N, dep_in, dep_out);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
@@ -1379,7 +1530,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
             hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), occ_lds(kOccFinalLarge), s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
-                               bufA_, sa_, rank_, N);
+                               bufA_, sa_, rank_, N, bmap_, dep_in, dep_out);
             QZ_HIP(hipGetLastError());
         }
         ++rounds_;

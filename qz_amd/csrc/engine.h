@@ -45,6 +45,7 @@ private:
     uint32_t* keys1_ = nullptr;
     uint32_t* vals0_ = nullptr;
     uint32_t* vals1_ = nullptr;
+    uint32_t* dep_[2] = {nullptr, nullptr};  // depth of each unsorted group at its start slot, per round parity
     uint32_t* tmp_ = nullptr;
     uint32_t* hist_ = nullptr;
     uint32_t* cnt_ = nullptr;      // group-list counters (two sets) + overflow flag

@@ -101,3 +101,22 @@ def test_pathological_repeats(gpu):
         print(f"n={blk.size}", flush=True)
         _check([blk], gpu)
     _check(blocks, gpu)
+
+
+@pytest.mark.parametrize("mode", ["1", "0"])
+def test_round0_key_modes_give_the_same_bwt(gpu, monkeypatch, mode):
+    """Run-length round-0 keys (per-group depths) and packed keys give the
+    oracle's BWT on every kind of block: forced on for reads, names and
+    random bytes, forced off for run-heavy qualities, alone and batched."""
+    monkeypatch.setenv("QZBSC_RLE", mode)
+    rng = np.random.default_rng(77)
+    blocks = [synth.quality_stream(20_000, 150, 33), synth.header_stream(20_000, 34),
+              synth.sequence_stream(200_000, 4_000, 150, 35, 0.002, 0.0001),
+              rng.integers(0, 256, size=50_000, dtype=np.uint8),
+              np.full(300_000, 70, np.uint8),
+              np.repeat(rng.integers(0, 4, 20_000).astype(np.uint8), rng.integers(1, 90, 20_000))[:800_000]]
+    print("key mode", mode, [b.size for b in blocks], flush=True)
+    _check(_cases(), gpu)
+    _check(blocks, gpu)
+    for blk in blocks[:2]:
+        _check([blk], gpu)
