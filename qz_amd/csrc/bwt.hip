@@ -343,41 +343,28 @@ __device__ __forceinline__ uint32_t key_depth(uint32_t key, const PackParams& pp
     return d;
 }
 
-// Run-length key of position i (block end `end`), see PackParams.
-__device__ __forceinline__ uint32_t rle_key(const uint8_t* __restrict__ text, const uint16_t* code, uint32_t i,
-                                            uint32_t end, const PackParams& pp)
-{
-    uint32_t key = 0, p = i;
-    for (uint32_t q = 0; q < pp.nseg; ++q) {
-        uint32_t part = 0;
-        if (p < end) {
-            const uint8_t b = text[p];
-            uint32_t r = 1;
-            while (r <= pp.cap && p + r < end && text[p + r] == b) ++r;
-            const uint32_t c = code[b];
-            uint32_t cls, v, len;
-            if (r > pp.cap) {
-                cls = 1u;
-                v = len = pp.cap;
-            } else {
-                const uint32_t nx = (p + r < end) ? code[text[p + r]] : 0u;
-                cls = nx < c ? 0u : 2u;
-                v = nx < c ? r : pp.cap + 1u - r;
-                len = r;
-            }
-            part = (c << (pp.lb + 2u)) | (cls << pp.lb) | v;
-            p += len;
-        }
-        key = (key << pp.segbits) | part;
-    }
-    return key << (32u - pp.nseg * pp.segbits);
-}
-
 // Packed prefix key of every position: the next k0 symbol codes in radix
 // sigma+1, code 0 past the block end. Each thread builds 16 consecutive keys
 // with a sliding window (drop the leading code, append one), restarting at
 // block starts, and stores them as four 16-byte vectors.
 constexpr int kKeyItems = 16;
+constexpr uint32_t kRleLook = 512;  // run-length keys read at most 511 positions ahead
+
+// A thread's kKeyItems consecutive keys, as 16-byte vectors when whole.
+__device__ __forceinline__ void store_keys(uint32_t* __restrict__ keys, uint32_t i0, uint32_t n,
+                                           const uint32_t (&out)[kKeyItems])
+{
+    if (i0 + kKeyItems <= n) {
+        uint4* dst = reinterpret_cast<uint4*>(keys + i0);
+#pragma unroll
+        for (int v = 0; v < kKeyItems / 4; ++v)
+            dst[v] = make_uint4(out[4 * v], out[4 * v + 1], out[4 * v + 2], out[4 * v + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kKeyItems; ++q)
+            if (i0 + q < n) keys[i0 + q] = out[q];
+    }
+}
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           BlockMap bm,
                                                           const PackParams* __restrict__ ppd,
@@ -388,17 +375,86 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
     const uint32_t radix = ppd->radix, k0 = ppd->k0;
     __syncthreads();
     const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kKeyItems;
-    if (i0 >= n) return;
-    if (ppd->rle) {
+    if (ppd->rle) {  // uniform: the whole grid takes this branch or none
+        // the workgroup's 4096 positions and the 511 the keys may look
+        // ahead, staged in LDS with each entry's remaining run length
+        // rem (capped at cap + 1; runs end at block ends)
+        constexpr uint32_t W = kThreads * kKeyItems + kRleLook;
+        constexpr uint32_t C = (W + kThreads - 1) / kThreads;  // entries per thread
+        __shared__ uint8_t win[W];
+        __shared__ uint16_t rem[W];
+        __shared__ uint8_t bstart[W];  // 1: a block starts here
         const PackParams& pp = *ppd;
-        uint32_t end = 0;
-        for (int q = 0; q < kKeyItems && i0 + q < n; ++q) {
-            const uint32_t i = i0 + q;
-            if (i >= end) end = bm.off[bm.find(i) + 1];
-            keys[i] = rle_key(text, code, i, end, pp);
+        const uint32_t base = blockIdx.x * kThreads * kKeyItems;
+        for (uint32_t k = threadIdx.x; k < W; k += kThreads) {
+            win[k] = base + k < n ? text[base + k] : 0;
+            bstart[k] = 0;
         }
+        __syncthreads();
+        {
+            // block starts inside the window (blocks of any size)
+            const uint32_t b0 = bm.find(base < n ? base : n - 1);
+            for (uint32_t b = b0 + 1 + threadIdx.x; b < bm.nblk; b += kThreads) {
+                const uint32_t o = bm.off[b];
+                if (o >= base + W || o >= n) break;
+                bstart[o - base] = 1;
+            }
+        }
+        __syncthreads();
+        {
+            // thread t: entries [t*C, t*C + C): the last one by a bounded
+            // forward scan, the others backwards from it
+            const uint32_t k0c = threadIdx.x * C;
+            const uint32_t k1c = k0c + C < W ? k0c + C : W;
+            if (k0c < k1c) {
+                uint32_t k = k1c - 1, r = 1;
+                while (r <= pp.cap && k + r < W && !bstart[k + r] && base + k + r < n && win[k + r] == win[k]) ++r;
+                rem[k] = (uint16_t)r;
+                while (k > k0c) {
+                    --k;
+                    const bool same = base + k + 1 < n && !bstart[k + 1] && win[k + 1] == win[k];
+                    r = same ? (r <= pp.cap ? r + 1 : r) : 1u;
+                    rem[k] = (uint16_t)r;
+                }
+            }
+        }
+        __syncthreads();
+        if (i0 >= n) return;
+        uint32_t end = 0, out[kKeyItems];
+#pragma unroll
+        for (int q = 0; q < kKeyItems; ++q) {
+            const uint32_t i = i0 + q;
+            out[q] = 0;
+            if (i < n) {
+                if (i >= end) end = bm.off[bm.find(i) + 1];
+                uint32_t key = 0, p = i;
+                for (uint32_t g = 0; g < pp.nseg; ++g) {
+                    uint32_t part = 0;
+                    if (p < end) {
+                        const uint32_t r = rem[p - base];
+                        const uint32_t c = code[win[p - base]];
+                        uint32_t cls, v, len;
+                        if (r > pp.cap) {
+                            cls = 1u;
+                            v = len = pp.cap;
+                        } else {
+                            const uint32_t nx = (p + r < end) ? code[win[p + r - base]] : 0u;
+                            cls = nx < c ? 0u : 2u;
+                            v = nx < c ? r : pp.cap + 1u - r;
+                            len = r;
+                        }
+                        part = (c << (pp.lb + 2u)) | (cls << pp.lb) | v;
+                        p += len;
+                    }
+                    key = (key << pp.segbits) | part;
+                }
+                out[q] = key << (32u - pp.nseg * pp.segbits);
+            }
+        }
+        store_keys(keys, i0, n, out);
         return;
     }
+    if (i0 >= n) return;
     uint32_t top = 1;  // weight of the leading code
     for (uint32_t j = 1; j < k0; ++j) top *= radix;
     uint32_t end = 0, key = 0;
@@ -422,16 +478,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         }
         out[q] = key;
     }
-    if (i0 + kKeyItems <= n) {
-        uint4* dst = reinterpret_cast<uint4*>(keys + i0);
-#pragma unroll
-        for (int v = 0; v < kKeyItems / 4; ++v)
-            dst[v] = make_uint4(out[4 * v], out[4 * v + 1], out[4 * v + 2], out[4 * v + 3]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < kKeyItems; ++q)
-            if (i0 + q < n) keys[i0 + q] = out[q];
-    }
+    store_keys(keys, i0, n, out);
 }
 
 // Symbol packing from the byte histogram (one workgroup of 256): present
