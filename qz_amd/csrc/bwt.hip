@@ -285,29 +285,18 @@ __device__ __forceinline__ uint32_t sub_depth(const uint32_t* __restrict__ dep_i
     return s < d ? 0xffffffffu : s;  // saturate (never reached: depth < 2^31)
 }
 
-// Byte histogram of the batch in hist[0..256) and, in hist[256], the number
-// of positions whose successor differs (run boundaries, block ends ignored).
 __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restrict__ text, uint32_t n,
                                                         uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[4][256];
-    __shared__ uint32_t bnd;
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
     for (int w = 0; w < 4; ++w) h[w][tid] = 0;
-    if (tid == 0) bnd = 0;
     __syncthreads();
-    uint32_t b = 0;
-    for (uint32_t i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) {
-        const uint8_t c = text[i];
-        atomicAdd(&h[wave][c], 1u);
-        b += (i + 1 < n && text[i + 1] != c) ? 1u : 0u;
-    }
-    for (int d = 1; d < QZ_WAVE; d <<= 1) b += __shfl_xor(b, d, QZ_WAVE);
-    if (lane_id() == 0 && b) atomicAdd(&bnd, b);
+    for (uint32_t i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
+        atomicAdd(&h[wave][text[i]], 1u);
     __syncthreads();
     uint32_t t = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
     if (t) atomicAdd(&hist[tid], t);
-    if (tid == 0 && bnd) atomicAdd(&hist[256], bnd);
 }
 
 // Round-0 keys. Packed: the next k0 symbol codes in radix sigma+1. Run
@@ -319,19 +308,35 @@ __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restric
 // followed by a smaller symbol sorts below any longer run of c, a run
 // followed by a larger one above it, so these keys order suffixes as their
 // text does; equal keys share the sum of their lengths (the key's depth).
+// The choice is per block (blocks are never compared in round 0): a block
+// whose mean run is at least kRleMinRun symbols takes run-length keys; when
+// any block does, every group of the batch carries its own depth.
 struct PackParams {
     uint16_t code[256];  // symbol -> 1..sigma (sigma may be 256)
     uint32_t radix;      // sigma + 1
     uint32_t k0;         // symbols per 32-bit key (packed keys)
-    uint32_t rle;        // 1: run-length keys, per-group depths
+    uint32_t rle;        // 1: some block has run-length keys (per-group depths)
     uint32_t sb, lb, nseg, segbits, cap;
+    int32_t force;       // -1 per block by its runs, 0 no block, 1 every block
 };
+constexpr uint32_t kRleMinRun = 3;
+
+// Whether block b takes run-length keys: brun[b] run boundaries among the
+// brun[nblk + b] sampled positions (k_block_runs); a block with fewer than
+// 64 samples keeps packed keys.
+__device__ __forceinline__ bool block_rle(const PackParams& pp, const BlockMap& bm, const uint32_t* __restrict__ brun,
+                                          uint32_t b)
+{
+    if (pp.force >= 0) return pp.force != 0;
+    const uint32_t seen = brun[bm.nblk + b];
+    return seen >= 64u && (uint64_t)seen >= (uint64_t)kRleMinRun * ((uint64_t)brun[b] + 1u);
+}
 
 // Symbols a round-0 key covers: k0 for packed keys, the sum of the run
 // lengths for run-length keys.
-__device__ __forceinline__ uint32_t key_depth(uint32_t key, const PackParams& pp)
+__device__ __forceinline__ uint32_t key_depth(uint32_t key, const PackParams& pp, bool rle)
 {
-    if (!pp.rle) return pp.k0;
+    if (!rle) return pp.k0;
     uint32_t d = 0;
     const uint32_t vmask = (1u << pp.lb) - 1u;
     for (uint32_t q = 0; q < pp.nseg; ++q) {
@@ -368,6 +373,7 @@ __device__ __forceinline__ void store_keys(uint32_t* __restrict__ keys, uint32_t
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           BlockMap bm,
                                                           const PackParams* __restrict__ ppd,
+                                                          const uint32_t* __restrict__ brun,
                                                           uint32_t* __restrict__ keys)
 {
     __shared__ uint16_t code[256];
@@ -421,12 +427,26 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         __syncthreads();
         if (i0 >= n) return;
         uint32_t end = 0, out[kKeyItems];
+        bool rleb = false;
 #pragma unroll
         for (int q = 0; q < kKeyItems; ++q) {
             const uint32_t i = i0 + q;
             out[q] = 0;
             if (i < n) {
-                if (i >= end) end = bm.off[bm.find(i) + 1];
+                if (i >= end) {
+                    const uint32_t b = bm.find(i);
+                    end = bm.off[b + 1];
+                    rleb = block_rle(pp, bm, brun, b);
+                }
+                if (!rleb) {  // a packed key from the window (k0 <= 32 < kRleLook)
+                    uint32_t key = 0;
+                    for (uint32_t j = 0; j < k0; ++j) {
+                        const uint32_t p = i + j;
+                        key = key * radix + ((p < end) ? code[win[p - base]] : 0u);
+                    }
+                    out[q] = key;
+                    continue;
+                }
                 uint32_t key = 0, p = i;
                 for (uint32_t g = 0; g < pp.nseg; ++g) {
                     uint32_t part = 0;
@@ -484,12 +504,59 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
 // Symbol packing from the byte histogram (one workgroup of 256): present
 // symbols get codes 1..sigma in byte order, 0 means past the block end, and
 // k0 is the largest k with (sigma+1)^k <= 2^32.
-// Run-length keys when the mean run is at least kRleMinRun symbols (force:
-// -1 decide, 0 never, 1 always).
-constexpr uint32_t kRleMinRun = 3;
-__global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp,
-                                                        int force)
+// Sampled run boundaries per block: one thread per 16-byte chunk at every
+// kRunSample-th position counts the chunk's positions whose successor in the
+// block differs (brun[b]) and the positions it looked at (brun[nblk + b]);
+// summed per wave when the wave lies in one block. The key mode only needs
+// the mean run length, which a 1/64 sample estimates well.
+constexpr uint32_t kRunSample = 1024;
+__global__ __launch_bounds__(kThreads) void k_block_runs(const uint8_t* __restrict__ text, uint32_t n, BlockMap bm,
+                                                         uint32_t* __restrict__ brun)
 {
+    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kRunSample;
+    const bool live = i0 < n;
+    const uint32_t b = live ? bm.find(i0) : 0xffffffffu;
+    uint32_t cnt = 0, seen = 0;
+    if (live) {
+        const uint32_t end = bm.off[b + 1];
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t i = i0 + q;
+            if (i + 1 >= end) break;  // the sample stays inside its block
+            cnt += text[i + 1] != text[i] ? 1u : 0u;
+            ++seen;
+        }
+    }
+    const uint32_t b0 = __shfl(b, 0, QZ_WAVE);
+    if (!__ballot(b != b0)) {  // the whole wave samples one block
+        for (int d = 1; d < QZ_WAVE; d <<= 1) {
+            cnt += __shfl_xor(cnt, d, QZ_WAVE);
+            seen += __shfl_xor(seen, d, QZ_WAVE);
+        }
+        if (lane_id() == 0 && b0 != 0xffffffffu) {
+            if (cnt) atomicAdd(&brun[b0], cnt);
+            if (seen) atomicAdd(&brun[bm.nblk + b0], seen);
+        }
+    } else if (live) {
+        if (cnt) atomicAdd(&brun[b], cnt);
+        if (seen) atomicAdd(&brun[bm.nblk + b], seen);
+    }
+}
+
+// Symbol packing and the key mode (force: -1 per block by its runs, 0 never
+// run-length keys, 1 always).
+__global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp,
+                                                        int force, BlockMap bm, const uint32_t* __restrict__ brun)
+{
+    __shared__ uint32_t any_rle;
+    if (threadIdx.x == 0) any_rle = 0;
+    __syncthreads();
+    if (force < 0) {
+        for (uint32_t b = threadIdx.x; b < bm.nblk; b += kThreads) {
+            const uint32_t seen = brun[bm.nblk + b];
+            if (seen >= 64u && (uint64_t)seen >= (uint64_t)kRleMinRun * ((uint64_t)brun[b] + 1u)) any_rle = 1;
+        }
+    }
+    __syncthreads();
     __shared__ uint32_t wsum[kThreads / QZ_WAVE];
     const uint32_t c = threadIdx.x, lane = lane_id(), w = c / QZ_WAVE;
     const uint32_t present = hist[c] ? 1u : 0u;
@@ -512,10 +579,7 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         }
         pp->radix = (uint32_t)radix;
         pp->k0 = k ? k : 1;
-        uint64_t n = 0;
-        for (int x = 0; x < 256; ++x) n += hist[x];
-        const uint64_t runs = (uint64_t)hist[256] + 1;
-        const bool rle = force >= 0 ? force != 0 : n >= (uint64_t)kRleMinRun * runs;
+        const bool rle = force >= 0 ? force != 0 : any_rle != 0;
         uint32_t sb = 0;
         while ((1ull << sb) < radix) ++sb;  // codes 0..sigma
         const uint32_t nseg = sb <= 5 ? 3u : 2u;
@@ -527,6 +591,7 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         pp->nseg = nseg;
         pp->segbits = segbits;
         pp->cap = (1u << lb) - 1u;
+        pp->force = force;
     }
 }
 
@@ -577,7 +642,8 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
                                                      const TileDesc* __restrict__ tiles,
                                                      const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                      uint32_t* __restrict__ rank, GroupLists gl,
-                                                     const PackParams* __restrict__ ppd, uint32_t* __restrict__ dep)
+                                                     const PackParams* __restrict__ ppd, uint32_t* __restrict__ dep,
+                                                     BlockMap bm, const uint32_t* __restrict__ brun)
 {
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);  // 1024 slots per wave
     __shared__ Stage S;
@@ -598,6 +664,8 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
     uint32_t carry = pre[blockIdx.x];
     for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
     const bool last_tile = td.tlocal + 1 == td.ntiles;
+    const bool track = ppd->rle != 0;  // groups carry depths
+    const bool rleb = track && block_rle(*ppd, bm, brun, td.block);
     for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
         const uint32_t i = b + lane;
         const bool valid = i < td.len;
@@ -618,7 +686,7 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
             if (end && j > g) {
                 stage_push(S, g, j - g + 1);
-                if (ppd->rle) dep[g] = key_depth(k, *ppd);
+                if (track) dep[g] = key_depth(k, *ppd, rleb);
             }
         }
     }
@@ -1232,7 +1300,7 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
     // text, out; sa, rank, keys0/1, vals0/1, dep0/1 (4 B each); bufB (key2 or radix histograms); bufA per tile
     return 2 * (c + 64) + 8 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
-           (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
+           3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
            2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
            sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
 }
@@ -1264,7 +1332,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&tmp_, tmpw * 4));
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
-    QZ_HIP(hipMalloc(&hist_, 264 * 4));
+    QZ_HIP(hipMalloc(&hist_, 256 * 4));
+    QZ_HIP(hipMalloc(&brun_, 2 * ((size_t)cb + 1) * 4));
     QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
     for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
     QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
@@ -1284,13 +1353,13 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 void BwtEngine::release()
 {
     void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, dep_[0], dep_[1], tmp_, off_,
-                  idx_, hist_, cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
+                  idx_, hist_, brun_, cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
     text_ = out_ = nullptr;
-    sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = cnt_ = lbase_ = nullptr;
+    sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = brun_ = cnt_ = lbase_ = nullptr;
     keys0_ = keys1_ = nullptr;
     dep_[0] = dep_[1] = nullptr;
     groups_[0] = groups_[1] = nullptr;
@@ -1329,17 +1398,22 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
                               uint32_t hist_words, bool timed, hipStream_t s)
 {
     const BlockMap bmap_{off_, btab_, nblk};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 264u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, brun_, 2 * nblk);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, 64u);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
                        text, n, hist_);
     QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force());
+    hipLaunchKernelGGL(k_block_runs, dim3(qz_div_up(n, kThreads * kRunSample)), dim3(kThreads), 0, s, text, n, bmap_,
+                       brun_);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force(), bmap_,
+                       (const uint32_t*)brun_);
     QZ_HIP(hipGetLastError());
     const uint32_t g = qz_div_up(n, kThreads);
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
-                        (const PackParams*)ppd_, keys0_);
+                        (const PackParams*)ppd_, (const uint32_t*)brun_, keys0_);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
@@ -1355,7 +1429,8 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
     const GroupLists gl = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_);
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
-                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, gl, (const PackParams*)ppd_, dep_[0]);
+                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, gl, (const PackParams*)ppd_, dep_[0], bmap_,
+                       (const uint32_t*)brun_);
     QZ_HIP(hipGetLastError());
     return 0;
 }

@@ -48,6 +48,7 @@ private:
     uint32_t* dep_[2] = {nullptr, nullptr};  // depth of each unsorted group at its start slot, per round parity
     uint32_t* tmp_ = nullptr;
     uint32_t* hist_ = nullptr;
+    uint32_t* brun_ = nullptr;     // run boundaries per block (round-0 key mode)
     uint32_t* cnt_ = nullptr;      // group-list counters (two sets) + overflow flag
     void* groups_[2] = {nullptr, nullptr};
     uint32_t* lbase_ = nullptr;    // prefix of large-group sizes
