@@ -1517,7 +1517,10 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             large_timed = false;
         }
         if (trace_) {
-            fprintf(stderr, "qzbsc: round %d h=%u groups", rounds_, h);
+            if (rle)
+                fprintf(stderr, "qzbsc: round %d (per-group depths) groups", rounds_);
+            else
+                fprintf(stderr, "qzbsc: round %d h=%u groups", rounds_, h);
             for (int k = 0; k < kClasses; ++k) fprintf(stderr, " %u", counts[k]);
             fprintf(stderr, "\n");
         }

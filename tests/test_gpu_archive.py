@@ -96,3 +96,24 @@ def test_libbsc_archive_is_rejected():
     g = (Path(__file__).resolve().parent / "golden" / "ref_ne50k.qz").read_bytes()
     with pytest.raises(BscError):
         A.decompress_archive(g)
+
+
+def test_archive_edge_cases():
+    """No records, one record, empty reads, a single long name."""
+    for text in (b"", b"@r\nACGT\n+\nIIII\n", b"@a\n\n+\n\n@b\nA\n+\n#\n",
+                 b"@" + b"n" * 5000 + b"\n" + b"ACGT" * 500 + b"\n+\n" + b"F" * 2000 + b"\n"):
+        arc = A.compress_fastq(text)
+        assert A.decompress_archive(arc) == text
+        assert A.parse_archive_header(arc).num_reads == text.count(b"\n") // 4
+
+
+def test_batch_forms_many_tiny_blocks():
+    """300 blocks of 1..299 bytes over 1..4 symbols: stored and coded blocks
+    mixed in one batch, each equal to its per-block call."""
+    rng = np.random.default_rng(12)
+    blocks = [rng.integers(65, 65 + int(rng.integers(1, 5)), int(rng.integers(1, 300)), dtype=np.uint8)
+              for _ in range(300)]
+    print("tiny blocks:", len(blocks), flush=True)
+    comp = bsc.compress_batch(blocks)
+    assert comp == [bsc.compress_adaptive_no_lzp(b) for b in blocks]
+    assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
