@@ -10,167 +10,92 @@
 // own: libbsc's qlfc_model constants are not restated, so its coded bytes
 // are not libbsc's.
 //
-// Model: every binary decision mixes three adaptive probabilities (a static
-// one per decision node, one selected by recent rank/run history, one by the
-// current symbol) in the logistic domain with per-context weights.
+// Model: every binary decision reads one adaptive probability from a context
+// made of the current run's symbol and a short history state; its adaptation
+// rate falls as 1/(n + 1.5) with the slot's update count n, down to 1/128.
 #include "qlfc_coder.h"
 #include "rangecoder.h"
 
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 namespace qz {
 namespace qlfc {
 
 namespace {
 
-// Logistic 4096/(1+exp(-x/256)) sampled at x = -2048..2048 step 128.
-const int kSquashPts[33] = {1,    2,    4,    6,    10,   17,   27,   45,   74,   120,  194,
-                            311,  488,  747,  1102, 1546, 2048, 2550, 2994, 3349, 3608, 3785,
-                            3902, 3976, 4022, 4051, 4069, 4079, 4086, 4090, 4092, 4094, 4095};
-
-struct Tables {
-    int16_t squash[4096];  // index x + 2048
-    int16_t stretch[4096];
-    uint16_t rate[256];    // adaptation rate after n updates, 1/4096 units
-    Tables()
+// Adaptation rate after n updates, in 1/4096 units.
+struct Rates {
+    uint16_t rate[256];
+    Rates()
     {
-        for (int d = -2048; d < 2048; ++d) {
-            const int x = d < -2047 ? -2047 : d;
-            const int w = x & 127, i = (x >> 7) + 16;
-            squash[d + 2048] = (int16_t)((kSquashPts[i] * (128 - w) + kSquashPts[i + 1] * w + 64) >> 7);
-        }
-        int pi = 0;
-        for (int x = -2047; x <= 2047; ++x) {
-            const int v = squash[x + 2048];
-            for (int p = pi; p <= v; ++p) stretch[p] = (int16_t)x;
-            pi = v + 1;
-        }
-        for (int p = pi; p < 4096; ++p) stretch[p] = 2047;
         for (int n = 0; n < 256; ++n) rate[n] = (uint16_t)(8192 / (2 * n + 3));
     }
 };
-// Built once at load time: the coding loops read it without a
-// first-use guard on every binary decision.
-const Tables g_tables;
-inline const Tables& tab() { return g_tables; }
+// Built once at load time: the coding loops read it without a first-use
+// guard on every binary decision.
+const Rates g_rates;
 
 struct Pred {
     uint16_t p;  // P(bit == 0), 16-bit
-    uint16_t n;
+    uint16_t n;  // updates so far, saturating at kLimit
 };
 
-struct Mix {
-    int32_t w[3];
-};
+constexpr int kLimit = 127;
 
-constexpr int kLimStatic = 60, kLimCtx = 30, kLr = 6;
-
-inline void upd(Pred& q, int bit, int lim)
+inline void upd(Pred& q, int bit)
 {
     const int target = bit ? 0 : 65535;
-    q.p = (uint16_t)(q.p + (((target - (int)q.p) * (int)tab().rate[q.n]) >> 12));
-    if (q.n < lim) ++q.n;
+    q.p = (uint16_t)(q.p + (((target - (int)q.p) * (int)g_rates.rate[q.n]) >> 12));
+    if (q.n < kLimit) ++q.n;
 }
 
 inline int bsr(uint32_t v) { return 31 - __builtin_clz(v); }
 
+// One binary decision coded with the probability held in q (12 bits for the
+// range coder, kept inside [1, 4095]).
 template <class IO>
-inline int cbit(IO& io, int bit, Pred& a, Pred& b, Pred& c, Mix& m)
-{
-    const Tables& t = tab();
-    const int s0 = t.stretch[a.p >> 4], s1 = t.stretch[b.p >> 4], s2 = t.stretch[c.p >> 4];
-    int d = (int)(((int64_t)m.w[0] * s0 + (int64_t)m.w[1] * s1 + (int64_t)m.w[2] * s2) >> 16);
-    d = d < -2047 ? -2047 : (d > 2047 ? 2047 : d);
-    const int pm = t.squash[d + 2048];
-    bit = io.code(bit, pm);
-    upd(a, bit, kLimStatic);
-    upd(b, bit, kLimCtx);
-    upd(c, bit, kLimCtx);
-    const int err = ((bit ? 0 : 4095) - pm) * kLr;
-    m.w[0] += (s0 * err) >> 14;
-    m.w[1] += (s1 * err) >> 14;
-    m.w[2] += (s2 * err) >> 14;
-    return bit;
-}
-
-template <class IO>
-inline int dbit(IO& io, int bit, Pred& q)
+inline int cbit(IO& io, int bit, Pred& q)
 {
     int p = q.p >> 4;
     p = p < 1 ? 1 : p;
     bit = io.code(bit, p);
-    upd(q, bit, kLimStatic);
+    upd(q, bit);
     return bit;
 }
 
+// Context slots. Rank v (0 .. nsym-1) is coded as a zero test, the unary
+// exponent e = floor(log2 v) and the e bits below the leading one; run length
+// L >= 1 as a test L > 1, the unary exponent of L and its bits below the
+// leading one (the top three by tree node, the rest by position).
 struct Model {
-    // rank: zero test, unary exponent, mantissa tree
-    Pred r0s, r0t[64], r0c[256];
-    Mix r0m[256];
-    Pred res[8], ret[64][8], rec[256][8];
-    Mix rem[8][8];
-    Pred rms[8][128], rmt[8][8][128], rmc[256][8][8];
-    Mix rmm[8];
-    // run length: >1 test, unary exponent, top mantissa bits, low bits
-    Pred l0s, l0t[128], l0c[256];
-    Mix l0m[256];
-    Pred les[32], lett[128][32], lec[256][32];
-    Mix lem[32];
-    Pred lms[32][8], lmt[8][32][8], lmc[256][32][8];
-    Mix lmm[32];
-    Pred lbs[32][32];
+    Pred rankZero[256][64];      // symbol x (3 recent zero-test flags, rank class)
+    Pred rankExp[256][8][8];     // symbol x rank class x exponent bit
+    Pred rankMant[256][8][8];    // symbol x exponent x tree node 1..7
+    Pred rankMantHi[8][128];     // exponent x tree node 8..127 (large alphabets)
+    Pred runGt1[256][128];       // symbol x (rank zero, 3 recent run flags, run class)
+    Pred runExp[256][8][32];     // symbol x run class x exponent bit
+    Pred runMant[256][32][8];    // symbol x exponent x tree node 1..7
+    Pred runLow[32][32];         // exponent x bit position
 };
+static_assert(std::is_trivially_copyable<Model>::value && sizeof(Model) % sizeof(Pred) == 0,
+              "the model is a flat array of slots");
 
-const Model& model_template()
+std::unique_ptr<Model> new_model()
 {
-    static const Model* m = [] {
-        Model* x = new Model;
-        Pred* p = reinterpret_cast<Pred*>(x);
-        // every member is an array of Pred or Mix; initialise by type
-        auto initp = [](Pred* a, size_t n) {
-            for (size_t i = 0; i < n; ++i) a[i] = Pred{32768, 0};
-        };
-        auto initm = [](Mix* a, size_t n) {
-            for (size_t i = 0; i < n; ++i) a[i] = Mix{{21845, 21845, 21845}};
-        };
-        (void)p;
-        initp(&x->r0s, 1);
-        initp(x->r0t, 64);
-        initp(x->r0c, 256);
-        initm(x->r0m, 256);
-        initp(x->res, 8);
-        initp(&x->ret[0][0], 64 * 8);
-        initp(&x->rec[0][0], 256 * 8);
-        initm(&x->rem[0][0], 64);
-        initp(&x->rms[0][0], 8 * 128);
-        initp(&x->rmt[0][0][0], 8 * 8 * 128);
-        initp(&x->rmc[0][0][0], 256 * 8 * 8);
-        initm(x->rmm, 8);
-        initp(&x->l0s, 1);
-        initp(x->l0t, 128);
-        initp(x->l0c, 256);
-        initm(x->l0m, 256);
-        initp(x->les, 32);
-        initp(&x->lett[0][0], 128 * 32);
-        initp(&x->lec[0][0], 256 * 32);
-        initm(x->lem, 32);
-        initp(&x->lms[0][0], 32 * 8);
-        initp(&x->lmt[0][0][0], 8 * 32 * 8);
-        initp(&x->lmc[0][0][0], 256 * 32 * 8);
-        initm(x->lmm, 32);
-        initp(&x->lbs[0][0], 32 * 32);
-        return x;
-    }();
-    return *m;
+    std::unique_ptr<Model> m(new Model);
+    Pred* p = reinterpret_cast<Pred*>(m.get());
+    std::fill(p, p + sizeof(Model) / sizeof(Pred), Pred{32768, 0});
+    return m;
 }
 
 struct RunState {
     int nsym = 0, maxE = 0, maxLE = 1;
     int flagsV = 0, flagsL = 0;
-    uint8_t rankHist[256];
-    uint8_t runHist[256];
+    uint8_t rankHist[256];  // per symbol: class of its last rank (0: zero, else min(e + 1, 7))
+    uint8_t runHist[256];   // per symbol: class of its last run length
     RunState(int nsym_, int n)
     {
         nsym = nsym_;
@@ -191,22 +116,19 @@ inline bool code_run(IO& io, Model& M, RunState& st, int c, int& rank, uint32_t&
     if (st.nsym > 1) {
         if (IO::kEnc) v = rank ? rank - 1 : st.nsym - 1;
         const int rh = st.rankHist[c];
-        const int ctxR = ((st.flagsV & 7) << 3) | rh;
-        const int nz = cbit(io, v != 0, M.r0s, M.r0t[ctxR], M.r0c[c], M.r0m[c]);
+        const int nz = cbit(io, v != 0, M.rankZero[c][((st.flagsV & 7) << 3) | rh]);
         int e = 0;
         if (nz) {
             const int ev = IO::kEnc ? bsr((uint32_t)v) : 0;
             int k = 0;
             for (; k < st.maxE; ++k)
-                if (!cbit(io, IO::kEnc ? (k < ev) : 0, M.res[k], M.ret[ctxR][k], M.rec[c][k], M.rem[rh][k])) break;
+                if (!cbit(io, IO::kEnc ? (k < ev) : 0, M.rankExp[c][rh][k])) break;
             e = k;
             int node = 1;
-            const int h3 = st.flagsV & 7;
             for (int b = e - 1; b >= 0; --b) {
                 const int bit = IO::kEnc ? ((v >> b) & 1) : 0;
-                const int got = cbit(io, bit, M.rms[e][node], M.rmt[h3][e][node], M.rmc[c][e][node < 8 ? node : 0],
-                                     M.rmm[e]);
-                node = node * 2 + got;
+                Pred& q = node < 8 ? M.rankMant[c][e][node] : M.rankMantHi[e][node];
+                node = node * 2 + cbit(io, bit, q);
             }
             v = node;
         }
@@ -220,25 +142,24 @@ inline bool code_run(IO& io, Model& M, RunState& st, int c, int& rank, uint32_t&
 
     const int rh = st.runHist[c];
     const int ctxL = ((v == 0) << 6) | ((st.flagsL & 7) << 3) | rh;
-    const int gt1 = cbit(io, IO::kEnc ? (len > 1) : 0, M.l0s, M.l0t[ctxL], M.l0c[c], M.l0m[c]);
+    const int gt1 = cbit(io, IO::kEnc ? (len > 1) : 0, M.runGt1[c][ctxL]);
     uint32_t L = 1;
     int e = 0;
     if (gt1) {
         const int ev = IO::kEnc ? bsr(len) : 0;
         int k = 1;
         for (; k < st.maxLE; ++k)
-            if (!cbit(io, IO::kEnc ? (k < ev) : 0, M.les[k], M.lett[ctxL][k], M.lec[c][k], M.lem[k])) break;
+            if (!cbit(io, IO::kEnc ? (k < ev) : 0, M.runExp[c][rh][k])) break;
         e = k;
-        const int h3 = st.flagsL & 7;
         int node = 1;
         for (int b = e - 1; b >= 0; --b) {
             const int bit = IO::kEnc ? (int)((len >> b) & 1u) : 0;
             int got;
             if (e - 1 - b < 3) {
-                got = cbit(io, bit, M.lms[e][node], M.lmt[h3][e][node], M.lmc[c][e][node], M.lmm[e]);
+                got = cbit(io, bit, M.runMant[c][e][node]);
                 node = node * 2 + got;
             } else {
-                got = dbit(io, bit, M.lbs[e][b]);
+                got = cbit(io, bit, M.runLow[e][b]);
             }
             L = L * 2 + (uint32_t)got;
         }
@@ -360,7 +281,7 @@ int encode_segment(const SegRuns& R, int n, uint8_t* out, int cap)
     uint8_t table[256];
     memcpy(table, R.table, (size_t)R.nsym);
     code_table(enc, table, R.nsym);
-    std::unique_ptr<Model> M(new Model(model_template()));
+    std::unique_ptr<Model> M = new_model();
     RunState st(R.nsym, n);
     for (uint32_t i = 0; i < R.count; ++i) {
         if (enc.eob()) return -3;
@@ -390,7 +311,7 @@ int decode_segment(const uint8_t* in, int in_size, uint8_t* out, int n)
     uint8_t list[256];
     const int nsym = code_table(dec, list, 0);
     if (nsym < 1) return -6;
-    std::unique_ptr<Model> M(new Model(model_template()));
+    std::unique_ptr<Model> M = new_model();
     RunState st(nsym, n);
     int pos = 0;
     while (pos < n) {
@@ -399,11 +320,15 @@ int decode_segment(const uint8_t* in, int in_size, uint8_t* out, int n)
         uint32_t len = 0;
         if (!code_run(dec, *M, st, c, rank, len)) return -6;
         if (len > (uint32_t)(n - pos) || dec.overrun()) return -6;
-        memset(out + pos, c, len);
+        if (len <= 16) {
+            for (uint32_t i = 0; i < len; ++i) out[pos + (int)i] = (uint8_t)c;
+        } else {
+            memset(out + pos, c, len);
+        }
         pos += (int)len;
         if (rank > 0) {
             if (rank >= nsym) return -6;
-            memmove(list, list + 1, (size_t)rank);
+            for (int i = 0; i < rank; ++i) list[i] = list[i + 1];
             list[rank] = (uint8_t)c;
         } else if (pos != n) {
             return -6;
