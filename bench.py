@@ -253,30 +253,31 @@ def archive_section(synth, reads: int, threads: int):
         back = A.decompress_archive(arc)
         td.append(time.perf_counter() - t)
     assert back == text
-    # the block calls alone: batched (all streams per call) ...
-    _, cs, cq, h, s, q = A.fastq_to_streams(text)
+    # the block calls alone: batched (every block of the three streams in one
+    # call each way, straight from and into one stream buffer) ...
+    _, cs, cq, buf, lens = A.fastq_to_stream_buffer(text)
+    B = bsc.BSC_BLOCK_SIZE
+    sizes = [min(B, n - i) for n in lens for i in range(0, n, B)]
     bc, bd = [], []
     for _ in range(2):
         t = time.perf_counter()
-        zs = A.compress_streams([h, s, q])
+        zs = bsc.compress_batch_buffer(buf, sizes)
         bc.append(time.perf_counter() - t)
         t = time.perf_counter()
-        _, hs, ss, qs = A.decompress_streams(A.archive_header(reads, *map(len, zs), cs, cq) + b"".join(zs))
+        out, _ = bsc.decompress_batch_buffer(zs)
         bd.append(time.perf_counter() - t)
-    assert (hs, ss, qs) == (h, s, q)
-    # ... and the per-block pattern over the same streams
+    assert np.array_equal(out, buf)
+    # ... and the per-block pattern over the same blocks
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    allp = [buf[offs[k]:offs[k + 1]] for k in range(len(sizes))]
     with ThreadPoolExecutor(threads) as pool:
-        def pieces(x):
-            a = np.frombuffer(x, np.uint8)
-            return [a[i:i + bsc.BSC_BLOCK_SIZE] for i in range(0, a.size, bsc.BSC_BLOCK_SIZE)]
-        allp = pieces(h) + pieces(s) + pieces(q)
         t = time.perf_counter()
         comp = list(pool.map(bsc.compress_adaptive_no_lzp, allp))
         pc = time.perf_counter() - t
         t = time.perf_counter()
         dec = list(pool.map(bsc.decompress, comp))
         pd = time.perf_counter() - t
-    assert dec == [p.tobytes() for p in allp]
+    assert comp == zs and dec == [p.tobytes() for p in allp]
     return {"workload": f"{reads} FASTQ records (Illumina names, 150 bp reads from a 5 Mbp genome, binned "
                         f"qualities) = {len(text)} B; default BSC pipeline (encoding 0, 7-bit lossless qualities)",
             "fastq_mb": round(mb, 1), "archive_bytes": len(arc), "ratio": round(len(arc) / len(text), 4),
@@ -287,8 +288,9 @@ def archive_section(synth, reads: int, threads: int):
             "per_block_compress_mb_s": round(mb / pc, 1), "per_block_decompress_mb_s": round(mb / pd, 1),
             "host_threads": threads,
             "note": "MB/s of FASTQ text. compress/decompress: end to end (FASTQ<->streams, container, blocks). "
-                    "batched_blocks / per_block: the block calls alone on the same streams, batched entry "
-                    "points vs one bsc_compress / bsc_decompress call per block from a pool of host threads"}
+                    "batched_blocks / per_block: the block calls alone on the same 25 MiB blocks, one "
+                    "qzbsc_compress_batch / qzbsc_decompress_batch call over all of them vs one bsc_compress / "
+                    "bsc_decompress call per block from a pool of host threads"}
 
 
 def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int):

@@ -257,17 +257,22 @@ def archive_header(num_reads: int, headers_len: int, sequences_len: int, qualiti
 
 def compress_streams(streams, block_size: int = bsc.BSC_BLOCK_SIZE) -> list:
     """Each stream in the multi-block framing (b"" for an empty stream). The
-    blocks of all streams go through one batch call: one device batch, and
+    blocks of all streams go through one batch call: device sub-batches, and
     the host coder on all of them at once."""
-    pieces, owner = [], []
-    for k, data in enumerate(streams):
-        a = np.frombuffer(data, dtype=np.uint8)
-        for i in range(0, a.size, block_size):
-            pieces.append(a[i:i + block_size])
+    buf = np.frombuffer(b"".join(streams), dtype=np.uint8)
+    return _compress_stream_buffer(buf, [len(x) for x in streams], block_size)
+
+
+def _compress_stream_buffer(buf: np.ndarray, lens, block_size: int) -> list:
+    """compress_streams for streams laid end to end in buf (lengths lens)."""
+    sizes, owner = [], []
+    for k, n in enumerate(lens):
+        for i in range(0, n, block_size):
+            sizes.append(min(block_size, n - i))
             owner.append(k)
-    blocks = bsc.compress_batch(pieces) if pieces else []
+    blocks = bsc.compress_batch_buffer(buf, sizes) if sizes else []
     out = []
-    for k in range(len(streams)):
+    for k in range(len(lens)):
         mine = [b for b, o in zip(blocks, owner) if o == k]
         if not mine:
             out.append(b"")
@@ -288,10 +293,10 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(u8p)
 
 
-def fastq_to_streams(fastq: bytes):
-    """FASTQ text -> (num_reads, const_seq_len, const_qual_len, header,
-    sequence, quality streams) by the native scan (qzbsc_fastq_to_streams);
-    the same streams as records_to_streams(parse_fastq(...))."""
+def fastq_to_stream_buffer(fastq: bytes):
+    """Native FASTQ scan (qzbsc_fastq_to_streams) into one uint8 array that
+    holds the header, sequence and quality streams end to end:
+    (num_reads, const_seq_len, const_qual_len, array, (hl, sl, ql))."""
     import ctypes as C
     src = np.frombuffer(fastq, dtype=np.uint8)
     info = (C.c_longlong * 6)()
@@ -299,11 +304,19 @@ def fastq_to_streams(fastq: bytes):
     if rc < 0:
         raise ArchiveError("malformed FASTQ (a record needs four lines, '@' and '+')")
     n, cs, cq, hl, sl, ql = (int(v) for v in info)
-    h, s, q = (np.empty(x, np.uint8) for x in (hl, sl, ql))
-    rc = lib().qzbsc_fastq_to_streams(_ptr(src), src.size, _ptr(h), _ptr(s), _ptr(q), info)
+    buf = np.empty(hl + sl + ql + 1, np.uint8)  # + 1: three valid pointers when streams are empty
+    rc = lib().qzbsc_fastq_to_streams(_ptr(src), src.size, _ptr(buf), _ptr(buf[hl:]), _ptr(buf[hl + sl:]), info)
     if rc < 0:
         raise ArchiveError("malformed FASTQ")
-    return n, cs, cq, h.tobytes(), s.tobytes(), q.tobytes()
+    return n, cs, cq, buf[:hl + sl + ql], (hl, sl, ql)
+
+
+def fastq_to_streams(fastq: bytes):
+    """FASTQ text -> (num_reads, const_seq_len, const_qual_len, header,
+    sequence, quality streams) by the native scan (qzbsc_fastq_to_streams);
+    the same streams as records_to_streams(parse_fastq(...))."""
+    n, cs, cq, buf, (hl, sl, ql) = fastq_to_stream_buffer(fastq)
+    return n, cs, cq, buf[:hl].tobytes(), buf[hl:hl + sl].tobytes(), buf[hl + sl:].tobytes()
 
 
 def compress_records(records, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
@@ -314,8 +327,8 @@ def compress_records(records, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
 
 def compress_fastq(fastq: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
     """FASTQ text -> .qz v2 archive bytes."""
-    n, cs, cq, h, s, q = fastq_to_streams(fastq)
-    hz, sz, qz = compress_streams([h, s, q], block_size)
+    n, cs, cq, buf, lens = fastq_to_stream_buffer(fastq)
+    hz, sz, qz = _compress_stream_buffer(buf, lens, block_size)
     return archive_header(n, len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
 
 
@@ -481,27 +494,58 @@ def streams_to_records(h: ArchiveHeader, hs: bytes, ss: bytes, qs: bytes) -> lis
     return [FastqRecord(hs[int(a):int(a + b)], seqs[i], quals[i]) for i, (a, b) in enumerate(zip(hst, hln))]
 
 
-def streams_to_fastq(h: ArchiveHeader, hs: bytes, ss: bytes, qs: bytes) -> bytes:
-    """The decoded streams -> FASTQ text by the native assembly
-    (qzbsc_streams_to_fastq); the same text as
+def _new_bytes(size: int):
+    """An uninitialised bytes object of `size` bytes and a pointer to its
+    storage, for native code to fill before the object is used anywhere
+    (CPython's PyBytes_FromStringAndSize(NULL, size) idiom)."""
+    import ctypes as C
+    api = C.pythonapi
+    api.PyBytes_FromStringAndSize.restype = C.py_object
+    api.PyBytes_FromStringAndSize.argtypes = (C.c_void_p, C.c_ssize_t)
+    api.PyBytes_AsString.restype = C.c_void_p
+    api.PyBytes_AsString.argtypes = (C.py_object,)
+    b = api.PyBytes_FromStringAndSize(None, size)
+    return b, C.cast(api.PyBytes_AsString(b), u8p)
+
+
+def streams_to_fastq(h: ArchiveHeader, hs, ss, qs) -> bytes:
+    """The decoded streams (bytes or uint8 arrays) -> FASTQ text by the
+    native assembly (qzbsc_streams_to_fastq); the same text as
     format_fastq(streams_to_records(...))."""
-    a = [np.frombuffer(x, dtype=np.uint8) for x in (hs, ss, qs)]
+    a = [np.frombuffer(x, dtype=np.uint8) if len(x) else np.zeros(1, np.uint8)[:0] for x in (hs, ss, qs)]
     args = (_ptr(a[0]), a[0].size, _ptr(a[1]), a[1].size, _ptr(a[2]), a[2].size, h.num_reads, h.const_seq_len,
             h.const_qual_len)
     size = lib().qzbsc_streams_to_fastq(*args, None, 0)
     if size < 0:
         raise ArchiveError("truncated stream: the streams hold fewer records than the header states")
-    out = np.empty(size, np.uint8)
-    rc = lib().qzbsc_streams_to_fastq(*args, _ptr(out), size)
+    out, ptr = _new_bytes(size)
+    rc = lib().qzbsc_streams_to_fastq(*args, ptr, size)
     if rc != size:
         raise ArchiveError("record assembly failed")
-    return out.tobytes()
+    return out
 
 
-def decompress_archive(buf, batch: int = DECOMPRESS_BATCH_SIZE) -> bytes:
-    """.qz v2 archive bytes -> FASTQ text."""
-    h, hs, ss, qs = decompress_streams(buf, batch)
-    return streams_to_fastq(h, hs, ss, qs)
+def _decompress_stream_buffer(buf):
+    """(header, array, (hl, sl, ql)): every block of the three streams in
+    one batch call, decoded end to end into one array, which then holds the
+    header, sequence and quality streams in that order."""
+    h = parse_archive_header(buf)
+    _check_supported(h)
+    per = [stream_blocks(buf, o, n) for o, n in stream_offsets(h)]
+    out, dsz = bsc.decompress_batch_buffer(per[0] + per[1] + per[2])
+    n0, n1 = len(per[0]), len(per[0]) + len(per[1])
+    return h, out, (sum(dsz[:n0]), sum(dsz[n0:n1]), sum(dsz[n1:]))
+
+
+def decompress_archive(buf, batch: int | None = None) -> bytes:
+    """.qz v2 archive bytes -> FASTQ text. By default all blocks go through
+    one batch call; batch=k decodes k blocks of every stream per call, as
+    qz's decompressor threads do (decompress_streams)."""
+    if batch is not None:
+        h, hs, ss, qs = decompress_streams(buf, batch)
+        return streams_to_fastq(h, hs, ss, qs)
+    h, out, (hl, sl, ql) = _decompress_stream_buffer(buf)
+    return streams_to_fastq(h, out[:hl], out[hl:hl + sl], out[hl + sl:hl + sl + ql])
 
 
 def compress_file(fastq_path: str, archive_path: str) -> int:

@@ -103,29 +103,43 @@ def compress_adaptive_mt_no_lzp(data) -> bytes:
 def compress_batch(blocks, lzp_hash_size: int = 0, lzp_min_len: int = 0,
                    block_sorter: int = LIBBSC_BLOCKSORTER_BWT, coder: int = LIBBSC_CODER_QLFC_ADAPTIVE,
                    features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> list:
-    """Many blocks through one qzbsc_compress_batch call (one device batch,
-    host coder on one thread per block); the same bytes per block as
-    compress_with_params. Replaces the per-block par_iter of
+    """Many blocks through one qzbsc_compress_batch call (device
+    sub-batches, host coder on all segments at once); the same bytes per
+    block as compress_with_params. Replaces the per-block par_iter of
     compress_stream_to_bsc_blocks (mod.rs:388-413)."""
-    _ensure_initialized()
     arrs = [_arr(b) for b in blocks]
     if not arrs:
         return []
-    for a in arrs:
-        if a.size > _I32_MAX:
-            raise BscError(LIBBSC_BAD_PARAMETER, f"BSC: input too large ({a.size} bytes, max {_I32_MAX})")
-    sizes = np.array([a.size for a in arrs], dtype=np.int32)
     inp = np.concatenate(arrs) if len(arrs) > 1 else arrs[0]
+    return compress_batch_buffer(inp, [a.size for a in arrs], lzp_hash_size, lzp_min_len, block_sorter, coder,
+                                 features)
+
+
+def compress_batch_buffer(buf: np.ndarray, sizes, lzp_hash_size: int = 0, lzp_min_len: int = 0,
+                          block_sorter: int = LIBBSC_BLOCKSORTER_BWT, coder: int = LIBBSC_CODER_QLFC_ADAPTIVE,
+                          features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> list:
+    """compress_batch over consecutive blocks of one uint8 buffer (block k
+    is the sizes[k] bytes after the first k), without gathering them."""
+    _ensure_initialized()
+    if not len(sizes):
+        return []
+    for n in sizes:
+        if n > _I32_MAX:
+            raise BscError(LIBBSC_BAD_PARAMETER, f"BSC: input too large ({n} bytes, max {_I32_MAX})")
+    sizes = np.asarray(sizes, dtype=np.int32)
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    if int(sizes.astype(np.int64).sum()) > buf.size:
+        raise BscError(LIBBSC_BAD_PARAMETER, "BSC: block sizes exceed the buffer")
     caps = sizes.astype(np.int64) + LIBBSC_HEADER_SIZE + 1024
     out = np.empty(int(caps.sum()), dtype=np.uint8)
-    osz = np.zeros(len(arrs), dtype=np.int32)
-    rc = lib().qzbsc_compress_batch(inp.ctypes.data_as(u8p), sizes.ctypes.data_as(i32p), len(arrs),
+    osz = np.zeros(len(sizes), dtype=np.int32)
+    rc = lib().qzbsc_compress_batch(buf.ctypes.data_as(u8p), sizes.ctypes.data_as(i32p), len(sizes),
                                     out.ctypes.data_as(u8p), osz.ctypes.data_as(i32p), lzp_hash_size, lzp_min_len,
                                     block_sorter, coder, features)
     if rc < 0:
         raise BscError(rc, "BSC batch compression")
     res, off = [], 0
-    for k in range(len(arrs)):
+    for k in range(len(sizes)):
         res.append(out[off:off + int(osz[k])].tobytes())
         off += int(caps[k])
     return res
@@ -133,14 +147,25 @@ def compress_batch(blocks, lzp_hash_size: int = 0, lzp_min_len: int = 0,
 
 def decompress_batch(blocks, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA) -> list:
     """Many blocks through one qzbsc_decompress_batch call: host coder
-    decode per block in parallel, one device batch of inverse BWTs. Each
-    block is checked as bsc.rs:384-436 checks one (size, header, all three
-    checksums). Replaces the batches of 8 per-block decompress calls of
-    decompress_impl.rs:97-121."""
+    decode of all segments in parallel, device sub-batches of inverse BWTs.
+    Each block is checked as bsc.rs:384-436 checks one (size, header, all
+    three checksums). Replaces the batches of 8 per-block decompress calls
+    of decompress_impl.rs:97-121."""
+    out, dsz = decompress_batch_buffer(blocks, features)
+    res, off = [], 0
+    for n in dsz:
+        res.append(out[off:off + n].tobytes())
+        off += n
+    return res
+
+
+def decompress_batch_buffer(blocks, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FEATURE_CUDA):
+    """decompress_batch with the decoded blocks laid end to end in one
+    uint8 array: (array, list of block sizes)."""
     _ensure_initialized()
     arrs = [_arr(b) for b in blocks]
     if not arrs:
-        return []
+        return np.zeros(0, np.uint8), []
     dsz = np.zeros(len(arrs), dtype=np.int32)
     for k, a in enumerate(arrs):
         if a.size < LIBBSC_HEADER_SIZE:
@@ -164,11 +189,7 @@ def decompress_batch(blocks, features: int = LIBBSC_FEATURE_FASTMODE | LIBBSC_FE
                                       rcs.ctypes.data_as(i32p))
     if rc != LIBBSC_NO_ERROR:
         raise BscError(rc, "BSC batch decompression")
-    res, off = [], 0
-    for k in range(len(arrs)):
-        res.append(out[off:off + int(dsz[k])].tobytes())
-        off += int(dsz[k])
-    return res
+    return out, [int(n) for n in dsz]
 
 
 def _compress_parallel_with(data, block_compressor) -> bytes:

@@ -30,7 +30,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -184,10 +188,11 @@ int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parall
 }
 
 // Host copy of the front end's results for a whole batch (front.run() done on
-// the batch's BWT `dbwt` of n bytes): per run symbol, rank and length, per
-// segment symbol table and symbol count, and the BWT itself (stored
-// segments copy it).
+// the batch's BWT `dbwt` of n bytes): the segment list, per run symbol, rank
+// and length, per segment symbol table and symbol count, and the BWT itself
+// (stored segments copy it). It outlives the next front.run() on the slot.
 struct FrontHost {
+    std::vector<QlfcFront::Seg> segs;
     std::vector<uint8_t> sym, rank, tables, bwt;
     std::vector<uint32_t> len;
     std::vector<uint16_t> nsym;
@@ -197,6 +202,7 @@ static int front_to_host(const QlfcFront& front, const uint8_t* dbwt, uint32_t n
 {
     const uint32_t R = front.runs_;  // runs of every segment, laid out in segment order
     const size_t S = front.segs_.size();
+    fh.segs = front.segs_;
     fh.sym.resize(R);
     fh.rank.resize(R);
     fh.len.resize(R);
@@ -225,11 +231,11 @@ struct SegCode {
 
 // Codes every segment of the batch from the host copy of the front end, all
 // segments of all blocks in parallel.
-static void code_segments_host(const QlfcFront& front, const FrontHost& fh, std::vector<SegCode>& sc)
+static void code_segments_host(const FrontHost& fh, std::vector<SegCode>& sc)
 {
-    sc.assign(front.segs_.size(), SegCode{});
-    parallel_blocks((int)front.segs_.size(), [&](int i) {
-        const auto& sg = front.segs_[i];
+    sc.assign(fh.segs.size(), SegCode{});
+    parallel_blocks((int)fh.segs.size(), [&](int i) {
+        const auto& sg = fh.segs[i];
         qlfc::SegRuns Rn;
         Rn.sym = fh.sym.data() + sg.r0;
         Rn.rank = fh.rank.data() + sg.r0;
@@ -244,20 +250,20 @@ static void code_segments_host(const QlfcFront& front, const FrontHost& fh, std:
 
 // Coder payload of batch block `block` from its coded segments (each block's
 // segments are taken once).
-static int payload_from_segments(const QlfcFront& front, const FrontHost& fh, std::vector<SegCode>& sc,
+static int payload_from_segments(const FrontHost& fh, std::vector<SegCode>& sc,
                                  uint32_t block, int n, uint8_t* out, int cap)
 {
     std::vector<int> sidx;
-    for (int i = 0; i < (int)front.segs_.size(); ++i)
-        if (front.segs_[i].block == block) sidx.push_back(i);
+    for (int i = 0; i < (int)fh.segs.size(); ++i)
+        if (fh.segs[i].block == block) sidx.push_back(i);
     const int nb = (int)sidx.size();
     if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_GPU_ERROR;
     int size[qlfc::kMaxSegments], start[qlfc::kMaxSegments];
     std::vector<std::vector<uint8_t>> segout(nb);
     std::vector<int> res(nb);
     for (int k = 0; k < nb; ++k) {
-        size[k] = (int)front.segs_[sidx[k]].size;
-        start[k] = (int)front.segs_[sidx[k]].start;
+        size[k] = (int)fh.segs[sidx[k]].size;
+        start[k] = (int)fh.segs[sidx[k]].start;
         segout[k] = std::move(sc[sidx[k]].out);
         res[k] = sc[sidx[k]].res;
     }
@@ -271,23 +277,23 @@ static int payload_from_segments(const QlfcFront& front, const FrontHost& fh, st
 }
 
 // Coder payload of batch block `block` from the host copy of the front end.
-static int coder_compress_host(const QlfcFront& front, const FrontHost& fh, uint32_t block, int n, uint8_t* out,
+static int coder_compress_host(const FrontHost& fh, uint32_t block, int n, uint8_t* out,
                                int cap, bool parallel)
 {
     std::vector<int> sidx;
-    for (int i = 0; i < (int)front.segs_.size(); ++i)
-        if (front.segs_[i].block == block) sidx.push_back(i);
+    for (int i = 0; i < (int)fh.segs.size(); ++i)
+        if (fh.segs[i].block == block) sidx.push_back(i);
     const int nb = (int)sidx.size();
     if (nb < 1 || nb > qlfc::kMaxSegments) return QZ_GPU_ERROR;
     int size[qlfc::kMaxSegments], start[qlfc::kMaxSegments];
     for (int k = 0; k < nb; ++k) {
-        size[k] = (int)front.segs_[sidx[k]].size;
-        start[k] = (int)front.segs_[sidx[k]].start;
+        size[k] = (int)fh.segs[sidx[k]].size;
+        start[k] = (int)fh.segs[sidx[k]].start;
     }
     std::vector<std::vector<uint8_t>> segout(nb);
     std::vector<int> res(nb, -1);
     for_segments(nb, parallel, [&](int k) {
-        const auto& sg = front.segs_[sidx[k]];
+        const auto& sg = fh.segs[sidx[k]];
         qlfc::SegRuns Rn;
         Rn.sym = fh.sym.data() + sg.r0;
         Rn.rank = fh.rank.data() + sg.r0;
@@ -492,6 +498,9 @@ int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, u
 
 // Device batches of the batch entry points hold at most this many bytes.
 constexpr uint64_t kBatchBytes = 1ull << 30;
+// Decode pipeline: device sub-batches of about 1/kPipeParts of a batch's
+// BWT bytes, none smaller than kPipeBytes.
+constexpr uint64_t kPipeParts = 4, kPipeBytes = 32ull << 20;
 
 // LZP blocks: the size m of the LZP data the BWT ran on (0 if the block
 // carries no LZP parameters), checked against the header.
@@ -552,7 +561,7 @@ QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, in
     QZ_TRY(front_to_host(sl.front, sl.fwd.out_, (uint32_t)dn, fh, sl.stream));
     const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
     return finish_block_with(input, n, dn, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
-        return coder_compress_host(sl.front, fh, 0, dn, p, cap, par);
+        return coder_compress_host(fh, 0, dn, p, cap, par);
     });
 }
 
@@ -605,10 +614,10 @@ QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned
 // Batch form of bsc_compress. Block b (sizes[b] bytes) starts at input +
 // sum of the earlier sizes; its output goes to output + sum over c < b of
 // (sizes[c] + 28 + 1024), libbsc's per-block buffer contract, and its size
-// (or a negative libbsc code) to out_sizes[b]. One device batch (up to
-// kBatchBytes) sorts the suffixes of all its blocks, the front end runs on
-// the device over the batch, and the host codes the blocks in parallel.
-// Returns 0, or the first block's error.
+// (or a negative libbsc code) to out_sizes[b]. Device sub-batches (up to
+// kBatchBytes each) sort the suffixes of their blocks and run the front end;
+// the host codes the segments of one sub-batch in parallel while the device
+// runs the next. Returns 0, or the first block's error.
 QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes, int nblocks, unsigned char* output,
                                    int* out_sizes, int lzpHashSize, int lzpMinLen, int blockSorter, int coder,
                                    int features)
@@ -647,7 +656,15 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
     SlotGuard g;
     if (!g.ok()) return QZ_GPU_ERROR;
     Slot& sl = g.slot();
-    auto flush = [&](const std::vector<int>& list) -> int {
+    // Device sub-batches of about 1/kPipeParts of the bytes (at least
+    // kPipeBytes, at most kBatchBytes): while the host pool codes the
+    // segments of sub-batch j, this thread sorts j + 1 on the device.
+    struct Job {
+        std::vector<int> list, prim, aux;
+        FrontHost fh;
+    };
+    auto run_device = [&](Job& job) -> int {
+        const std::vector<int>& list = job.list;
         const int k = (int)list.size();
         std::vector<uint32_t> offs(k + 1, 0);
         for (int i = 0; i < k; ++i) offs[i + 1] = offs[i] + (uint32_t)dn[list[i]];
@@ -657,37 +674,53 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
             const uint8_t* data = lz[b].empty() ? input + ioff[b] : lz[b].data();
             QZ_HIP(hipMemcpyAsync(sl.fwd.text_ + offs[i], data, (size_t)dn[b], hipMemcpyHostToDevice, sl.stream));
         }
-        std::vector<int> prim(k), aux((size_t)k * 256);
-        QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, prim.data(), aux.data(), nullptr));
+        job.prim.assign(k, 0);
+        job.aux.assign((size_t)k * 256, 0);
+        QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, job.prim.data(), job.aux.data(), nullptr));
         QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
-        FrontHost fh;
-        QZ_TRY(front_to_host(sl.front, sl.fwd.out_, offs[k], fh, sl.stream));
+        return front_to_host(sl.front, sl.fwd.out_, offs[k], job.fh, sl.stream);
+    };
+    auto code = [&](Job& job) {
         std::vector<SegCode> sc;
-        code_segments_host(sl.front, fh, sc);  // every segment of every block at once
-        parallel_blocks(k, [&](int i) {
-            const int b = list[i];
-            out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], prim[i],
-                                             aux.data() + (size_t)i * 256, output + ooff[b],
+        code_segments_host(job.fh, sc);  // every segment of every block at once
+        parallel_blocks((int)job.list.size(), [&](int i) {
+            const int b = job.list[i];
+            out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], job.prim[i],
+                                             job.aux.data() + (size_t)i * 256, output + ooff[b],
                                              [&](uint8_t* p, int cap) {
-                                                 return payload_from_segments(sl.front, fh, sc, (uint32_t)i, dn[b],
-                                                                              p, cap);
+                                                 return payload_from_segments(job.fh, sc, (uint32_t)i, dn[b], p,
+                                                                              cap);
                                              });
         });
-        return 0;
+        job.fh = FrontHost{};
     };
-    std::vector<int> list;
-    uint64_t bytes = 0;
-    for (int b = 0; b < nblocks; ++b) {
-        if (sizes[b] <= kHeader) continue;
-        if (!list.empty() && bytes + (uint64_t)dn[b] > kBatchBytes) {
-            QZ_TRY(flush(list));
-            list.clear();
-            bytes = 0;
+    std::vector<Job> jobs;
+    {
+        uint64_t total = 0;
+        for (int b = 0; b < nblocks; ++b)
+            if (sizes[b] > kHeader) total += (uint64_t)dn[b];
+        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / kPipeParts));
+        uint64_t bytes = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            if (sizes[b] <= kHeader) continue;
+            if (jobs.empty() || (bytes + (uint64_t)dn[b] > target && !jobs.back().list.empty())) {
+                jobs.emplace_back();
+                bytes = 0;
+            }
+            jobs.back().list.push_back(b);
+            bytes += (uint64_t)dn[b];
         }
-        list.push_back(b);
-        bytes += (uint64_t)dn[b];
     }
-    if (!list.empty()) QZ_TRY(flush(list));
+    int dev_rc = 0;
+    std::thread coding;
+    for (auto& job : jobs) {
+        dev_rc = run_device(job);
+        if (coding.joinable()) coding.join();
+        if (dev_rc) break;
+        coding = std::thread(code, std::ref(job));
+    }
+    if (coding.joinable()) coding.join();
+    if (dev_rc) return dev_rc;
     for (int b = 0; b < nblocks; ++b)
         if (out_sizes[b] < 0) return out_sizes[b];
     return LIBBSC_NO_ERROR;
@@ -696,10 +729,10 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
 // Batch form of bsc_decompress. Block b (in_sizes[b] bytes) starts at input
 // + sum of the earlier in_sizes and decodes into output + sum of the earlier
 // out_sizes, out_sizes[b] being at least its data size (bsc_block_info).
-// The host decodes every coder payload in parallel, then one device batch
-// (up to kBatchBytes) inverts all the BWTs; each block's result (0 or a
-// negative libbsc code) goes to rcs[b]. Returns 0, or the first block's
-// error.
+// The host decodes the coder segments in parallel, sub-batch by sub-batch,
+// and each device sub-batch (up to kBatchBytes) inverts its BWTs as soon as
+// its segments are decoded; each block's result (0 or a negative libbsc
+// code) goes to rcs[b]. Returns 0, or the first block's error.
 QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_sizes, int nblocks,
                                      unsigned char* output, const int* out_sizes, int features, int* rcs)
 {
@@ -747,30 +780,35 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
         }
         rcs[b] = rc;
     });
+    // Device sub-batches: the blocks to invert, in order, each sub-batch
+    // about 1/kPipeParts of their BWT bytes (at least kPipeBytes, at most
+    // kBatchBytes). Segment tasks run in sub-batch order, longest first within
+    // one, on a host pool while this thread inverts each sub-batch on the
+    // device as soon as its last segment is decoded.
+    std::vector<int> stage(nblocks, 0);
+    std::vector<std::vector<int>> batches;
     {
-        // every coder segment of every block at once
-        std::vector<std::pair<int, const SegTask*>> all;
-        for (int b = 0; b < nblocks; ++b)
-            for (const auto& t : tasks[b]) all.push_back({b, &t});
-        std::vector<int> src(all.size(), 0);
-        parallel_blocks((int)all.size(), [&](int i) { src[i] = run_seg_task(*all[i].second); });
-        for (size_t i = 0; i < all.size(); ++i)
-            if (src[i] && !rcs[all[i].first]) rcs[all[i].first] = QZ_DATA_CORRUPT;
-    }
-    std::vector<std::vector<int>> batches(1);
-    uint64_t bytes = 0;
-    for (int b = 0; b < nblocks; ++b) {
-        if (rcs[b] || !need[b]) continue;
-        if (bh[b].index < 1 || bh[b].index > need[b]) {  // a primary the inverse cannot take
-            rcs[b] = QZ_DATA_CORRUPT;
-            continue;
+        uint64_t total = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            if (rcs[b] || !need[b]) continue;
+            if (bh[b].index < 1 || bh[b].index > need[b]) {  // a primary the inverse cannot take
+                rcs[b] = QZ_DATA_CORRUPT;
+                continue;
+            }
+            total += (uint64_t)need[b];
         }
-        if (!batches.back().empty() && bytes + (uint64_t)need[b] > kBatchBytes) {
-            batches.emplace_back();
-            bytes = 0;
+        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / kPipeParts));
+        uint64_t bytes = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            if (rcs[b] || !need[b]) continue;
+            if (batches.empty() || (bytes + (uint64_t)need[b] > target && !batches.back().empty())) {
+                batches.emplace_back();
+                bytes = 0;
+            }
+            batches.back().push_back(b);
+            bytes += (uint64_t)need[b];
+            stage[b] = (int)batches.size() - 1;
         }
-        batches.back().push_back(b);
-        bytes += (uint64_t)need[b];
     }
     auto flush = [&](Slot& sl, const std::vector<int>& list) -> int {
         const int k = (int)list.size();
@@ -791,11 +829,53 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
         QZ_HIP(hipStreamSynchronize(sl.stream));
         return 0;
     };
-    if (!batches[0].empty()) {  // stored and single-byte blocks need no device
-        SlotGuard g;
-        if (!g.ok()) return QZ_GPU_ERROR;
-        for (const auto& list : batches) QZ_TRY(flush(g.slot(), list));
+    std::unique_ptr<SlotGuard> guard;
+    if (!batches.empty()) {  // stored and single-byte blocks need no device
+        guard.reset(new SlotGuard);
+        if (!guard->ok()) return QZ_GPU_ERROR;
     }
+    struct Item {
+        int b;
+        const SegTask* t;
+    };
+    std::vector<Item> all;
+    for (int b = 0; b < nblocks; ++b)
+        for (const auto& t : tasks[b]) all.push_back({b, &t});
+    std::stable_sort(all.begin(), all.end(), [&](const Item& x, const Item& y) {
+        return stage[x.b] != stage[y.b] ? stage[x.b] < stage[y.b] : x.t->osz > y.t->osz;
+    });
+    std::vector<int> left(std::max<size_t>(1, batches.size()), 0);  // undecoded segments per sub-batch
+    for (const auto& it : all) ++left[stage[it.b]];
+    std::vector<uint8_t> bad(nblocks, 0);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    const int T = std::min<int>((int)all.size(), (int)std::max(1u, std::thread::hardware_concurrency()));
+    for (int t = 0; t < T; ++t)
+        pool.emplace_back([&] {
+            for (int i; (i = next.fetch_add(1)) < (int)all.size();) {
+                const int r = run_seg_task(*all[i].t);
+                std::lock_guard<std::mutex> lk(mu);
+                if (r) bad[all[i].b] = 1;
+                if (--left[stage[all[i].b]] == 0) cv.notify_all();
+            }
+        });
+    int dev_rc = 0;
+    for (size_t j = 0; j < batches.size(); ++j) {
+        std::vector<int> list;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return left[j] == 0; });
+            for (int b : batches[j])
+                if (!bad[b]) list.push_back(b);
+        }
+        if (!list.empty() && !dev_rc) dev_rc = flush(guard->slot(), list);
+    }
+    for (auto& t : pool) t.join();
+    for (int b = 0; b < nblocks; ++b)
+        if (bad[b] && !rcs[b]) rcs[b] = QZ_DATA_CORRUPT;
+    if (dev_rc) return dev_rc;
     parallel_blocks(nblocks, [&](int b) {
         if (rcs[b]) return;
         uint8_t* out = output + ooff[b];

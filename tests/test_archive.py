@@ -190,8 +190,43 @@ def test_native_conversions_match_record_functions(kind):
 
 def test_native_fastq_edge_cases():
     assert A.fastq_to_streams(b"") == (0, 0, 0, b"", b"", b"")
+    # no records: nothing to assemble, with or without an output buffer
+    hdr = A.parse_archive_header(A.archive_header(0, 0, 0, 0))
+    assert A.streams_to_fastq(hdr, b"", b"", b"") == b""
+    assert A.streams_to_fastq(hdr, np.zeros(0, np.uint8), np.zeros(0, np.uint8), np.zeros(0, np.uint8)) == b""
     # no trailing newline
     assert A.fastq_to_streams(b"@a\nAC\n+\nII")[:3] == (1, 2, 2)
     for bad in (b"@a\nAC\n+\n", b"a\nAC\n+\nII\n", b"@a\nAC\n-\nII\n"):
         with pytest.raises(A.ArchiveError):
             A.fastq_to_streams(bad)
+
+
+@pytest.mark.parametrize("kind", ["quality_at_signs", "sequence_at_signs"])
+def test_native_conversions_parallel_stretches(kind):
+    """Texts over 4 MB are cut at record starts and converted in parallel.
+    Quality lines beginning with '@' (phred 31) must not be taken for record
+    starts. Sequence lines beginning with '@' above quality lines beginning
+    with '+' do look like record starts; the cut is then not confirmed and
+    the text is parsed in one piece. Both give the per-record streams."""
+    rng = np.random.default_rng(15)
+    recs = []
+    for i in range(40_000):
+        L = int(rng.integers(60, 200))
+        seq = bytes(rng.choice(list(b"ACGTN"), L).astype(np.uint8))
+        qual = bytes(rng.integers(35, 75, L, dtype=np.uint8))
+        if kind == "quality_at_signs" or i % 3:
+            qual = b"@" + qual[1:]
+        if kind == "sequence_at_signs":
+            seq, qual = b"@" + seq[1:], b"+" + qual[1:]
+        recs.append(A.FastqRecord(b"@read%d" % i, seq, qual))
+    text = A.format_fastq(recs)
+    assert len(text) > 8 << 20
+    cs, cq = A.detect_const_lengths(recs)
+    h, s, q = A.records_to_streams(recs, cs, cq)
+    assert A.fastq_to_streams(text) == (len(recs), cs, cq, h, s, q)
+    hdr = A.parse_archive_header(A.archive_header(len(recs), len(h), len(s), len(q), cs, cq) + h + s + q)
+    assert A.streams_to_fastq(hdr, h, s, q) == text
+    # a malformed record deep inside a stretch still fails the whole text
+    bad = text[:len(text) // 2] + b"\n" + text[len(text) // 2:]
+    with pytest.raises(A.ArchiveError):
+        A.fastq_to_streams(bad)

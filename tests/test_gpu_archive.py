@@ -4,11 +4,14 @@
   (default and LZP parameter sets, stored and tiny blocks included), and
   qzbsc_decompress_batch returns what per-block bsc_decompress returns;
 * FASTQ -> archive -> FASTQ is the identity for constant and variable read
-  lengths, with streams of many blocks (several device batches of 8);
+  lengths, with streams of many blocks, decoded in one batch call or in
+  calls of 8 blocks per stream;
 * every block of an archive stream decodes alone to its part of the stream;
 * a corrupted block, a truncated archive and the libbsc archive under
   tests/golden (whose QLFC payloads this library's coder does not read)
-  raise instead of returning data.
+  raise instead of returning data;
+* batches of 60 MB, coded and decoded over several device sub-batches, give
+  the per-block bytes.
 
 No torch here: the library must initialise the device itself.
 """
@@ -62,8 +65,9 @@ def test_archive_roundtrip(variable):
     assert (h.const_seq_len, h.const_qual_len) == ((0, 0) if variable else (150, 150))
     spans = A.stream_offsets(h)
     nblocks = [len(A.stream_blocks(arc, o, n)) for o, n in spans]
-    assert max(nblocks) > A.DECOMPRESS_BATCH_SIZE  # several device batches per stream
+    assert max(nblocks) > A.DECOMPRESS_BATCH_SIZE  # several batch calls per stream with batch=8
     assert A.decompress_archive(arc) == text
+    assert A.decompress_archive(arc, batch=A.DECOMPRESS_BATCH_SIZE) == text
     # each block alone gives its 256 KiB piece of the stream
     cs, cq = A.detect_const_lengths(A.parse_fastq(text))
     streams = A.records_to_streams(A.parse_fastq(text), cs, cq)
@@ -114,6 +118,25 @@ def test_batch_forms_many_tiny_blocks():
     blocks = [rng.integers(65, 65 + int(rng.integers(1, 5)), int(rng.integers(1, 300)), dtype=np.uint8)
               for _ in range(300)]
     print("tiny blocks:", len(blocks), flush=True)
+    comp = bsc.compress_batch(blocks)
+    assert comp == [bsc.compress_adaptive_no_lzp(b) for b in blocks]
+    assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
+
+
+def test_batch_forms_across_device_sub_batches():
+    """60 MB of blocks: the batch calls split them into several device
+    sub-batches (32 MB at least, so here one 20 MB block each, small blocks
+    joining the current one), overlapping host coding with device sorting.
+    The bytes still equal the per-block calls and decode back."""
+    rng = np.random.default_rng(14)
+    blocks = [
+        synth.config2_sequences(140_000, 5)[:20_000_000],
+        rng.integers(65, 69, 5_000, dtype=np.uint8),
+        synth.quality_stream(140_000, 150, 6)[:20_000_000],
+        np.frombuffer(b"ACGT", np.uint8),
+        synth.header_stream(600_000, 7)[:20_000_000],
+    ]
+    print("sub-batches:", [b.size for b in blocks], flush=True)
     comp = bsc.compress_batch(blocks)
     assert comp == [bsc.compress_adaptive_no_lzp(b) for b in blocks]
     assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
