@@ -226,7 +226,8 @@ QZBSC_API int qzbsc_quality_ctx_decompress(const unsigned char* data, long long 
 
 /* Pool control: at most n engines (streams + resident workspaces). */
 QZBSC_API int qzbsc_set_max_engines(int n);
-/* Releases the resident workspaces of every idle engine (no reference
+/* Releases the resident workspaces of every idle engine and the cached
+ * pinned host buffers of the front end's downloads (no reference
  * counterpart: for long-running processes between workloads). Returns the
  * bytes released. */
 QZBSC_API long long qzbsc_trim(void);

@@ -30,8 +30,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -187,40 +191,113 @@ int coder_compress(const uint8_t* bwt, int n, uint8_t* out, int cap, bool parall
         out, cap);
 }
 
-// Host copy of the front end's results for a whole batch (front.run() done on
-// the batch's BWT `dbwt` of n bytes): the segment list, per run symbol, rank
-// and length, per segment symbol table and symbol count, and the BWT itself
-// (stored segments copy it). It outlives the next front.run() on the slot.
+// Host copy of the front end's results for a whole batch (after front.run()):
+// the segment list, per run symbol, rank and length, per segment symbol table
+// and symbol count. The runs also give the BWT bytes of a segment that is
+// stored raw. It outlives the next front.run() on the slot.
+// The runs (6 bytes each: length, symbol, rank) live in one pinned buffer
+// from the cache, or in pageable memory if none can be pinned.
 struct FrontHost {
     std::vector<QlfcFront::Seg> segs;
-    std::vector<uint8_t> sym, rank, tables, bwt;
-    std::vector<uint32_t> len;
+    std::vector<uint8_t> tables;
     std::vector<uint16_t> nsym;
+    const uint32_t* len = nullptr;
+    const uint8_t* sym = nullptr;
+    const uint8_t* rank = nullptr;
+
+    FrontHost() = default;
+    FrontHost(const FrontHost&) = delete;
+    FrontHost& operator=(const FrontHost&) = delete;
+    FrontHost(FrontHost&& o) noexcept { *this = std::move(o); }
+    FrontHost& operator=(FrontHost&& o) noexcept
+    {
+        if (this != &o) {
+            free_runs();
+            segs = std::move(o.segs);
+            tables = std::move(o.tables);
+            nsym = std::move(o.nsym);
+            heap_ = std::move(o.heap_);
+            pin_ = o.pin_;
+            pin_cap_ = o.pin_cap_;
+            len = o.len;
+            sym = o.sym;
+            rank = o.rank;
+            o.pin_ = nullptr;
+            o.pin_cap_ = 0;
+            o.len = nullptr;
+            o.sym = o.rank = nullptr;
+        }
+        return *this;
+    }
+    ~FrontHost() { free_runs(); }
+
+    // Storage for R runs, laid out lengths | symbols | ranks.
+    uint8_t* alloc_runs(size_t R)
+    {
+        free_runs();
+        const size_t bytes = R * 6;
+        uint8_t* p = bytes ? pinned_acquire(bytes, &pin_cap_) : nullptr;
+        if (p) {
+            pin_ = p;
+        } else {
+            heap_.resize(bytes / 4 + 2);
+            p = reinterpret_cast<uint8_t*>(heap_.data());
+        }
+        len = reinterpret_cast<const uint32_t*>(p);
+        sym = p + 4 * R;
+        rank = p + 5 * R;
+        return p;
+    }
+
+private:
+    void free_runs()
+    {
+        if (pin_) pinned_release(pin_, pin_cap_);
+        pin_ = nullptr;
+        pin_cap_ = 0;
+        std::vector<uint32_t>().swap(heap_);
+        len = nullptr;
+        sym = rank = nullptr;
+    }
+    uint8_t* pin_ = nullptr;
+    size_t pin_cap_ = 0;
+    std::vector<uint32_t> heap_;
 };
 
-static int front_to_host(const QlfcFront& front, const uint8_t* dbwt, uint32_t n, FrontHost& fh, hipStream_t stream)
+static int front_to_host(const QlfcFront& front, FrontHost& fh, hipStream_t stream)
 {
     const uint32_t R = front.runs_;  // runs of every segment, laid out in segment order
     const size_t S = front.segs_.size();
     fh.segs = front.segs_;
-    fh.sym.resize(R);
-    fh.rank.resize(R);
-    fh.len.resize(R);
+    uint8_t* p = fh.alloc_runs(R);
     fh.tables.resize(S * 256);
     fh.nsym.resize(S);
-    fh.bwt.resize(n);
     if (R) {
-        QZ_HIP(hipMemcpyAsync(fh.sym.data(), front.rsym_, R, hipMemcpyDeviceToHost, stream));
-        QZ_HIP(hipMemcpyAsync(fh.rank.data(), front.rrank_, R, hipMemcpyDeviceToHost, stream));
-        QZ_HIP(hipMemcpyAsync(fh.len.data(), front.rlen_, (size_t)R * 4, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(p, front.rlen_, (size_t)R * 4, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(p + (size_t)R * 4, front.rsym_, R, hipMemcpyDeviceToHost, stream));
+        QZ_HIP(hipMemcpyAsync(p + (size_t)R * 5, front.rrank_, R, hipMemcpyDeviceToHost, stream));
     }
     if (S) {
         QZ_HIP(hipMemcpyAsync(fh.tables.data(), front.table_, S * 256, hipMemcpyDeviceToHost, stream));
         QZ_HIP(hipMemcpyAsync(fh.nsym.data(), front.nsym_, S * 2, hipMemcpyDeviceToHost, stream));
     }
-    QZ_HIP(hipMemcpyAsync(fh.bwt.data(), dbwt, n, hipMemcpyDeviceToHost, stream));
     QZ_HIP(hipStreamSynchronize(stream));
     return 0;
+}
+
+// The BWT bytes of segment `seg`, rebuilt from its runs (a segment the
+// coder cannot shrink is stored raw).
+static int raw_from_runs(const FrontHost& fh, int seg, uint8_t* dst)
+{
+    const auto& sg = fh.segs[seg];
+    uint64_t w = 0;
+    for (uint32_t r = sg.r0; r < sg.r1; ++r) {
+        const uint32_t L = fh.len[r];
+        if (w + L > sg.size) return QZ_GPU_ERROR;
+        memset(dst + w, fh.sym[r], L);
+        w += L;
+    }
+    return w == sg.size ? 0 : QZ_GPU_ERROR;
 }
 
 // One coded segment of a batch.
@@ -231,21 +308,18 @@ struct SegCode {
 
 // Codes every segment of the batch from the host copy of the front end, all
 // segments of all blocks in parallel.
-static void code_segments_host(const FrontHost& fh, std::vector<SegCode>& sc)
+static void code_segment(const FrontHost& fh, int i, SegCode& sc)
 {
-    sc.assign(fh.segs.size(), SegCode{});
-    parallel_blocks((int)fh.segs.size(), [&](int i) {
-        const auto& sg = fh.segs[i];
-        qlfc::SegRuns Rn;
-        Rn.sym = fh.sym.data() + sg.r0;
-        Rn.rank = fh.rank.data() + sg.r0;
-        Rn.len = fh.len.data() + sg.r0;
-        Rn.count = sg.r1 - sg.r0;
-        Rn.nsym = fh.nsym[i];
-        memcpy(Rn.table, fh.tables.data() + (size_t)i * 256, 256);
-        sc[i].out.resize((size_t)sg.size + 64);
-        sc[i].res = qlfc::encode_segment(Rn, (int)sg.size, sc[i].out.data(), (int)sg.size);
-    });
+    const auto& sg = fh.segs[i];
+    qlfc::SegRuns Rn;
+    Rn.sym = fh.sym + sg.r0;
+    Rn.rank = fh.rank + sg.r0;
+    Rn.len = fh.len + sg.r0;
+    Rn.count = sg.r1 - sg.r0;
+    Rn.nsym = fh.nsym[i];
+    memcpy(Rn.table, fh.tables.data() + (size_t)i * 256, 256);
+    sc.out.resize((size_t)sg.size + 64);
+    sc.res = qlfc::encode_segment(Rn, (int)sg.size, sc.out.data(), (int)sg.size);
 }
 
 // Coder payload of batch block `block` from its coded segments (each block's
@@ -270,8 +344,7 @@ static int payload_from_segments(const FrontHost& fh, std::vector<SegCode>& sc,
     return assemble_payload(
         n, nb, size, segout, res,
         [&](int sg, uint8_t* dst) {
-            memcpy(dst, fh.bwt.data() + start[sg], (size_t)size[sg]);
-            return 0;
+            return raw_from_runs(fh, sidx[sg], dst);
         },
         out, cap);
 }
@@ -295,9 +368,9 @@ static int coder_compress_host(const FrontHost& fh, uint32_t block, int n, uint8
     for_segments(nb, parallel, [&](int k) {
         const auto& sg = fh.segs[sidx[k]];
         qlfc::SegRuns Rn;
-        Rn.sym = fh.sym.data() + sg.r0;
-        Rn.rank = fh.rank.data() + sg.r0;
-        Rn.len = fh.len.data() + sg.r0;
+        Rn.sym = fh.sym + sg.r0;
+        Rn.rank = fh.rank + sg.r0;
+        Rn.len = fh.len + sg.r0;
         Rn.count = sg.r1 - sg.r0;
         Rn.nsym = fh.nsym[sidx[k]];
         memcpy(Rn.table, fh.tables.data() + (size_t)sidx[k] * 256, 256);
@@ -307,8 +380,7 @@ static int coder_compress_host(const FrontHost& fh, uint32_t block, int n, uint8
     return assemble_payload(
         n, nb, size, segout, res,
         [&](int sg, uint8_t* dst) {
-            memcpy(dst, fh.bwt.data() + start[sg], (size_t)size[sg]);
-            return 0;
+            return raw_from_runs(fh, sidx[sg], dst);
         },
         out, cap);
 }
@@ -498,9 +570,16 @@ int decode_payload(const uint8_t* input, int inputSize, const BlockHeader& bh, u
 
 // Device batches of the batch entry points hold at most this many bytes.
 constexpr uint64_t kBatchBytes = 1ull << 30;
-// Decode pipeline: device sub-batches of about 1/kPipeParts of a batch's
-// BWT bytes, none smaller than kPipeBytes.
+// Batch pipelines: device sub-batches of about 1/pipe_parts() of a batch's
+// bytes, none smaller than kPipeBytes. QZBSC_PIPE_PARTS (1-64) overrides the
+// default count.
 constexpr uint64_t kPipeParts = 4, kPipeBytes = 32ull << 20;
+static uint64_t pipe_parts()
+{
+    const char* e = getenv("QZBSC_PIPE_PARTS");
+    const long v = e ? strtol(e, nullptr, 10) : 0;
+    return (v >= 1 && v <= 64) ? (uint64_t)v : kPipeParts;
+}
 
 // LZP blocks: the size m of the LZP data the BWT ran on (0 if the block
 // carries no LZP parameters), checked against the header.
@@ -547,18 +626,23 @@ QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, in
             mode &= 0xff;
         }
     }
-    SlotGuard g;
-    if (!g.ok()) return QZ_GPU_ERROR;
-    Slot& sl = g.slot();
     std::vector<uint32_t> offs = {0u, (uint32_t)dn};
     int primary = 0;
     std::vector<int> aux(256);
-    QZ_TRY(sl.fwd.reserve((uint64_t)dn, 1));
-    QZ_HIP(hipMemcpyAsync(sl.fwd.text_, data, (size_t)dn, hipMemcpyHostToDevice, sl.stream));
-    QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, &primary, aux.data(), nullptr));
-    QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
     FrontHost fh;
-    QZ_TRY(front_to_host(sl.front, sl.fwd.out_, (uint32_t)dn, fh, sl.stream));
+    {
+        // the device slot is held for the device work only: the host coder
+        // below runs with it released, so concurrent callers keep the device
+        // busy while this one codes
+        SlotGuard g;
+        if (!g.ok()) return QZ_GPU_ERROR;
+        Slot& sl = g.slot();
+        QZ_TRY(sl.fwd.reserve((uint64_t)dn, 1));
+        QZ_HIP(hipMemcpyAsync(sl.fwd.text_, data, (size_t)dn, hipMemcpyHostToDevice, sl.stream));
+        QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, &primary, aux.data(), nullptr));
+        QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
+        QZ_TRY(front_to_host(sl.front, fh, sl.stream));
+    }
     const bool par = (features & LIBBSC_FEATURE_MULTITHREADING) != 0;
     return finish_block_with(input, n, dn, mode, primary, aux.data(), output, [&](uint8_t* p, int cap) {
         return coder_compress_host(fh, 0, dn, p, cap, par);
@@ -653,15 +737,28 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
         else device = true;
     }
     if (!device) return LIBBSC_NO_ERROR;  // only stored blocks: no device needed
-    SlotGuard g;
-    if (!g.ok()) return QZ_GPU_ERROR;
-    Slot& sl = g.slot();
+    std::unique_ptr<SlotGuard> guard(new SlotGuard);
+    if (!guard->ok()) return QZ_GPU_ERROR;
+    Slot& sl = guard->slot();
     // Device sub-batches of about 1/kPipeParts of the bytes (at least
-    // kPipeBytes, at most kBatchBytes): while the host pool codes the
-    // segments of sub-batch j, this thread sorts j + 1 on the device.
+    // kPipeBytes, at most kBatchBytes). After each one this thread queues its
+    // coder segments, longest first, for a pool of coder threads that runs
+    // for the whole call; the thread that codes a block's last segment
+    // assembles that block, and the one that assembles a sub-batch's last
+    // block frees its host copy of the front end.
     struct Job {
         std::vector<int> list, prim, aux;
         FrontHost fh;
+        std::vector<SegCode> sc;
+        std::vector<int> left;  // uncoded segments per block of the sub-batch
+        int blocks_left = 0;
+    };
+    // QZBSC_TRACE_BATCH=1 prints, per sub-batch, the time of its upload,
+    // device work and front-end download, and when the coder pool finished.
+    const bool trace = getenv("QZBSC_TRACE_BATCH") != nullptr;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     auto run_device = [&](Job& job) -> int {
         const std::vector<int>& list = job.list;
@@ -669,37 +766,35 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
         std::vector<uint32_t> offs(k + 1, 0);
         for (int i = 0; i < k; ++i) offs[i + 1] = offs[i] + (uint32_t)dn[list[i]];
         QZ_TRY(sl.fwd.reserve(offs[k], (uint32_t)k));
+        const auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < k; ++i) {
             const int b = list[i];
             const uint8_t* data = lz[b].empty() ? input + ioff[b] : lz[b].data();
             QZ_HIP(hipMemcpyAsync(sl.fwd.text_ + offs[i], data, (size_t)dn[b], hipMemcpyHostToDevice, sl.stream));
         }
+        if (trace) QZ_HIP(hipStreamSynchronize(sl.stream));
+        const double up = ms_since(t0);
         job.prim.assign(k, 0);
         job.aux.assign((size_t)k * 256, 0);
         QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, job.prim.data(), job.aux.data(), nullptr));
+        if (trace) QZ_HIP(hipStreamSynchronize(sl.stream));
+        const double sort = ms_since(t0) - up;
         QZ_TRY(sl.front.run(sl.fwd.out_, offs, sl.stream));
-        return front_to_host(sl.front, sl.fwd.out_, offs[k], job.fh, sl.stream);
-    };
-    auto code = [&](Job& job) {
-        std::vector<SegCode> sc;
-        code_segments_host(job.fh, sc);  // every segment of every block at once
-        parallel_blocks((int)job.list.size(), [&](int i) {
-            const int b = job.list[i];
-            out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], job.prim[i],
-                                             job.aux.data() + (size_t)i * 256, output + ooff[b],
-                                             [&](uint8_t* p, int cap) {
-                                                 return payload_from_segments(job.fh, sc, (uint32_t)i, dn[b], p,
-                                                                              cap);
-                                             });
-        });
-        job.fh = FrontHost{};
+        if (trace) QZ_HIP(hipStreamSynchronize(sl.stream));
+        const double front = ms_since(t0) - up - sort;
+        const int rc = front_to_host(sl.front, job.fh, sl.stream);
+        if (trace)
+            fprintf(stderr, "[qzbsc batch] sub-batch of %d blocks, %u B: upload %.1f ms, sort %.1f ms, front end %.1f ms, "
+                            "runs %u down %.1f ms (at %.1f ms)\n",
+                    k, offs[k], up, sort, front, sl.front.runs_, ms_since(t0) - up - sort - front, ms_since(t_start));
+        return rc;
     };
     std::vector<Job> jobs;
     {
         uint64_t total = 0;
         for (int b = 0; b < nblocks; ++b)
             if (sizes[b] > kHeader) total += (uint64_t)dn[b];
-        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / kPipeParts));
+        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / pipe_parts()));
         uint64_t bytes = 0;
         for (int b = 0; b < nblocks; ++b) {
             if (sizes[b] <= kHeader) continue;
@@ -711,15 +806,87 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
             bytes += (uint64_t)dn[b];
         }
     }
+    auto finish = [&](Job& job, int i) {
+        const int b = job.list[i];
+        out_sizes[b] = finish_block_with(input + ioff[b], sizes[b], dn[b], mode[b], job.prim[i],
+                                         job.aux.data() + (size_t)i * 256, output + ooff[b],
+                                         [&](uint8_t* p, int cap) {
+                                             return payload_from_segments(job.fh, job.sc, (uint32_t)i, dn[b], p, cap);
+                                         });
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::pair<int, int>> queue;  // (sub-batch, segment)
+    bool closed = false;
+    auto worker = [&] {
+        for (;;) {
+            std::pair<int, int> t;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !queue.empty() || closed; });
+                if (queue.empty()) return;
+                t = queue.front();
+                queue.pop_front();
+            }
+            Job& job = jobs[t.first];
+            code_segment(job.fh, t.second, job.sc[t.second]);
+            const int i = (int)job.fh.segs[t.second].block;
+            bool block_done, job_done = false;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                block_done = --job.left[i] == 0;
+            }
+            if (!block_done) continue;
+            finish(job, i);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                job_done = --job.blocks_left == 0;
+            }
+            if (job_done) {
+                job.fh = FrontHost{};
+                job.sc = std::vector<SegCode>{};
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    const int T = (int)std::max(1u, std::thread::hardware_concurrency());
+    for (int t = 0; t < T; ++t) pool.emplace_back(worker);
     int dev_rc = 0;
-    std::thread coding;
-    for (auto& job : jobs) {
+    for (size_t j = 0; j < jobs.size() && !dev_rc; ++j) {
+        Job& job = jobs[j];
         dev_rc = run_device(job);
-        if (coding.joinable()) coding.join();
         if (dev_rc) break;
-        coding = std::thread(code, std::ref(job));
+        const int S = (int)job.fh.segs.size(), k = (int)job.list.size();
+        job.sc.assign(S, SegCode{});
+        job.left.assign(k, 0);
+        for (int i = 0; i < S; ++i) ++job.left[job.fh.segs[i].block];
+        job.blocks_left = k;
+        std::vector<int> order(S);
+        for (int i = 0; i < S; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return job.fh.segs[x].size > job.fh.segs[y].size; });
+        // blocks without segments (the front end gives every block one) are
+        // finished here, before any worker can see this sub-batch
+        for (int i = 0; i < k; ++i)
+            if (!job.left[i]) {
+                finish(job, i);
+                --job.blocks_left;
+            }
+        if (!S) continue;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int i : order) queue.push_back({(int)j, i});
+        }
+        cv.notify_all();
     }
-    if (coding.joinable()) coding.join();
+    guard.reset();  // the device work is done: other callers may take the slot
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        closed = true;
+    }
+    cv.notify_all();
+    for (auto& t : pool) t.join();
+    if (trace) fprintf(stderr, "[qzbsc batch] coder pool done at %.1f ms\n", ms_since(t_start));
     if (dev_rc) return dev_rc;
     for (int b = 0; b < nblocks; ++b)
         if (out_sizes[b] < 0) return out_sizes[b];
@@ -797,7 +964,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
             }
             total += (uint64_t)need[b];
         }
-        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / kPipeParts));
+        const uint64_t target = std::min<uint64_t>(kBatchBytes, std::max<uint64_t>(kPipeBytes, total / pipe_parts()));
         uint64_t bytes = 0;
         for (int b = 0; b < nblocks; ++b) {
             if (rcs[b] || !need[b]) continue;

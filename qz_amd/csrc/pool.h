@@ -22,8 +22,17 @@ struct Slot {
 Slot* pool_acquire();
 void pool_release(Slot* s);
 int pool_set_limit(int n);
-// Releases the workspaces of every idle slot; returns their footprint in bytes.
+// Releases the workspaces of every idle slot and the cached pinned host
+// buffers; returns their footprint in bytes.
 uint64_t pool_trim();
+
+// Pinned host buffers for device-to-host downloads, cached across calls
+// (pinning is slow, and DMA into pinned memory is several times faster than
+// into pageable memory). pinned_acquire returns at least `bytes` bytes and
+// their capacity in *cap, or nullptr if no memory can be pinned;
+// pinned_release returns a buffer to the cache (up to 4 GiB cached).
+uint8_t* pinned_acquire(size_t bytes, size_t* cap);
+void pinned_release(uint8_t* p, size_t cap);
 int pool_device_ok();
 
 class SlotGuard {

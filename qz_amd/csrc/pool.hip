@@ -58,10 +58,76 @@ void pool_release(Slot* s)
     g_cv.notify_one();
 }
 
+namespace {
+std::mutex g_pin_mu;
+std::vector<std::pair<uint8_t*, size_t>> g_pin_free;  // cached pinned buffers
+size_t g_pin_cached = 0;
+constexpr size_t kPinnedCache = 4ull << 30;
+constexpr size_t kPinnedLarge = 64ull << 20;
+
+// Capacities are powers of two from 1 MiB up to 64 MiB, then multiples of
+// 64 MiB, so that buffers are reused across calls of similar sizes.
+size_t pinned_round(size_t bytes)
+{
+    if (bytes > kPinnedLarge) return (bytes + kPinnedLarge - 1) / kPinnedLarge * kPinnedLarge;
+    size_t c = 1ull << 20;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+}  // namespace
+
+uint8_t* pinned_acquire(size_t bytes, size_t* cap)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        int best = -1;
+        for (int i = 0; i < (int)g_pin_free.size(); ++i)
+            if (g_pin_free[i].second >= bytes && (best < 0 || g_pin_free[i].second < g_pin_free[best].second)) best = i;
+        if (best >= 0) {
+            auto b = g_pin_free[best];
+            g_pin_free.erase(g_pin_free.begin() + best);
+            g_pin_cached -= b.second;
+            *cap = b.second;
+            return b.first;
+        }
+    }
+    const size_t c = pinned_round(bytes);
+    void* p = nullptr;
+    if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    *cap = c;
+    return static_cast<uint8_t*>(p);
+}
+
+void pinned_release(uint8_t* p, size_t cap)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (g_pin_cached + cap <= kPinnedCache) {
+            g_pin_free.push_back({p, cap});
+            g_pin_cached += cap;
+            return;
+        }
+    }
+    (void)hipHostFree(p);
+}
+
 uint64_t pool_trim()
 {
-    std::lock_guard<std::mutex> lk(g_mu);
     uint64_t freed = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (auto& b : g_pin_free) {
+            (void)hipHostFree(b.first);
+            freed += b.second;
+        }
+        g_pin_free.clear();
+        g_pin_cached = 0;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
     for (auto& s : g_slots) {
         if (s->busy) continue;
         freed += s->fwd.cap_ ? BwtEngine::footprint(s->fwd.cap_, s->fwd.cap_blk_) : 0;

@@ -20,7 +20,8 @@
 //   2. runs are cut into chunks of kChunk runs inside each segment; each
 //      chunk's signature = its symbols in first-occurrence order;
 //   3. per segment, right to left, the list a chunk starts from is
-//      sig(k+1) followed by the rest of chunk k+1's starting list;
+//      sig(k+1) followed by the rest of chunk k+1's starting list (a
+//      two-level scan: stretches of chunks per lane, then across lanes);
 //   4. each chunk replays its runs backwards from that list (in LDS).
 #include "prims.h"
 #include "qlfc_gpu.h"
@@ -32,7 +33,7 @@
 namespace qz {
 
 constexpr int kFThreads = 256;
-constexpr uint32_t kChunk = 2048;   // runs per chunk
+constexpr uint32_t kChunk = 512;    // runs per chunk
 constexpr int kRankThreads = 64;    // one chunk per thread, 256-byte list each in LDS
 
 static int num_segments_host(int n) { return qlfc::num_segments(n); }
@@ -166,53 +167,95 @@ __global__ __launch_bounds__(kFThreads) void k_chunk_sig(const uint8_t* __restri
     siglen[c] = (uint16_t)L;
 }
 
-// One thread per segment, right to left over its chunks: the list each
-// chunk starts its backward replay from, and the segment table.
-__global__ __launch_bounds__(64) void k_chunk_state(const Chunk* __restrict__ chunks,
-                                                    const uint32_t* __restrict__ seg_chunk0, uint32_t nseg,
+// out = sg ++ (L \ sg): the list after a chunk whose signature is sg (its
+// symbols in first-occurrence order) is replayed backwards from list L.
+// Returns the length of out (out must not alias either input).
+__device__ uint32_t prepend_sig(const uint8_t* sg, uint32_t ls, const uint8_t* L, uint32_t ll, uint8_t* out)
+{
+    uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t q = 0; q < ls; ++q) {
+        const uint32_t x = sg[q];
+        out[q] = (uint8_t)x;
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+            if ((uint32_t)w == (x >> 5)) in[w] |= 1u << (x & 31);
+    }
+    uint32_t lo = ls;
+    for (uint32_t q = 0; q < ll; ++q) {
+        const uint32_t x = L[q];
+        uint32_t iw = 0;
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+            if ((uint32_t)w == (x >> 5)) iw = in[w];
+        if (!(iw & (1u << (x & 31)))) out[lo++] = (uint8_t)x;
+    }
+    return lo;
+}
+
+// One 64-lane workgroup per segment: the list each chunk starts its backward
+// replay from (state), and the segment's symbol table. Chunk c maps a list L
+// to sig(c) ++ (L \ sig(c)); a run of such maps is again of that form, its
+// list being the maps applied to the empty list. So each lane composes one
+// stretch of consecutive chunks (A), lane 0 chains the stretches right to
+// left into the list entering each stretch (B), and each lane then walks its
+// stretch again, writing every chunk's starting list (C).
+__global__ __launch_bounds__(64) void k_chunk_state(const uint32_t* __restrict__ seg_chunk0,
                                                     const uint8_t* __restrict__ sig,
                                                     const uint16_t* __restrict__ siglen, uint8_t* __restrict__ state,
                                                     uint16_t* __restrict__ statelen, uint8_t* __restrict__ table,
                                                     uint16_t* __restrict__ nsym)
 {
-    __shared__ uint8_t cur[64][256];
-    __shared__ uint8_t nxt[64][256];
-    const uint32_t sgi = blockIdx.x * 64 + threadIdx.x;
-    if (sgi >= nseg) return;
-    uint8_t* a = cur[threadIdx.x];
-    uint8_t* b = nxt[threadIdx.x];
-    uint32_t la = 0;
+    __shared__ uint8_t comp[64][256];  // (A) stretch composite; (C) work list
+    __shared__ uint8_t work[64][256];  // (A), (C) work list
+    __shared__ uint8_t inl[64][256];   // (B) list entering each stretch from the right
+    __shared__ uint16_t clen[64], ilen[64];
+    const uint32_t sgi = blockIdx.x;
+    const uint32_t l = threadIdx.x;
     const uint32_t c0 = seg_chunk0[sgi], c1 = seg_chunk0[sgi + 1];
-    for (uint32_t c = c1; c-- > c0;) {
-        // starting list of chunk c = current list (after chunks c+1..)
-        uint8_t* st = state + (size_t)c * 256;
-        for (uint32_t q = 0; q < la; ++q) st[q] = a[q];
-        statelen[c] = (uint16_t)la;
-        // list after chunk c = sig(c) ++ (a \ sig(c))
-        const uint8_t* sg = sig + (size_t)c * 256;
-        const uint32_t ls = siglen[c];
-        uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t q = 0; q < ls; ++q) {
-            const uint32_t s = sg[q];
-            b[q] = (uint8_t)s;
-            for (int w = 0; w < 8; ++w)
-                if ((uint32_t)w == (s >> 5)) in[w] |= 1u << (s & 31);
-        }
-        uint32_t lb = ls;
-        for (uint32_t q = 0; q < la; ++q) {
-            const uint32_t s = a[q];
-            uint32_t iw = 0;
-            for (int w = 0; w < 8; ++w)
-                if ((uint32_t)w == (s >> 5)) iw = in[w];
-            if (!(iw & (1u << (s & 31)))) b[lb++] = (uint8_t)s;
-        }
-        uint8_t* t = a;
-        a = b;
-        b = t;
-        la = lb;
+    const uint32_t n = c1 - c0, per = (n + 63) / 64;
+    const uint32_t a = c0 + min(n, l * per), b = c0 + min(n, (l + 1) * per);
+    // (A)
+    uint8_t* x = comp[l];
+    uint8_t* y = work[l];
+    uint32_t lx = 0;
+    for (uint32_t c = b; c-- > a;) {
+        const uint32_t ly = prepend_sig(sig + (size_t)c * 256, siglen[c], x, lx, y);
+        uint8_t* t = x;
+        x = y;
+        y = t;
+        lx = ly;
     }
-    for (uint32_t q = 0; q < la; ++q) table[(size_t)sgi * 256 + q] = a[q];
-    nsym[sgi] = (uint16_t)la;
+    if (x != comp[l])
+        for (uint32_t q = 0; q < lx; ++q) comp[l][q] = x[q];
+    clen[l] = (uint16_t)lx;
+    __syncthreads();
+    // (B)
+    if (l == 0) {
+        ilen[63] = 0;
+        for (int k = 62; k >= 0; --k)
+            ilen[k] = (uint16_t)prepend_sig(comp[k + 1], clen[k + 1], inl[k + 1], ilen[k + 1], inl[k]);
+        const uint32_t lt = prepend_sig(comp[0], clen[0], inl[0], ilen[0], work[0]);
+        for (uint32_t q = 0; q < lt; ++q) table[(size_t)sgi * 256 + q] = work[0][q];
+        nsym[sgi] = (uint16_t)lt;
+    }
+    __syncthreads();
+    // (C)
+    x = inl[l];
+    uint8_t* y2 = comp[l];
+    uint8_t* z = work[l];
+    lx = ilen[l];
+    for (uint32_t c = b; c-- > a;) {
+        uint8_t* st = state + (size_t)c * 256;
+        for (uint32_t q = 0; q < lx; ++q) st[q] = x[q];
+        statelen[c] = (uint16_t)lx;
+        if (c == a) break;
+        const uint32_t ly = prepend_sig(sig + (size_t)c * 256, siglen[c], x, lx, y2);
+        x = y2;
+        lx = ly;
+        uint8_t* t = y2;
+        y2 = z;
+        z = t;
+    }
 }
 
 // One thread per chunk: replays its runs backwards from its starting list.
@@ -391,8 +434,8 @@ int QlfcFront::run(const uint8_t* U, const std::vector<uint32_t>& offs, hipStrea
     hipLaunchKernelGGL(k_chunk_sig, dim3(qz_div_up(nch, kFThreads)), dim3(kFThreads), 0, s, rsym_,
                        (const Chunk*)chunks_, nch, sig_, siglen_);
     QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_chunk_state, dim3(qz_div_up(nseg, 64)), dim3(64), 0, s, (const Chunk*)chunks_, segchunk_,
-                       nseg, sig_, siglen_, state_, statelen_, table_, nsym_);
+    hipLaunchKernelGGL(k_chunk_state, dim3(nseg), dim3(64), 0, s, segchunk_, sig_, siglen_, state_, statelen_,
+                       table_, nsym_);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_chunk_rank, dim3(qz_div_up(nch, kRankThreads)), dim3(kRankThreads), 0, s, rsym_,
                        (const Chunk*)chunks_, nch, state_, statelen_, rrank_);

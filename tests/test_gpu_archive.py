@@ -11,7 +11,9 @@
   tests/golden (whose QLFC payloads this library's coder does not read)
   raise instead of returning data;
 * batches of 60 MB, coded and decoded over several device sub-batches, give
-  the per-block bytes.
+  the per-block bytes;
+* segments stored raw (bytes rebuilt from the device runs) in a block of
+  reads and random bytes.
 
 No torch here: the library must initialise the device itself.
 """
@@ -140,3 +142,20 @@ def test_batch_forms_across_device_sub_batches():
     comp = bsc.compress_batch(blocks)
     assert comp == [bsc.compress_adaptive_no_lzp(b) for b in blocks]
     assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
+
+
+def test_raw_segments_rebuilt_from_runs():
+    """A block of reads followed by random bytes: some coder segments do not
+    shrink and are stored raw, their bytes rebuilt from the device front
+    end's runs. The block must hold at least one raw segment, equal the
+    per-block call and decode back, alone and in a batch."""
+    rng = np.random.default_rng(16)
+    blk = np.concatenate([synth.sequence_stream(200_000, 8_000, 150, 9, 0.002)[:1_200_000],
+                          rng.integers(0, 256, 1_000_000, dtype=np.uint8)])
+    comp = bsc.compress_adaptive_no_lzp(blk)
+    nb = comp[28]
+    segs = [struct.unpack_from("<ii", comp, 29 + 8 * s) for s in range(nb)]
+    print("segments (size, coded):", segs, flush=True)
+    assert nb > 1 and any(i == o for i, o in segs) and any(o < i for i, o in segs)
+    assert bsc.compress_batch([blk, blk[:300_000]]) == [comp, bsc.compress_adaptive_no_lzp(blk[:300_000])]
+    assert bsc.decompress(comp) == blk.tobytes()
