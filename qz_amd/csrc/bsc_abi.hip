@@ -977,6 +977,13 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
             stage[b] = (int)batches.size() - 1;
         }
     }
+    // QZBSC_TRACE_BATCH=1 prints, per sub-batch, when its segments were
+    // decoded and the time of its upload, inverse and download.
+    const bool trace = getenv("QZBSC_TRACE_BATCH") != nullptr;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     auto flush = [&](Slot& sl, const std::vector<int>& list) -> int {
         const int k = (int)list.size();
         std::vector<uint32_t> offs(k + 1, 0);
@@ -986,14 +993,24 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
             prim[i] = bh[list[i]].index;
         }
         QZ_TRY(sl.inv.reserve(offs[k], (uint32_t)k));
+        const double t_dec = ms_since(t_start);
+        const auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < k; ++i)
             QZ_HIP(hipMemcpyAsync(sl.inv.U_ + offs[i], bw[list[i]], (size_t)need[list[i]], hipMemcpyHostToDevice,
                                   sl.stream));
+        if (trace) QZ_HIP(hipStreamSynchronize(sl.stream));
+        const double up = ms_since(t0);
         QZ_TRY(inv_batch_device(sl, sl.inv.U_, sl.inv.T_, offs, prim.data(), nullptr));
+        if (trace) QZ_HIP(hipStreamSynchronize(sl.stream));
+        const double inv = ms_since(t0) - up;
         for (int i = 0; i < k; ++i)
             QZ_HIP(hipMemcpyAsync(bw[list[i]], sl.inv.T_ + offs[i], (size_t)need[list[i]], hipMemcpyDeviceToHost,
                                   sl.stream));
         QZ_HIP(hipStreamSynchronize(sl.stream));
+        if (trace)
+            fprintf(stderr, "[qzbsc batch] decoded %d blocks, %u B by %.1f ms: upload %.1f ms, inverse %.1f ms, "
+                            "download %.1f ms\n",
+                    k, offs[k], t_dec, up, inv, ms_since(t0) - up - inv);
         return 0;
     };
     std::unique_ptr<SlotGuard> guard;
@@ -1055,6 +1072,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
         }
         if (adler32(out, (size_t)bh[b].data_size) != bh[b].adler_data) rcs[b] = QZ_DATA_CORRUPT;
     });
+    if (trace) fprintf(stderr, "[qzbsc batch] checks done at %.1f ms\n", ms_since(t_start));
     for (int b = 0; b < nblocks; ++b)
         if (rcs[b]) return rcs[b];
     return LIBBSC_NO_ERROR;
