@@ -129,7 +129,8 @@ def test_batch_forms_across_device_sub_batches():
     """60 MB of blocks: the batch calls split them into several device
     sub-batches (32 MB at least, so here one 20 MB block each, small blocks
     joining the current one), overlapping host coding with device sorting.
-    The bytes still equal the per-block calls and decode back."""
+    The bytes still equal the per-block calls and decode back, without and
+    with LZP."""
     rng = np.random.default_rng(14)
     blocks = [
         synth.config2_sequences(140_000, 5)[:20_000_000],
@@ -142,6 +143,10 @@ def test_batch_forms_across_device_sub_batches():
     comp = bsc.compress_batch(blocks)
     assert comp == [bsc.compress_adaptive_no_lzp(b) for b in blocks]
     assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
+    print("sub-batches with LZP", flush=True)
+    lz = bsc.compress_batch(blocks, 16, 128)
+    assert lz == [bsc.compress_adaptive(b) for b in blocks]
+    assert bsc.decompress_batch(lz) == [b.tobytes() for b in blocks]
 
 
 def test_raw_segments_rebuilt_from_runs():
@@ -159,3 +164,19 @@ def test_raw_segments_rebuilt_from_runs():
     assert nb > 1 and any(i == o for i, o in segs) and any(o < i for i, o in segs)
     assert bsc.compress_batch([blk, blk[:300_000]]) == [comp, bsc.compress_adaptive_no_lzp(blk[:300_000])]
     assert bsc.decompress(comp) == blk.tobytes()
+
+
+def test_trim_releases_workspaces_and_pinned_buffers():
+    """qzbsc_trim frees the idle engines' workspaces and the cached pinned
+    download buffers; later calls allocate again and give the same bytes."""
+    from qz_amd._native import lib
+    blocks = [synth.sequence_stream(300_000, 20_000, 150, 17, 0.002), synth.quality_stream(20_000, 150, 18)]
+    print("trim:", [b.size for b in blocks], flush=True)
+    comp = bsc.compress_batch(blocks)
+    one = bsc.compress_adaptive_no_lzp(blocks[0])
+    freed = lib().qzbsc_trim()
+    assert freed > 0
+    assert lib().qzbsc_trim() == 0  # nothing left to free
+    assert bsc.compress_batch(blocks) == comp
+    assert bsc.compress_adaptive_no_lzp(blocks[0]) == one
+    assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
