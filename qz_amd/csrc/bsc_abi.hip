@@ -740,7 +740,7 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
     std::unique_ptr<SlotGuard> guard(new SlotGuard);
     if (!guard->ok()) return QZ_GPU_ERROR;
     Slot& sl = guard->slot();
-    // Device sub-batches of about 1/kPipeParts of the bytes (at least
+    // Device sub-batches of about 1/pipe_parts() of the bytes (at least
     // kPipeBytes, at most kBatchBytes). After each one this thread queues its
     // coder segments, longest first, for a pool of coder threads that runs
     // for the whole call; the thread that codes a block's last segment
@@ -849,8 +849,11 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
         }
     };
     std::vector<std::thread> pool;
-    const int T = (int)std::max(1u, std::thread::hardware_concurrency());
-    for (int t = 0; t < T; ++t) pool.emplace_back(worker);
+    int nsegs = 0;  // coder segments of the whole batch (the front end's rule)
+    for (int b = 0; b < nblocks; ++b)
+        if (sizes[b] > kHeader) nsegs += qlfc::num_segments(dn[b]);
+    const int T = std::min<int>(nsegs, (int)std::max(1u, std::thread::hardware_concurrency()));
+    for (int t = 0; t < std::max(T, 1); ++t) pool.emplace_back(worker);
     int dev_rc = 0;
     for (size_t j = 0; j < jobs.size() && !dev_rc; ++j) {
         Job& job = jobs[j];
