@@ -61,6 +61,9 @@ BITS_PER_QUAL = 7             # QualityBinning::None (columnar.rs:34)
 ENCODING_PLAIN = 0
 BINNING_LOSSLESS = 0
 COMPRESSOR_BSC = 1
+COMPRESSOR_QUALITY_CTX = 4       # QualityCompressor::QualityCtx (mod.rs:109)
+MIN_READS_QUALITY_CTX = 100_000  # mod.rs:34
+QUALITY_CTX_BLOCK_SIZE = 500_000  # records per quality_ctx blob (cli.rs:132)
 
 
 class ArchiveError(ValueError):
@@ -237,15 +240,17 @@ def records_to_streams(records, const_seq_len: int, const_qual_len: int):
 
 
 def archive_header(num_reads: int, headers_len: int, sequences_len: int, qualities_len: int,
-                   const_seq_len: int = 0, const_qual_len: int = 0) -> bytes:
-    """mod.rs:430-482 for encoding 0, lossless qualities and BSC for all three
-    streams: prefix, body, and the constant lengths when either is set."""
+                   const_seq_len: int = 0, const_qual_len: int = 0,
+                   quality_compressor: int = COMPRESSOR_BSC) -> bytes:
+    """mod.rs:430-482 for encoding 0, lossless qualities, BSC for the header
+    and sequence streams and BSC or quality_ctx for the quality stream:
+    prefix, body, and the constant lengths when either is set."""
     has_const = const_seq_len > 0 or const_qual_len > 0
     header_size = V2_PREFIX_SIZE + BODY_SIZE + (CONST_LEN_SIZE if has_const else 0)
     out = bytearray(ARCHIVE_MAGIC)
     out += bytes([ARCHIVE_VERSION, 0])
     out += struct.pack("<I", header_size)
-    out += bytes([ENCODING_PLAIN, 0x02 if has_const else 0x00, BINNING_LOSSLESS, COMPRESSOR_BSC, COMPRESSOR_BSC,
+    out += bytes([ENCODING_PLAIN, 0x02 if has_const else 0x00, BINNING_LOSSLESS, quality_compressor, COMPRESSOR_BSC,
                   COMPRESSOR_BSC, 0, 0, 0])  # quality model / delta / dictionary disabled
     out += struct.pack("<H", 0)              # template prefix length
     out += bytes([0])                        # no comment
@@ -325,11 +330,56 @@ def compress_records(records, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
     return archive_header(len(records), len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
 
 
-def compress_fastq(fastq: bytes, block_size: int = bsc.BSC_BLOCK_SIZE) -> bytes:
-    """FASTQ text -> .qz v2 archive bytes."""
+def _threads(n: int) -> int:
+    import os
+    return max(1, min(n, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)))
+
+
+def _fastq_columns(fastq: bytes, n: int):
+    """(sequences, qualities) of the n records of valid FASTQ text, as lists
+    of bytes (the text was validated by the native scan)."""
+    lines = fastq.split(b"\n")
+    return lines[1:4 * n:4], lines[3:4 * n:4]
+
+
+def quality_ctx_stream(qualities, sequences, block_size: int = QUALITY_CTX_BLOCK_SIZE) -> bytes:
+    """The quality stream of a quality_ctx archive (compress_impl.rs:100-126,
+    written by write_blocks like the BSC streams): [u32 blobs][u32 len, blob]...,
+    one quality_ctx blob per block_size records, compressed in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import quality_ctx as QC
+    n = len(qualities)
+    if n == 0:
+        return b""
+    spans = [(i, min(n, i + block_size)) for i in range(0, n, block_size)]
+    with ThreadPoolExecutor(_threads(len(spans))) as ex:
+        blobs = list(ex.map(lambda sp: QC.compress_qualities_ctx(qualities[sp[0]:sp[1]], sequences[sp[0]:sp[1]]),
+                            spans))
+    parts = [struct.pack("<I", len(blobs))]
+    for b in blobs:
+        parts += [struct.pack("<I", len(b)), b]
+    return b"".join(parts)
+
+
+def compress_fastq(fastq: bytes, block_size: int = bsc.BSC_BLOCK_SIZE, quality_compressor: str = "auto",
+                   quality_ctx_block_size: int = QUALITY_CTX_BLOCK_SIZE) -> bytes:
+    """FASTQ text -> .qz v2 archive bytes. quality_compressor: "bsc" (7-bit
+    packed qualities in BSC blocks), "quality_ctx", or "auto", qz's rule for
+    lossless qualities (compress_impl.rs:203-206): quality_ctx from
+    MIN_READS_QUALITY_CTX records on. quality_ctx archives record no
+    constant quality length (compress_impl.rs:210-215)."""
+    if quality_compressor not in ("auto", "bsc", "quality_ctx"):
+        raise ValueError(f"unknown quality compressor {quality_compressor!r}")
     n, cs, cq, buf, lens = fastq_to_stream_buffer(fastq)
-    hz, sz, qz = _compress_stream_buffer(buf, lens, block_size)
-    return archive_header(n, len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
+    ctx = quality_compressor == "quality_ctx" or (quality_compressor == "auto" and n >= MIN_READS_QUALITY_CTX)
+    if not ctx:
+        hz, sz, qz = _compress_stream_buffer(buf, lens, block_size)
+        return archive_header(n, len(hz), len(sz), len(qz), cs, cq) + hz + sz + qz
+    hl, sl, _ = lens
+    hz, sz = _compress_stream_buffer(buf[:hl + sl], [hl, sl], block_size)
+    seqs, quals = _fastq_columns(fastq, n)
+    qz = quality_ctx_stream(quals, seqs, quality_ctx_block_size)
+    return archive_header(n, len(hz), len(sz), len(qz), cs, 0, COMPRESSOR_QUALITY_CTX) + hz + sz + qz
 
 
 # ---- reader -----------------------------------------------------------------
@@ -416,7 +466,7 @@ def _check_supported(h: ArchiveHeader) -> None:
         raise ArchiveError(f"encoding type {h.encoding_type} is not read by this build")
     if h.quality_binning != BINNING_LOSSLESS:
         raise ArchiveError(f"quality binning {h.quality_binning} is not read by this build")
-    if h.qualities_len and h.quality_compressor != COMPRESSOR_BSC:
+    if h.qualities_len and h.quality_compressor not in (COMPRESSOR_BSC, COMPRESSOR_QUALITY_CTX):
         raise ArchiveError(f"quality compressor {h.quality_compressor} is not read by this build")
     if h.sequence_compressor != COMPRESSOR_BSC or h.header_compressor != COMPRESSOR_BSC:
         raise ArchiveError("only BSC sequence and header streams are read by this build")
@@ -425,12 +475,15 @@ def _check_supported(h: ArchiveHeader) -> None:
 
 
 def decompress_streams(buf, batch: int = DECOMPRESS_BATCH_SIZE):
-    """(header, raw streams h, s, q). The three streams advance together:
-    each batch call takes up to `batch` blocks of every stream (qz runs
-    three decompressor threads, each 8 blocks at a time), so one device
-    batch and one host coder pool serve all three."""
+    """(header, raw streams h, s, q) of an archive whose three streams are
+    BSC. The three streams advance together: each batch call takes up to
+    `batch` blocks of every stream (qz runs three decompressor threads, each
+    8 blocks at a time), so one device batch and one host coder pool serve
+    all three."""
     h = parse_archive_header(buf)
     _check_supported(h)
+    if h.qualities_len and h.quality_compressor == COMPRESSOR_QUALITY_CTX:
+        raise ArchiveError("the quality stream is quality_ctx, not BSC: decode the archive with decompress_archive")
     per = [stream_blocks(buf, o, n) for o, n in stream_offsets(h)]
     out = [[], [], []]
     for i in range(0, max(len(b) for b in per), batch):
@@ -537,10 +590,51 @@ def _decompress_stream_buffer(buf):
     return h, out, (sum(dsz[:n0]), sum(dsz[n0:n1]), sum(dsz[n1:]))
 
 
+def _sequence_list(h: ArchiveHeader, ss) -> list:
+    """The records' sequences from a decoded sequence stream."""
+    n = h.num_reads
+    ss = bytes(ss)
+    if h.const_seq_len:
+        L = h.const_seq_len
+        if len(ss) < n * L:
+            raise ArchiveError("truncated sequence stream")
+        return [ss[i * L:(i + 1) * L] for i in range(n)]
+    st, ln = _scan_lengths(ss, n, "sequence")
+    return [ss[a:a + b] for a, b in zip(st.tolist(), ln.tolist())]
+
+
+def _decompress_quality_ctx_archive(buf, h: ArchiveHeader) -> bytes:
+    """decompress_impl.rs:1103-1200 with quality_ctx qualities: the header
+    and sequence streams through one BSC batch call, then the quality_ctx
+    blobs with the sequences as their context, then the records."""
+    from . import quality_ctx as QC
+    spans = stream_offsets(h)
+    per = [stream_blocks(buf, o, n) for o, n in spans[:2]]
+    out, dsz = bsc.decompress_batch_buffer(per[0] + per[1])
+    hl = sum(dsz[:len(per[0])])
+    hs, ss = out[:hl].tobytes(), out[hl:].tobytes()
+    seqs = _sequence_list(h, ss)
+    o, n = spans[2]
+    try:
+        quals = QC.decompress_quality_ctx_multiblock(bytes(memoryview(buf)[o:o + n]), seqs)
+    except QC.QualityCtxError as e:
+        raise ArchiveError(f"quality_ctx stream: {e}") from None
+    hst, hln = _scan_lengths(hs, h.num_reads, "header")
+    parts = []
+    for i, (a, b) in enumerate(zip(hst.tolist(), hln.tolist())):
+        parts += [hs[a:a + b], b"\n", seqs[i], b"\n+\n", quals[i], b"\n"]
+    return b"".join(parts)
+
+
 def decompress_archive(buf, batch: int | None = None) -> bytes:
     """.qz v2 archive bytes -> FASTQ text. By default all blocks go through
     one batch call; batch=k decodes k blocks of every stream per call, as
-    qz's decompressor threads do (decompress_streams)."""
+    qz's decompressor threads do (decompress_streams). Quality streams
+    written by quality_ctx decode with the sequences as context."""
+    h0 = parse_archive_header(buf)
+    _check_supported(h0)
+    if h0.qualities_len and h0.quality_compressor == COMPRESSOR_QUALITY_CTX:
+        return _decompress_quality_ctx_archive(buf, h0)
     if batch is not None:
         h, hs, ss, qs = decompress_streams(buf, batch)
         return streams_to_fastq(h, hs, ss, qs)

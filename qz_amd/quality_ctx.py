@@ -110,7 +110,14 @@ def decompress_quality_ctx_multiblock(data, sequences):
         cur += nr
     if cur != len(sequences):
         raise QualityCtxError(-6, f"multiblock: blocks cover {cur} reads but {len(sequences)} sequences provided")
+    # the blobs decode independently, in parallel (as the reference's second
+    # phase does); the native decoder runs without the interpreter lock
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+    threads = max(1, min(len(parts), os.cpu_count() or 1))
+    with ThreadPoolExecutor(threads) as ex:
+        decoded = list(ex.map(lambda p: decompress_qualities_ctx(p[0], sequences[p[1]:p[1] + p[2]], p[2]), parts))
     out = []
-    for blob, start, nr in parts:
-        out.extend(decompress_qualities_ctx(blob, sequences[start:start + nr], nr))
+    for d in decoded:
+        out.extend(d)
     return out

@@ -230,3 +230,35 @@ def test_native_conversions_parallel_stretches(kind):
     bad = text[:len(text) // 2] + b"\n" + text[len(text) // 2:]
     with pytest.raises(A.ArchiveError):
         A.fastq_to_streams(bad)
+
+
+def test_quality_ctx_archive_small():
+    """quality_compressor="quality_ctx": the header records compressor 4 and
+    no constant quality length; the quality stream is [u32 blobs][u32 len,
+    blob]... of quality_ctx blobs (compress_impl.rs:100-126, write_blocks),
+    one per quality_ctx_block_size records. Streams this small are stored
+    BSC blocks, so no device is needed."""
+    from qz_amd import quality_ctx as QC
+    recs = [A.FastqRecord(b"@a", b"ACGT", b"II#!"), A.FastqRecord(b"@b", b"GGA", b"#I5"),
+            A.FastqRecord(b"@c", b"TTTT", b"FFFF")]
+    text = A.format_fastq(recs)
+    for per in (A.QUALITY_CTX_BLOCK_SIZE, 2, 1):
+        arc = A.compress_fastq(text, quality_compressor="quality_ctx", quality_ctx_block_size=per)
+        h = A.parse_archive_header(arc)
+        assert (h.quality_compressor, h.const_qual_len) == (A.COMPRESSOR_QUALITY_CTX, 0)
+        o, n = A.stream_offsets(h)[2]
+        blobs = [bytes(b) for b in A.stream_blocks(arc, o, n)]
+        spans = [(i, min(len(recs), i + per)) for i in range(0, len(recs), per)]
+        assert blobs == [QC.compress_qualities_ctx([r.quality for r in recs[a:b]], [r.sequence for r in recs[a:b]])
+                         for a, b in spans]
+        assert A.decompress_archive(arc) == text
+        with pytest.raises(A.ArchiveError):
+            A.decompress_streams(arc)
+    # "auto" keeps BSC below MIN_READS_QUALITY_CTX records
+    assert A.parse_archive_header(A.compress_fastq(text)).quality_compressor == A.COMPRESSOR_BSC
+    # a blob count that does not cover the reads fails
+    arc = bytearray(A.compress_fastq(text, quality_compressor="quality_ctx", quality_ctx_block_size=1))
+    h = A.parse_archive_header(bytes(arc))
+    struct.pack_into("<Q", arc, 8 + 12, 2)  # two reads stated, three blobs of one read
+    with pytest.raises(A.ArchiveError):
+        A.decompress_archive(bytes(arc))

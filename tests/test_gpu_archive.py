@@ -13,7 +13,9 @@
 * batches of 60 MB, coded and decoded over several device sub-batches, give
   the per-block bytes;
 * segments stored raw (bytes rebuilt from the device runs) in a block of
-  reads and random bytes.
+  reads and random bytes;
+* quality_ctx archives (qz's choice from 100,000 reads on) round-trip, in
+  one or several blobs, next to the BSC-quality archive of the same text.
 
 No torch here: the library must initialise the device itself.
 """
@@ -180,3 +182,26 @@ def test_trim_releases_workspaces_and_pinned_buffers():
     assert bsc.compress_batch(blocks) == comp
     assert bsc.compress_adaptive_no_lzp(blocks[0]) == one
     assert bsc.decompress_batch(comp) == [b.tobytes() for b in blocks]
+
+
+@pytest.mark.parametrize("variable", [False, True])
+def test_quality_ctx_archive_auto_rule(variable):
+    """From MIN_READS_QUALITY_CTX records on, "auto" writes quality_ctx
+    qualities as qz does (compress_impl.rs:203-206), in 500,000-record blobs
+    by default; the archive decodes back, as does the BSC-quality archive of
+    the same text."""
+    text = synth.fastq(A.MIN_READS_QUALITY_CTX, 150, 23, variable=variable)
+    print("quality_ctx archive: variable", variable, len(text), "bytes", flush=True)
+    arc = A.compress_fastq(text)
+    h = A.parse_archive_header(arc)
+    assert h.quality_compressor == A.COMPRESSOR_QUALITY_CTX and h.const_qual_len == 0
+    assert h.const_seq_len == (0 if variable else 150)
+    assert A.decompress_archive(arc) == text
+    small = A.compress_fastq(text, quality_ctx_block_size=30_000)  # four blobs, decoded in parallel
+    o, n = A.stream_offsets(A.parse_archive_header(small))[2]
+    assert len(A.stream_blocks(small, o, n)) == 4
+    assert A.decompress_archive(small) == text
+    b = A.compress_fastq(text, quality_compressor="bsc")
+    assert A.parse_archive_header(b).quality_compressor == A.COMPRESSOR_BSC
+    assert A.decompress_archive(b) == text
+    print("sizes: quality_ctx", len(arc), "bsc", len(b), flush=True)
