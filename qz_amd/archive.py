@@ -26,10 +26,12 @@ stream_decompressor_inner (72-135: blocks in batches of 8, a block claiming
 more than 64 MB is rejected), the offset scans (444-496) and the record
 assembly of reconstruct_records_parallel (504-636: "id\\nseq\\n+\\nqual\\n").
 
-This build covers the container qz writes by default for the BSC path:
-encoding 0, lossless 7-bit qualities (binning code 0), and BSC for the
-quality, sequence and header streams (compressor codes 1/1/1). Archives
-using other encodings or codecs raise ArchiveError.
+This build covers the container qz writes by default: encoding 0, lossless
+qualities (binning code 0), BSC for the sequence and header streams
+(compressor code 1), and for the quality stream either BSC over 7-bit packed
+qualities (code 1) or quality_ctx blobs (code 4), which qz itself picks from
+MIN_READS_QUALITY_CTX records on (compress_impl.rs:203-206). Archives using
+other encodings or codecs raise ArchiveError.
 
 Blocks go through the batch C-ABI (bsc.compress_batch /
 bsc.decompress_batch), where qz fans out per-block libbsc calls over rayon:
