@@ -205,3 +205,17 @@ def test_quality_ctx_archive_auto_rule(variable):
     assert A.parse_archive_header(b).quality_compressor == A.COMPRESSOR_BSC
     assert A.decompress_archive(b) == text
     print("sizes: quality_ctx", len(arc), "bsc", len(b), flush=True)
+
+
+def test_concurrent_batch_calls_share_the_engines():
+    """Six batch calls at once, each of two device sub-batches, with four
+    engines: the calls wait for engines, each keeping one for its device
+    work only, and all finish with the bytes of a lone call."""
+    from concurrent.futures import ThreadPoolExecutor
+    blocks = [synth.sequence_stream(2_000_000, 80_000, 150, 30 + k, 0.002)[:12_000_000] for k in range(3)]
+    print("concurrent batch calls:", [b.size for b in blocks], flush=True)
+    want = bsc.compress_batch(blocks)
+    with ThreadPoolExecutor(6) as ex:
+        got = list(ex.map(lambda _: bsc.compress_batch(blocks), range(6)))
+    assert all(g == want for g in got)
+    assert bsc.decompress_batch(want) == [b.tobytes() for b in blocks]
