@@ -308,28 +308,45 @@ int decode_segment(const uint8_t* in, int in_size, uint8_t* out, int n)
     dec.init(in, in_size);
     const uint32_t word = dec.get_word();
     if ((int64_t)word != (int64_t)n) return -6;
-    uint8_t list[256];
+    uint8_t list[256] = {};
     const int nsym = code_table(dec, list, 0);
     if (nsym < 1) return -6;
     std::unique_ptr<Model> M = new_model();
     RunState st(nsym, n);
     int pos = 0;
+    // Up to 8 symbols, the list lives in one word (byte i = list[i]) and a
+    // run's move to position `rank` is done with masks; short runs are
+    // written as one 16-byte store when 16 bytes remain. Both avoid branches
+    // whose trip counts depend on decoded values.
+    const bool small = nsym <= 8;
+    uint64_t W = 0;
+    if (small) memcpy(&W, list, 8);
     while (pos < n) {
-        const int c = list[0];
+        const int c = small ? (int)(W & 0xff) : list[0];
         int rank = 0;
         uint32_t len = 0;
         if (!code_run(dec, *M, st, c, rank, len)) return -6;
         if (len > (uint32_t)(n - pos) || dec.overrun()) return -6;
-        if (len <= 16) {
-            for (uint32_t i = 0; i < len; ++i) out[pos + (int)i] = (uint8_t)c;
+        if (n - pos >= 16) {
+            uint8_t fill[16];
+            memset(fill, c, 16);
+            memcpy(out + pos, fill, 16);
+            if (len > 16) memset(out + pos + 16, c, len - 16);
         } else {
             memset(out + pos, c, len);
         }
         pos += (int)len;
         if (rank > 0) {
             if (rank >= nsym) return -6;
-            for (int i = 0; i < rank; ++i) list[i] = list[i + 1];
-            list[rank] = (uint8_t)c;
+            if (small) {
+                const int sh = 8 * rank;
+                const uint64_t low = (W >> 8) & ((1ull << sh) - 1);
+                const uint64_t high = sh + 8 >= 64 ? 0 : W & ~((1ull << (sh + 8)) - 1);
+                W = low | ((uint64_t)c << sh) | high;
+            } else {
+                for (int i = 0; i < rank; ++i) list[i] = list[i + 1];
+                list[rank] = (uint8_t)c;
+            }
         } else if (pos != n) {
             return -6;
         }

@@ -202,3 +202,17 @@ def test_libbsc_payloads_are_rejected_not_misdecoded():
         out = np.zeros(n, np.uint8)
         rc = L.qzbsc_coder_decompress(pay.ctypes.data_as(u8p), pay.size, out.ctypes.data_as(u8p), n, 0)
         assert rc == -6
+
+
+def test_coder_roundtrip_small_alphabets_and_short_segments():
+    """Decoder paths: alphabets up to 8 symbols (list in one word) and larger,
+    runs written by 16-byte stores, segments shorter than 16 bytes."""
+    rng = np.random.default_rng(7)
+    for _ in range(400):
+        nsym = int(rng.integers(1, 12))
+        n = int(rng.integers(1, 3000))
+        runs = rng.geometric(1.0 / float(rng.uniform(1, 30)), n)
+        u = np.repeat(rng.integers(0, nsym, n).astype(np.uint8) + 60, runs)[:n].copy()
+        r, back = _coder_rt(u)
+        if r >= 0:
+            assert np.array_equal(back, u), (nsym, n)
