@@ -87,12 +87,13 @@ def cpu_baseline(blocks, threads: int):
     }
 
 
-def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
-    """Batched vs per-block over blocks of log-uniform size in 100 KB..25 MiB
-    from the mixed generators (one device pass per sub-batch of <= cap
-    bytes, versus one call per block)."""
+def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int, lo: int = 100_000, hi: int | None = None):
+    """Batched vs per-block over blocks of log-uniform size in lo..hi bytes
+    (default 100 KB..25 MiB) from the mixed generators (one device pass per
+    sub-batch of <= cap bytes, versus one call per block)."""
     t = time.perf_counter()
-    blocks, kinds = synth.mixed_blocks(count, seed)
+    hi = synth.BLOCK if hi is None else hi
+    blocks, kinds = synth.mixed_blocks(count, seed, lo, hi)
     sizes = np.array([b.size for b in blocks], dtype=np.int32)
     stream = np.concatenate(blocks)
     total = int(sizes.sum())
@@ -102,6 +103,22 @@ def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
     d_bwt = torch.empty_like(d_in)
     d_back = torch.empty_like(d_in)
     offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    # batched: contiguous sub-batches of <= cap bytes
+    groups, cur, acc = [], [], 0
+    for b in range(count):
+        if cur and acc + int(sizes[b]) > cap:
+            groups.append(cur)
+            cur, acc = [], 0
+        cur.append(b)
+        acc += int(sizes[b])
+    groups.append(cur)
+    # warm-up, untimed: one block alone, then the largest sub-batch (the
+    # workspace grows to its size and block count once, the tuner settles)
+    for g in ([0], max(groups, key=lambda g: int(sizes[g[0]:g[-1] + 1].sum()))):
+        o = int(offs[g[0]])
+        sz = sizes[g[0]:g[-1] + 1]
+        prim, _, _ = bwt.dev_encode_batch(d_in.data_ptr() + o, d_bwt.data_ptr() + o, sz)
+        bwt.dev_decode_batch(d_bwt.data_ptr() + o, d_back.data_ptr() + o, sz, prim)
     torch.cuda.synchronize()
     # per-block
     lat = []
@@ -117,15 +134,6 @@ def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
     t_pb = time.perf_counter() - t0
     ok_pb = torch.equal(d_back, d_in)
     d_back.zero_()
-    # batched: contiguous sub-batches of <= cap bytes
-    groups, cur, acc = [], [], 0
-    for b in range(count):
-        if cur and acc + int(sizes[b]) > cap:
-            groups.append(cur)
-            cur, acc = [], 0
-        cur.append(b)
-        acc += int(sizes[b])
-    groups.append(cur)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     blat = []
@@ -145,8 +153,8 @@ def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int):
     del d_in, d_bwt, d_back
     torch.cuda.empty_cache()
     return {
-        "workload": f"{count} blocks, log-uniform 100 KB..25 MiB, seeded mix of sequence/header/quality "
-                    f"generators ({total} B); forward BWT + inverse BWT",
+        "workload": f"{count} blocks, log-uniform {lo / 1e3:.0f} KB..{hi / 2**20:.2f} MiB, seeded mix of "
+                    f"sequence/header/quality generators ({total} B); forward BWT + inverse BWT",
         "blocks": count,
         "bytes": total,
         "per_block": {"jobs_per_s": round(count / t_pb, 2), "mb_s": round(total / t_pb / 1e6, 2),
@@ -532,6 +540,10 @@ def main():
     if not args.no_uneven:
         uneven = uneven_section(torch, bwt, synth, args.uneven_blocks, 4, args.batch_cap)
         log(f"uneven: {uneven}")
+        # many small jobs: where one call per block leaves the GPU idle
+        small = uneven_section(torch, bwt, synth, 2000, 6, args.batch_cap, 100_000, 100_000)
+        log(f"small blocks: {small}")
+        uneven["small_blocks"] = small
 
     serving = None
     if not args.no_serve:
