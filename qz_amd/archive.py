@@ -174,7 +174,10 @@ def unpack_qualities(packed: bytes, nqual: int) -> bytes:
 # ---- FASTQ text ---------------------------------------------------------------
 
 def parse_fastq(data: bytes) -> list:
-    """Four-line FASTQ records (id line starting with '@', '+' separator)."""
+    """Four-line FASTQ records as qz's FastqReader reads them (io/fastq.rs:
+    141-197): the id line as it is, the third line starting with '+',
+    trailing '\r's dropped from each line, the quality as long as the
+    sequence."""
     lines = data.split(b"\n")
     if lines and lines[-1] == b"":
         lines.pop()
@@ -183,8 +186,11 @@ def parse_fastq(data: bytes) -> list:
     recs = []
     for i in range(0, len(lines), 4):
         h, s, p, q = lines[i:i + 4]
-        if not h.startswith(b"@") or not p.startswith(b"+"):
-            raise ArchiveError(f"FASTQ: malformed record {i // 4}")
+        if not p.startswith(b"+"):
+            raise ArchiveError(f"FASTQ: record {i // 4} has no '+' separator line")
+        h, s, q = h.rstrip(b"\r"), s.rstrip(b"\r"), q.rstrip(b"\r")
+        if len(q) != len(s):
+            raise ArchiveError(f"FASTQ: record {i // 4}: sequence length {len(s)} != quality length {len(q)}")
         recs.append(FastqRecord(h, s, q))
     return recs
 
@@ -341,7 +347,7 @@ def _fastq_columns(fastq: bytes, n: int):
     """(sequences, qualities) of the n records of valid FASTQ text, as lists
     of bytes (the text was validated by the native scan)."""
     lines = fastq.split(b"\n")
-    return lines[1:4 * n:4], lines[3:4 * n:4]
+    return [x.rstrip(b"\r") for x in lines[1:4 * n:4]], [x.rstrip(b"\r") for x in lines[3:4 * n:4]]
 
 
 def quality_ctx_stream(qualities, sequences, block_size: int = QUALITY_CTX_BLOCK_SIZE) -> bytes:

@@ -2,6 +2,9 @@
 // layout of qz's records_to_streams (crates/qz-lib/src/compression/
 // mod.rs:280-320) for encoding 0 with lossless qualities, and the offset
 // scans and record assembly of its decoder (decompress_impl.rs:444-636).
+// Records are read as qz's FastqReader reads them (io/fastq.rs:141-197):
+// four lines, the third starting with '+', trailing '\r's dropped, the
+// quality as long as the sequence; the id line is taken as it is.
 //
 // Streams, per record in order:
 //   header   varint(len) + id (the id keeps its '@')
@@ -113,6 +116,13 @@ inline bool next_line(const unsigned char* t, long long size, long long* pos, Li
     return true;
 }
 
+// Line ends as qz's reader takes them (io/fastq.rs:132-139): trailing '\r's
+// are dropped, so CRLF text reads as LF text.
+inline void trim_cr(const unsigned char* t, Line* l)
+{
+    while (l->e > l->s && t[l->e - 1] == '\r') --l->e;
+}
+
 // Texts and stream sets below this size convert on one thread.
 constexpr long long kParallelBytes = 4ll << 20;
 
@@ -159,8 +169,12 @@ bool parse_records(const unsigned char* fastq, long long size, long long start, 
         if (!next_line(fastq, size, &pos, &h) || !next_line(fastq, size, &pos, &s) ||
             !next_line(fastq, size, &pos, &p) || !next_line(fastq, size, &pos, &q))
             return false;  // a record of fewer than four lines
-        if (h.e == h.s || fastq[h.s] != '@' || p.e == p.s || fastq[p.s] != '+') return false;
+        if (p.e == p.s || fastq[p.s] != '+') return false;
+        trim_cr(fastq, &h);
+        trim_cr(fastq, &s);
+        trim_cr(fastq, &q);
         const long long hl = h.e - h.s, sl = s.e - s.s, ql = q.e - q.s;
+        if (ql != sl) return false;  // io/fastq.rs:183-190
         sc->h += varint_len((unsigned long long)hl) + hl;
         sc->s_raw += sl;
         sc->s_var += varint_len((unsigned long long)sl) + sl;

@@ -196,18 +196,32 @@ def test_native_fastq_edge_cases():
     assert A.streams_to_fastq(hdr, np.zeros(0, np.uint8), np.zeros(0, np.uint8), np.zeros(0, np.uint8)) == b""
     # no trailing newline
     assert A.fastq_to_streams(b"@a\nAC\n+\nII")[:3] == (1, 2, 2)
-    for bad in (b"@a\nAC\n+\n", b"a\nAC\n+\nII\n", b"@a\nAC\n-\nII\n"):
+    # qz's reader rules (io/fastq.rs:141-197): a missing line, no '+' line,
+    # or a quality of another length than the sequence fail ...
+    for bad in (b"@a\nAC\n+\n", b"@a\nAC\n-\nII\n", b"@a\nAC\n+\nI\n", b"@a\nAC\n+\nIII\n"):
         with pytest.raises(A.ArchiveError):
             A.fastq_to_streams(bad)
+        with pytest.raises(A.ArchiveError):
+            A.parse_fastq(bad)
+    # ... an id line is taken as it is, with or without '@' ...
+    assert A.fastq_to_streams(b"a\nAC\n+\nII\n")[3] == b"\x01a"
+    assert A.parse_fastq(b"\nAC\n+x\nII\n") == [A.FastqRecord(b"", b"AC", b"II")]
+    # ... and trailing '\r's are dropped: CRLF text gives the LF streams
+    lf = b"@r1 x\nACGT\n+\nIIII\n@r2\nGG\n+r2\n#I\n"
+    crlf = lf.replace(b"\n", b"\r\n")
+    assert A.fastq_to_streams(crlf) == A.fastq_to_streams(lf)
+    assert A.parse_fastq(crlf) == A.parse_fastq(lf)
+    assert A.fastq_to_streams(b"@a\r\r\nAC\r\n+\r\nII\r\r")[3] == b"\x02@a"
 
 
-@pytest.mark.parametrize("kind", ["quality_at_signs", "sequence_at_signs"])
+@pytest.mark.parametrize("kind", ["quality_at_signs", "sequence_at_signs", "crlf"])
 def test_native_conversions_parallel_stretches(kind):
     """Texts over 4 MB are cut at record starts and converted in parallel.
     Quality lines beginning with '@' (phred 31) must not be taken for record
     starts. Sequence lines beginning with '@' above quality lines beginning
     with '+' do look like record starts; the cut is then not confirmed and
-    the text is parsed in one piece. Both give the per-record streams."""
+    the text is parsed in one piece. CRLF text gives the streams of its LF
+    form. All give the per-record streams."""
     rng = np.random.default_rng(15)
     recs = []
     for i in range(40_000):
@@ -223,7 +237,8 @@ def test_native_conversions_parallel_stretches(kind):
     assert len(text) > 8 << 20
     cs, cq = A.detect_const_lengths(recs)
     h, s, q = A.records_to_streams(recs, cs, cq)
-    assert A.fastq_to_streams(text) == (len(recs), cs, cq, h, s, q)
+    src = text.replace(b"\n", b"\r\n") if kind == "crlf" else text  # CRLF reads as LF
+    assert A.fastq_to_streams(src) == (len(recs), cs, cq, h, s, q)
     hdr = A.parse_archive_header(A.archive_header(len(recs), len(h), len(s), len(q), cs, cq) + h + s + q)
     assert A.streams_to_fastq(hdr, h, s, q) == text
     # a malformed record deep inside a stretch still fails the whole text
