@@ -1487,23 +1487,26 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     uint64_t large_n = 0;
     uint32_t* hc = (uint32_t*)pinned_;
     for (;;) {
-        QZ_HIP(hipMemcpyAsync(hc, gls[cur].cnt, kClasses * 4, hipMemcpyDeviceToHost, s));
-        QZ_HIP(hipMemcpyAsync(hc + 16, overflow, 4, hipMemcpyDeviceToHost, s));
+        // one copy brings back the current set's class counts and the
+        // overflow word, which lies between the two sets in cnt_
+        const int lo = cur ? kOverflow : 0, hi = cur ? kCntB + kClasses : kOverflow + 1;
+        QZ_HIP(hipMemcpyAsync(hc, cnt_ + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToHost, s));
         if (rounds_ == 1)
-            QZ_HIP(hipMemcpyAsync(hc + 20, &((const PackParams*)ppd_)->k0, 8, hipMemcpyDeviceToHost, s));  // k0, rle
+            QZ_HIP(hipMemcpyAsync(hc + 64, &((const PackParams*)ppd_)->k0, 8, hipMemcpyDeviceToHost, s));  // k0, rle
         QZ_HIP(hipStreamSynchronize(s));
         if (rounds_ == 1) {
-            h = ((volatile uint32_t*)hc)[20];
-            rle = ((volatile uint32_t*)hc)[21] != 0;
+            h = ((volatile uint32_t*)hc)[64];
+            rle = ((volatile uint32_t*)hc)[65] != 0;
             k0_ = h;
         }
+        const int cofs = (cur ? kCntB : 0) - lo;
         uint32_t counts[kClasses];
         uint64_t total = 0;
         for (int k = 0; k < kClasses; ++k) {
-            counts[k] = ((volatile uint32_t*)hc)[k];
+            counts[k] = ((volatile uint32_t*)hc)[cofs + k];
             total += counts[k];
         }
-        if (((volatile uint32_t*)hc)[16]) {
+        if (((volatile uint32_t*)hc)[kOverflow - lo]) {
             fprintf(stderr, "qzbsc: group list overflow\n");
             return QZ_GPU_ERROR;
         }
