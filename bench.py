@@ -416,8 +416,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only; blocks are independent and would be "
-              "dealt to GPUs round-robin with no collective (see DESIGN.md). Run with --gpus 1.",
+        print("bench.py: this benchmark measures one GPU only; blocks are independent and would be "
+              "dealt to GPUs round-robin with no collective. Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
