@@ -166,20 +166,27 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
 {
     constexpr uint32_t D = 1u << BITS;
     constexpr int W = kSortThreads / QZ_WAVE;
-    __shared__ uint32_t cnt[W][D];
+    // R copies of each wave's counters, a digit's copies in adjacent words
+    // (different banks), the copy chosen by lane: lanes of one wave that
+    // share a digit (frequent with 2-bit symbols packed into digits) then
+    // mostly update different words instead of queueing on one.
+    constexpr uint32_t R = BITS <= 9 ? 4u : 1u;
+    __shared__ uint32_t cnt[W][D * R];
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
-    for (uint32_t d = tid; d < D; d += kSortThreads)
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, rep = tid & (R - 1);
+    for (uint32_t d = tid; d < D * R; d += kSortThreads)
 #pragma unroll
         for (int w = 0; w < W; ++w) cnt[w][d] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < td.len; i += kSortThreads)
-        atomicAdd(&cnt[wave][(uint32_t)(keys[td.start + i] >> shift) & (D - 1)], 1u);
+        atomicAdd(&cnt[wave][((uint32_t)(keys[td.start + i] >> shift) & (D - 1)) * R + rep], 1u);
     __syncthreads();
     for (uint32_t d = tid; d < D; d += kSortThreads) {
         uint32_t t = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) t += cnt[w][d];
+        for (int w = 0; w < W; ++w)
+#pragma unroll
+            for (uint32_t r = 0; r < R; ++r) t += cnt[w][d * R + r];
         hist[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] = t;
     }
 }
