@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
-    ap.add_argument("--serve-jobs", type=int, default=40)
+    ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
