@@ -316,7 +316,8 @@ def archive_section(synth, reads: int, threads: int):
                     "bsc_decompress call per block from a pool of host threads"}
 
 
-def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int):
+def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int,
+                    lane_bytes: int = 32 << 20):
     """Poisson arrivals of compress (forward BWT) and decompress (inverse
     BWT) jobs, 70/30, each 1..40 blocks of 100 KB..25 MiB from the mixed
     generators; continuous batching with HBM-footprint admission."""
@@ -335,10 +336,11 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
     mean_bytes = float(np.mean([t["inp"].size for t in templates]))
     res = {"workload": f"Poisson arrivals, 70% compress (forward BWT) / 30% decompress (inverse BWT), "
                        f"jobs of 1..40 blocks of 100 KB..25 MiB (mean {mean_bytes / 1e6:.1f} MB), "
-                       f"{jobs_per_rate} jobs per rate, {workers} workers",
+                       f"{jobs_per_rate} jobs per rate, {workers} workers + a lane for jobs <= "
+                       f"{lane_bytes >> 20} MiB",
            "rates": []}
     budget = 200 << 30
-    with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers) as srv:
+    with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers, small_lane_bytes=lane_bytes) as srv:
         # each job alone (both kinds), twice; the second run counts
         solo = {}
         for rep in range(2):
@@ -409,7 +411,8 @@ def main():
                     help="first copy this many MiB device-to-device (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
     ap.add_argument("--serve-jobs", type=int, default=100)
-    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-lane-mib", type=int, default=128, help="small-job lane: jobs of at most this many MiB")
+    ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
     ap.add_argument("--archive-reads", type=int, default=1_000_000)
@@ -549,7 +552,8 @@ def main():
     if not args.no_serve:
         from qz_amd import serve
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers)
+        serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers,
+                                  args.serve_lane_mib << 20)
 
     streams = None
     if not args.no_streams:
