@@ -551,6 +551,11 @@ def main():
     serving = None
     if not args.no_serve:
         from qz_amd import serve
+        from qz_amd._native import lib as _lib
+        # the server's workers keep their own full-batch workspaces resident:
+        # give back what the idle engines of the sections above still hold
+        torch.cuda.empty_cache()
+        log(f"trim before serving: {_lib().qzbsc_trim() / 1e9:.1f} GB released")
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers,
                                   args.serve_lane_mib << 20)
