@@ -93,7 +93,7 @@ std::unique_ptr<Model> new_model()
 
 struct RunState {
     int nsym = 0, maxE = 0, maxLE = 1;
-    int flagsV = 0, flagsL = 0;
+    uint32_t flagsV = 0, flagsL = 0;  // recent zero-test / length flags, newest in bit 0 (shifts wrap)
     uint8_t rankHist[256];  // per symbol: class of its last rank (0: zero, else min(e + 1, 7))
     uint8_t runHist[256];   // per symbol: class of its last run length
     RunState(int nsym_, int n)
@@ -134,7 +134,7 @@ inline bool code_run(IO& io, Model& M, RunState& st, int c, int& rank, uint32_t&
         }
         if (v > st.nsym - 1) return false;
         st.rankHist[c] = (uint8_t)(nz ? std::min(e + 1, 7) : 0);
-        st.flagsV = (st.flagsV << 1) | nz;
+        st.flagsV = (st.flagsV << 1) | (uint32_t)nz;
         if (!IO::kEnc) rank = (v == st.nsym - 1) ? 0 : v + 1;
     } else if (!IO::kEnc) {
         rank = 0;
@@ -165,7 +165,7 @@ inline bool code_run(IO& io, Model& M, RunState& st, int c, int& rank, uint32_t&
         }
     }
     st.runHist[c] = (uint8_t)(gt1 ? std::min(e + 1, 7) : 0);
-    st.flagsL = (st.flagsL << 1) | gt1;
+    st.flagsL = (st.flagsL << 1) | (uint32_t)gt1;
     if (!IO::kEnc) len = L;
     return true;
 }
