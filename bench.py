@@ -14,6 +14,18 @@ Two forms are timed:
 
 The CPU baseline runs the oracle (SA-IS + LF-walk restatement of libbsc's
 BWT stages, not upstream libsais) one block per thread on the host.
+
+Further sections, each in the same JSON line (and each with a --no-* flag):
+  * graph_replay — one-block jobs, eager launches versus HIP-graph replay;
+  * uneven_batch — 512 blocks of 100 KB..25 MiB from mixed generators, and
+    under small_blocks 2000 blocks of 100 KB, batched versus per-block;
+  * serving — Poisson arrivals of compress / decompress jobs through the
+    serving loop (continuous batching, footprint admission, small-job lane):
+    p50 / p99 latency against each job alone;
+  * streams — config 3 (binned qualities) and the config-2 header stream as
+    their own batches;
+  * archive — FASTQ -> .qz -> FASTQ end to end (BSC qualities, and qz's
+    quality_ctx choice), and the batch block calls against one call per block.
 """
 from __future__ import annotations
 
