@@ -288,14 +288,37 @@ __device__ __forceinline__ uint32_t sub_depth(const uint32_t* __restrict__ dep_i
 __global__ __launch_bounds__(kThreads) void k_byte_hist(const uint8_t* __restrict__ text, uint32_t n,
                                                         uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t h[4][256];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
-    for (int w = 0; w < 4; ++w) h[w][tid] = 0;
+    // R copies of each wave's counters, a value's copies in adjacent words,
+    // chosen by lane: 16 counts per lane per load, from few distinct bytes
+    // (DNA), would otherwise queue on a handful of words
+    constexpr uint32_t R = 4;
+    __shared__ uint32_t h[4][256 * R];
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, rep = tid & (R - 1);
+    for (uint32_t j = tid; j < 256 * R; j += kThreads)
+        for (int w = 0; w < 4; ++w) h[w][j] = 0;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
-        atomicAdd(&h[wave][text[i]], 1u);
+    // the 16-byte aligned middle in 16-byte loads; the unaligned head and the
+    // tail byte by byte (device-resident callers may pass any offset)
+    const uint32_t head = (uint32_t)min((uint64_t)((16u - ((uintptr_t)text & 15u)) & 15u), (uint64_t)n);
+    const uint32_t nv = (n - head) / 16u;
+    const uint32_t tail = head + nv * 16u;
+    const uint32_t g = blockIdx.x * kThreads + tid, gs = gridDim.x * kThreads;
+    for (uint32_t i = g; i < head; i += gs) atomicAdd(&h[wave][text[i] * R + rep], 1u);
+    for (uint32_t i = tail + g; i < n; i += gs) atomicAdd(&h[wave][text[i] * R + rep], 1u);
+    const uint4* v = reinterpret_cast<const uint4*>(text + head);
+    for (uint32_t i = g; i < nv; i += gs) {
+        const uint4 x = v[i];
+        const uint32_t w4[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) atomicAdd(&h[wave][((w4[k] >> (8 * b)) & 255u) * R + rep], 1u);
+    }
     __syncthreads();
-    uint32_t t = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+    uint32_t t = 0;
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) t += h[w][tid * R + r];
     if (t) atomicAdd(&hist[tid], t);
 }
 
