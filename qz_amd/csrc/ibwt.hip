@@ -40,15 +40,38 @@ __global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict
                                                         const TileDesc* __restrict__ tiles,
                                                         uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t cnt[4][256];
+    // R copies of each wave's counters, a value's copies in adjacent words,
+    // chosen by lane: 16 counts per lane per load from few distinct bytes
+    // would otherwise queue on a handful of words
+    constexpr uint32_t R = 4;
+    __shared__ uint32_t cnt[4][256 * R];
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
-    for (int w = 0; w < 4; ++w) cnt[w][tid] = 0;
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, rep = tid & (R - 1);
+    for (uint32_t j = tid; j < 256 * R; j += kIThreads)
+        for (int w = 0; w < 4; ++w) cnt[w][j] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < td.len; i += kIThreads) atomicAdd(&cnt[wave][U[td.start + i]], 1u);
+    // tiles start at any byte: the 16-byte aligned middle in 16-byte loads,
+    // the head before it and the tail after it byte by byte
+    const uint8_t* t = U + td.start;
+    const uint32_t head = min((uint32_t)((16u - ((uintptr_t)t & 15u)) & 15u), td.len);
+    const uint32_t nv = (td.len - head) / 16u, tail = head + nv * 16u;
+    for (uint32_t i = tid; i < head; i += kIThreads) atomicAdd(&cnt[wave][t[i] * R + rep], 1u);
+    for (uint32_t i = tail + tid; i < td.len; i += kIThreads) atomicAdd(&cnt[wave][t[i] * R + rep], 1u);
+    const uint4* v = reinterpret_cast<const uint4*>(t + head);
+    for (uint32_t i = tid; i < nv; i += kIThreads) {
+        const uint4 x = v[i];
+        const uint32_t w4[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) atomicAdd(&cnt[wave][((w4[k] >> (8 * b)) & 255u) * R + rep], 1u);
+    }
     __syncthreads();
-    const uint32_t t = cnt[0][tid] + cnt[1][tid] + cnt[2][tid] + cnt[3][tid];
-    hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] = t;
+    uint32_t s = 0;
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) s += cnt[w][tid * R + r];
+    hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] = s;
 }
 
 // hist has been exclusive-scanned: entry = off[b] + #(< c in block) + #(c in
