@@ -399,20 +399,28 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
                                                           const uint32_t* __restrict__ brun,
                                                           uint32_t* __restrict__ keys)
 {
+    // the workgroup's 4096 positions and the 511 a run-length key may look
+    // ahead: W window bytes, W remaining run lengths, W block-start flags;
+    // the packed path reuses the same LDS for its 4096 keys
+    constexpr uint32_t W = kThreads * kKeyItems + kRleLook;
+    // packed path: 4096 keys, then the 4096 positions and the 31 (k0 <= 32)
+    // that their keys read ahead
+    constexpr uint32_t KB = kThreads * kKeyItems * 4, WP = kThreads * kKeyItems + 32;
+    constexpr uint32_t kSmem = 4 * W > KB + WP ? 4 * W : KB + WP;
+    static_assert(W % 16 == 0 && WP % 16 == 0 && KB % 16 == 0, "LDS carve");
     __shared__ uint16_t code[256];
+    __shared__ __align__(16) uint8_t smem[kSmem];
     code[threadIdx.x] = ppd->code[threadIdx.x];
     const uint32_t radix = ppd->radix, k0 = ppd->k0;
     __syncthreads();
     const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kKeyItems;
     if (ppd->rle) {  // uniform: the whole grid takes this branch or none
-        // the workgroup's 4096 positions and the 511 the keys may look
-        // ahead, staged in LDS with each entry's remaining run length
-        // rem (capped at cap + 1; runs end at block ends)
-        constexpr uint32_t W = kThreads * kKeyItems + kRleLook;
+        // window entries with each entry's remaining run length rem
+        // (capped at cap + 1; runs end at block ends)
         constexpr uint32_t C = (W + kThreads - 1) / kThreads;  // entries per thread
-        __shared__ uint8_t win[W];
-        __shared__ uint16_t rem[W];
-        __shared__ uint8_t bstart[W];  // 1: a block starts here
+        uint8_t* win = smem;
+        uint16_t* rem = reinterpret_cast<uint16_t*>(smem + W);
+        uint8_t* bstart = smem + 3 * W;  // 1: a block starts here
         const PackParams& pp = *ppd;
         const uint32_t base = blockIdx.x * kThreads * kKeyItems;
         for (uint32_t k = threadIdx.x; k < W; k += kThreads) {
@@ -497,31 +505,62 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         store_keys(keys, i0, n, out);
         return;
     }
-    if (i0 >= n) return;
-    uint32_t top = 1;  // weight of the leading code
-    for (uint32_t j = 1; j < k0; ++j) top *= radix;
-    uint32_t end = 0, key = 0;
-    uint32_t out[kKeyItems];
-#pragma unroll
-    for (int q = 0; q < kKeyItems; ++q) {
-        const uint32_t i = i0 + q;
-        if (i < n) {
-            if (i >= end) {  // first position or a new block: full key
-                end = bm.off[bm.find(i) + 1];
-                key = 0;
-                for (uint32_t j = 0; j < k0; ++j) {
-                    const uint32_t p = i + j;
-                    key = key * radix + ((p < end) ? code[text[p]] : 0u);
-                }
+    // packed keys: the workgroup stages its text window in LDS (16-byte
+    // loads when the text is aligned), each thread slides its key over 16
+    // positions, the keys go to LDS, and the workgroup stores them as
+    // contiguous 16-byte vectors
+    const uint32_t base = blockIdx.x * kThreads * kKeyItems;
+    if (base >= n) return;  // uniform over the workgroup
+    uint32_t* kl = reinterpret_cast<uint32_t*>(smem);
+    uint8_t* win = smem + KB;
+    {
+        const uint8_t* tb = text + base;
+        const bool al = ((uintptr_t)tb & 15u) == 0;
+        for (uint32_t v = threadIdx.x; v < WP / 16u; v += kThreads) {
+            const uint32_t p = v * 16u;
+            if (al && base + p + 16u <= n) {
+                *reinterpret_cast<uint4*>(win + p) = *reinterpret_cast<const uint4*>(tb + p);
             } else {
-                const uint32_t lead = code[text[i - 1]];
-                const uint32_t p = i + k0 - 1;
-                key = (key - lead * top) * radix + ((p < end) ? code[text[p]] : 0u);
+                for (uint32_t b = 0; b < 16u; ++b) win[p + b] = base + p + b < n ? tb[p + b] : 0;
             }
         }
-        out[q] = key;
     }
-    store_keys(keys, i0, n, out);
+    __syncthreads();
+    if (i0 < n) {
+        uint32_t top = 1;  // weight of the leading code
+        for (uint32_t j = 1; j < k0; ++j) top *= radix;
+        uint32_t end = 0, key = 0;
+        uint32_t out[kKeyItems];
+#pragma unroll
+        for (int q = 0; q < kKeyItems; ++q) {
+            const uint32_t i = i0 + q;
+            if (i < n) {
+                if (i >= end) {  // first position or a new block: full key
+                    end = bm.off[bm.find(i) + 1];
+                    key = 0;
+                    for (uint32_t j = 0; j < k0; ++j) {
+                        const uint32_t p = i + j;
+                        key = key * radix + ((p < end) ? code[win[p - base]] : 0u);
+                    }
+                } else {
+                    const uint32_t lead = code[win[i - 1 - base]];
+                    const uint32_t p = i + k0 - 1;
+                    key = (key - lead * top) * radix + ((p < end) ? code[win[p - base]] : 0u);
+                }
+            }
+            out[q] = key;
+        }
+        uint4* dst = reinterpret_cast<uint4*>(kl + threadIdx.x * kKeyItems);
+#pragma unroll
+        for (int v = 0; v < kKeyItems / 4; ++v)
+            dst[v] = make_uint4(out[4 * v], out[4 * v + 1], out[4 * v + 2], out[4 * v + 3]);
+    }
+    __syncthreads();
+    const uint32_t cnt = min(n - base, (uint32_t)(kThreads * kKeyItems));
+    const uint4* src = reinterpret_cast<const uint4*>(kl);
+    uint4* kv = reinterpret_cast<uint4*>(keys + base);
+    for (uint32_t v = threadIdx.x; v < cnt / 4u; v += kThreads) kv[v] = src[v];
+    for (uint32_t i = (cnt & ~3u) + threadIdx.x; i < cnt; i += kThreads) keys[base + i] = kl[i];
 }
 
 // Symbol packing from the byte histogram (one workgroup of 256): present
