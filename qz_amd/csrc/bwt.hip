@@ -676,22 +676,29 @@ __global__ __launch_bounds__(kThreads) void k_tile_lasthead(const uint32_t* __re
                                                             const TileDesc* __restrict__ tiles,
                                                             uint32_t* __restrict__ lh)
 {
+    // backwards in slices of 256 slots, stopping at the first slice that
+    // holds a head (keys that change often: one slice)
     __shared__ uint32_t wmax[kThreads / QZ_WAVE];
     const TileDesc td = tiles[blockIdx.x];
-    uint32_t m = 0;
-    for (uint32_t i = threadIdx.x; i < td.len; i += kThreads)
-        if (slot_head(keys, td, i)) m = td.start + i;  // i grows: the last head wins
-    for (int d = 1; d < QZ_WAVE; d <<= 1) {
-        const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
-        m = o > m ? o : m;
+    uint32_t r = 0;
+    for (uint32_t hi = td.len; hi > 0;) {
+        const uint32_t lo = hi > kThreads ? hi - kThreads : 0;
+        const uint32_t i = lo + threadIdx.x;
+        const bool h = i < hi && slot_head(keys, td, i);
+        if (__syncthreads_or(h)) {  // the same for every thread
+            uint32_t m = h ? td.start + i : 0u;
+            for (int d = 1; d < QZ_WAVE; d <<= 1) {
+                const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
+                m = o > m ? o : m;
+            }
+            if (lane_id() == 0) wmax[threadIdx.x / QZ_WAVE] = m;
+            __syncthreads();
+            for (int w = 0; w < kThreads / QZ_WAVE; ++w) r = wmax[w] > r ? wmax[w] : r;
+            break;
+        }
+        hi = lo;
     }
-    if (lane_id() == 0) wmax[threadIdx.x / QZ_WAVE] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t r = 0;
-        for (int w = 0; w < kThreads / QZ_WAVE; ++w) r = wmax[w] > r ? wmax[w] : r;
-        lh[blockIdx.x] = r;
-    }
+    if (threadIdx.x == 0) lh[blockIdx.x] = r;
 }
 
 // Round-0 finish, one tile per workgroup: SA, ranks and the unsorted groups.
