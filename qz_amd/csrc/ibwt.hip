@@ -513,8 +513,12 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     // descriptors are built when the graph is not replayed)
     uint32_t ntiles = 0;
     const uint32_t hbase = count_tiles(offs.data(), nblk, &ntiles, 256);
-    uint32_t nsplit = 0;
-    for (uint32_t b = 0; b < nblk; ++b) nsplit += (offs[b + 1] - offs[b]) / split + 1;
+    uint32_t nsplit = 0, chain = 1;  // chain: the most splitters of one block
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t m = (offs[b + 1] - offs[b]) / split + 1;
+        nsplit += m;
+        chain = m > chain ? m : chain;
+    }
     if (ntiles > cap_tiles_ || nsplit > cap_split_) return QZ_BAD_PARAMETER;
     auto prepare = [&](hipStream_t st) -> int {
         // host staging lives in the engine, so the copies may complete late
@@ -553,9 +557,10 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         // later rounds alternate between d0_ and d1_
         const uint32_t* din = len_;
         uint32_t *na = nxt0_, *nb_ = nxt1_, *dout = d0_;
-        // ceil(log2(nsplit)) + 1 rounds always suffice
+        // a block's splitters form one chain (its LF cycle), so
+        // ceil(log2(longest chain)) + 1 rounds always suffice
         int rounds = 1;
-        while ((1u << rounds) < nsplit && rounds < 31) ++rounds;
+        while ((1u << rounds) < chain && rounds < 31) ++rounds;
         for (int r = 0; r <= rounds; ++r) {
             hipLaunchKernelGGL(k_ib_jump, dim3(gs), dim3(kIThreads), 0, st, na, din, nb_, dout, nsplit, flag_);
             QZ_HIP(hipGetLastError());
