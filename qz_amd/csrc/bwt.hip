@@ -1207,22 +1207,58 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
 
 // libsais-convention BWT of every block: U[0] = T[n-1]; U[j] = T[SA[j-1]-1]
 // for 0 < j < p, T[SA[j]-1] for j >= p, with p = 1 + rank of suffix 0.
+// The text as 4-bit symbol codes, eight per word, when the batch has at most
+// 15 distinct bytes (sigma + 1 <= 16): k_bwt_out's random gathers then read
+// half the bytes, so twice as much of a block's text stays in L2. A larger
+// alphabet leaves the kernel idle and k_bwt_out reads the bytes.
+__global__ __launch_bounds__(kThreads) void k_pack_nib(const uint8_t* __restrict__ text, uint32_t n,
+                                                       const PackParams* __restrict__ ppd, uint32_t* __restrict__ nib)
+{
+    if (ppd->radix > 16u) return;  // uniform over the grid
+    __shared__ uint8_t code[256];
+    code[threadIdx.x] = (uint8_t)ppd->code[threadIdx.x];
+    __syncthreads();
+    const uint64_t i0 = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) * 8u;  // text bytes [i0, i0 + 8)
+    if (i0 >= n) return;
+    uint32_t w = 0;
+    if (i0 + 8u <= n && ((uintptr_t)(text + i0) & 7u) == 0) {
+        const uint2 v = *reinterpret_cast<const uint2*>(text + i0);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+            w |= (uint32_t)code[(v.x >> (8 * b)) & 255u] << (4 * b);
+            w |= (uint32_t)code[(v.y >> (8 * b)) & 255u] << (16 + 4 * b);
+        }
+    } else {
+        for (uint32_t b = 0; b < 8u; ++b)
+            if (i0 + b < n) w |= (uint32_t)code[text[i0 + b]] << (4 * b);
+    }
+    nib[i0 / 8u] = w;
+}
+
 __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict__ text,
+                                                      const uint32_t* __restrict__ nib,
+                                                      const PackParams* __restrict__ ppd,
                                                       const uint32_t* __restrict__ sa,
                                                       const uint32_t* __restrict__ rank,
                                                       BlockMap bm,
                                                       uint32_t n, uint8_t* __restrict__ out)
 {
+    // code -> byte for the packed form (k_pack_nib ran with the same test)
+    __shared__ uint8_t dec[16];
+    const bool packed = ppd->radix <= 16u;
+    if (packed) {
+        const uint32_t cd = ppd->code[threadIdx.x];
+        if (cd != 0u && cd < 16u) dec[cd] = (uint8_t)threadIdx.x;
+    }
+    __syncthreads();
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
     if (j >= n) return;
     const uint32_t b = bm.find(j);
     const uint32_t o = bm.off[b], e = bm.off[b + 1];
     const uint32_t local = j - o;
     const uint32_t p = rank[o] - o + 1u;
-    uint8_t c;
-    if (local == 0) c = text[e - 1];
-    else c = text[(local < p ? sa[j - 1] : sa[j]) - 1u];
-    out[j] = c;
+    const uint32_t x = local == 0 ? e - 1u : (local < p ? sa[j - 1] : sa[j]) - 1u;
+    out[j] = packed ? dec[(nib[x >> 3] >> (4u * (x & 7u))) & 15u] : text[x];
 }
 
 // Per block: primary index and sampled indexes I[k] = 1 + rank(k*r).
@@ -1744,8 +1780,13 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
     build_block_tab(offs_host_, n, btab_host_);
     QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
     QZ_TRY(sort_suffixes(text, n, nblk, s));
-    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text, sa_, rank_, bmap_, n,
-                       out);
+    // the sort is done with its keys: keys0_ (4 bytes per position) holds the
+    // 4-bit text when the alphabet allows it
+    hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
+                       (const PackParams*)ppd_, keys0_);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text,
+                       (const uint32_t*)keys0_, (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
     QZ_HIP(hipGetLastError());
