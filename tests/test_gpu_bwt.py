@@ -120,3 +120,54 @@ def test_round0_key_modes_give_the_same_bwt(gpu, monkeypatch, mode):
     _check(blocks, gpu)
     for blk in blocks[:2]:
         _check([blk], gpu)
+
+
+def _hip_runtime():
+    """The HIP runtime that the library has already loaded in this process."""
+    import ctypes as C
+    with open("/proc/self/maps") as f:
+        path = next(line.split()[-1] for line in f if "libamdhip64" in line)
+    return C.CDLL(path)
+
+
+def test_device_entry_at_unaligned_pointers(gpu):
+    """The device-resident batch entry with input and output at odd byte
+    offsets (the round-0 and packing kernels take 16- and 8-byte loads only
+    when the text is aligned): forward against the oracle, inverse back to
+    the input. One batch of at most 15 symbols (4-bit text for the output
+    gathers) and one with a full byte alphabet."""
+    import ctypes as C
+    from qz_amd import bwt
+    hip = _hip_runtime()
+    rng = np.random.default_rng(91)
+    low = [synth.sequence_stream(50_000, 300, 150, 5, 0.002, 0.001)[:40_003], synth.quality_stream(300, 150, 6)[:30_001]]
+    high = [rng.integers(0, 256, 25_005, dtype=np.uint8), synth.header_stream(500, 7)[:20_007]]
+    for batch in (low, high):
+        sizes = np.array([b.size for b in batch], np.int32)
+        flat = np.ascontiguousarray(np.concatenate(batch), dtype=np.uint8)
+        n = flat.size
+        for off_in, off_out in ((1, 3), (8, 5), (3, 0)):
+            print("sigma", np.unique(flat).size, "offsets", off_in, off_out, "sizes", sizes.tolist(), flush=True)
+            bufs = [C.c_void_p(), C.c_void_p(), C.c_void_p()]
+            for d in bufs:
+                assert hip.hipMalloc(C.byref(d), C.c_size_t(n + 64)) == 0
+            try:
+                d_in, d_out, d_back = (bufs[0].value + off_in, bufs[1].value + off_out, bufs[2].value + off_in)
+                assert hip.hipMemcpy(C.c_void_p(d_in), flat.ctypes.data_as(C.c_void_p), C.c_size_t(n), 1) == 0
+                prim, aux, _ = bwt.dev_encode_batch(d_in, d_out, sizes)
+                out = np.empty(n, np.uint8)
+                assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(d_out), C.c_size_t(n), 2) == 0
+                pos = 0
+                for b, blk in enumerate(batch):
+                    u, p, a = O.bwt_aux(blk)
+                    assert np.array_equal(out[pos:pos + blk.size], u), f"block {b} BWT differs"
+                    assert prim[b] == p, f"block {b} primary {prim[b]} != {p}"
+                    assert np.array_equal(aux[b][:len(a)], a), f"block {b} aux differs"
+                    pos += blk.size
+                bwt.dev_decode_batch(d_out, d_back, sizes, prim)
+                back = np.empty(n, np.uint8)
+                assert hip.hipMemcpy(back.ctypes.data_as(C.c_void_p), C.c_void_p(d_back), C.c_size_t(n), 2) == 0
+                assert np.array_equal(back, flat), "inverse round trip differs"
+            finally:
+                for d in bufs:
+                    hip.hipFree(d)
