@@ -34,6 +34,13 @@ extern "C" {
 #define LIBBSC_BLOCKSORTER_BWT 1
 #define LIBBSC_CODER_QLFC_STATIC 1
 #define LIBBSC_CODER_QLFC_ADAPTIVE 2
+#define LIBBSC_CODER_QLFC_FAST 3
+/* Coder id this library writes into bits 5-7 of a block's mode word: its own
+ * adaptive rank/run model, whose bitstream is not libbsc's QLFC. libbsc
+ * defines ids 1-3 only, so upstream libbsc rejects these blocks instead of
+ * misreading them, and this library reports libbsc-coded blocks (ids 1-3) as
+ * LIBBSC_NOT_SUPPORTED. */
+#define QZBSC_CODER_QZ_ADAPTIVE 4
 #define LIBBSC_FEATURE_FASTMODE 1
 #define LIBBSC_FEATURE_MULTITHREADING 2
 #define LIBBSC_FEATURE_CUDA 8 /* "use the GPU" — selects the HIP path here */
@@ -49,8 +56,9 @@ QZBSC_API int bsc_init(int features);
  * bsc.rs:172-183 and bsc.rs:234-247. `output` holds n + 28 + 1024 bytes
  * (bsc.rs:170). Returns the block size or a negative error. Parameter sets
  * qz uses: (0, 0, BWT, ADAPTIVE) default, (0, 0, BWT, STATIC), (16, 128, BWT,
- * ADAPTIVE) with LZP; the static and fast coder requests are served by the
- * adaptive coder and recorded as such in the block header. */
+ * ADAPTIVE) with LZP. Every coder request (static, adaptive, fast) is served
+ * by this library's adaptive model and recorded in the block header as
+ * QZBSC_CODER_QZ_ADAPTIVE; LZP blocks use this library's token format. */
 QZBSC_API int bsc_compress(const unsigned char* input, unsigned char* output, int n, int lzpHashSize,
                            int lzpMinLen, int blockSorter, int coder, int features);
 
@@ -60,7 +68,8 @@ QZBSC_API int bsc_block_info(const unsigned char* blockHeader, int headerSize, i
                              int features);
 
 /* Replaces libbsc bsc_decompress; bound at bsc.rs:118-124, called at
- * bsc.rs:421-429 with outputSize == dataSize. Returns 0 on success. */
+ * bsc.rs:421-429 with outputSize == dataSize. Returns 0 on success, and
+ * LIBBSC_NOT_SUPPORTED for blocks coded by libbsc's own QLFC coders. */
 QZBSC_API int bsc_decompress(const unsigned char* input, int inputSize, unsigned char* output, int outputSize,
                              int features);
 

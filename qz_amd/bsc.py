@@ -28,6 +28,9 @@ QZBSC_GPU_ERROR = -7
 LIBBSC_BLOCKSORTER_BWT = 1
 LIBBSC_CODER_QLFC_STATIC = 1
 LIBBSC_CODER_QLFC_ADAPTIVE = 2
+LIBBSC_CODER_QLFC_FAST = 3
+# the coder id this library writes into a block's mode word (its own model)
+QZBSC_CODER_QZ_ADAPTIVE = 4
 LIBBSC_FEATURE_FASTMODE = 1
 LIBBSC_FEATURE_MULTITHREADING = 2
 LIBBSC_FEATURE_CUDA = 8
@@ -73,8 +76,8 @@ def compress_with_params(data, lzp_hash_size: int, lzp_min_len: int, block_sorte
 
 
 def compress(data) -> bytes:
-    """bsc.rs:146-150: the static-QLFC request, served by the adaptive coder
-    (the block header records coder 2)."""
+    """bsc.rs:146-150: the static-QLFC request, served by this library's
+    adaptive model (the block header records QZBSC_CODER_QZ_ADAPTIVE = 4)."""
     return compress_with_params(data, 0, 0, LIBBSC_BLOCKSORTER_BWT, LIBBSC_CODER_QLFC_STATIC)
 
 

@@ -5,6 +5,9 @@
 // Block layout (libbsc's, pinned by the libbsc archive under tests/golden):
 //   +0  i32 block size (header included)   +4  i32 n
 //   +8  i32 mode = sorter | coder << 5 | lzpMinLen << 8 | lzpHashSize << 16
+//       (coder: QZBSC_CODER_QZ_ADAPTIVE = 4 for every block this library
+//       codes, since its rank/run model is not libbsc's QLFC; libbsc's own
+//       ids 1-3 parse but their payloads are LIBBSC_NOT_SUPPORTED here)
 //   +12 i32 primary index                  +16 adler32(data)
 //   +20 adler32(bytes 28..end)             +24 adler32(bytes 0..23)
 //   28.. coder payload: [nb u8][nb x (in i32, out i32)][segments]
@@ -14,9 +17,9 @@
 // With LZP (mode bits 8..23 set) the data is LZP-encoded first (lzp.cpp),
 // the coder payload is preceded by the LZP output size m (i32), and the
 // BWT, primary and sampled indexes refer to those m bytes; a block that LZP
-// does not shrink is coded without it. Requests for the static (1) or fast
-// (3) QLFC coder are served by the adaptive coder, and the header records
-// coder 2, which is what the payload holds.
+// does not shrink is coded without it. Requests for the static (1), adaptive
+// (2) or fast (3) QLFC coder are all served by this library's adaptive model,
+// and the header records coder 4, which is what the payload holds.
 //
 // The forward and inverse BWT always run on the GPU (there is no CPU
 // transform in this library); the QLFC coder runs on the host.
@@ -449,7 +452,7 @@ int check_params(int n, int lzpHashSize, int lzpMinLen, int blockSorter, int cod
     if (coder < 1 || coder > 3) return QZ_BAD_PARAMETER;
     if ((lzpMinLen != 0 || lzpHashSize != 0) && !lzp::params_ok(lzpHashSize, lzpMinLen)) return QZ_BAD_PARAMETER;
     if (n < 0 || n > (1 << 30)) return QZ_BAD_PARAMETER;
-    *mode = blockSorter | (LIBBSC_CODER_QLFC_ADAPTIVE << 5);
+    *mode = blockSorter | (QZBSC_CODER_QZ_ADAPTIVE << 5);
     if (lzpHashSize) *mode |= (lzpMinLen << 8) | (lzpHashSize << 16);
     return 0;
 }
@@ -523,7 +526,8 @@ int parse_header(const uint8_t* h, int size, BlockHeader* bh)
     const int lzpMin = (bh->mode >> 8) & 0xff, lzpHash = (bh->mode >> 16) & 0xff;
     bool ok = bh->block_size >= kHeader && bh->data_size >= 0 && bh->index >= 0 && bh->index <= bh->data_size;
     if (bh->mode != 0) {
-        ok = ok && sorter >= 1 && sorter <= 8 && coder >= 1 && coder <= 3 && (bh->mode >> 24) == 0;
+        ok = ok && sorter >= 1 && sorter <= 8 && coder >= 1 && coder <= QZBSC_CODER_QZ_ADAPTIVE &&
+             (bh->mode >> 24) == 0;
         ok = ok && ((lzpMin == 0 && lzpHash == 0) || (lzpMin >= 4 && lzpHash >= 10 && lzpHash <= 28));
     } else {
         ok = ok && bh->block_size == bh->data_size + kHeader;
@@ -544,7 +548,7 @@ static int payload_tasks(const uint8_t* input, int inputSize, const BlockHeader&
         return 1;
     }
     const int sorter = bh.mode & 0x1f, coder = (bh.mode >> 5) & 0x7;
-    if (sorter != LIBBSC_BLOCKSORTER_BWT || coder != LIBBSC_CODER_QLFC_ADAPTIVE || (bh.mode >> 8) != 0)
+    if (sorter != LIBBSC_BLOCKSORTER_BWT || coder != QZBSC_CODER_QZ_ADAPTIVE || (bh.mode >> 8) != 0)
         return QZ_NOT_SUPPORTED;
     if (bh.data_size < 1 || bh.index < 1) return QZ_DATA_CORRUPT;
     const int num_indexes = input[bh.block_size - 1];
@@ -585,6 +589,8 @@ static uint64_t pipe_parts()
 int lzp_data_size(const uint8_t* input, const BlockHeader& bh)
 {
     if ((bh.mode >> 8) == 0 || bh.mode == 0) return 0;
+    if ((bh.mode & 0x1f) != LIBBSC_BLOCKSORTER_BWT || ((bh.mode >> 5) & 0x7) != QZBSC_CODER_QZ_ADAPTIVE)
+        return QZ_NOT_SUPPORTED;  // libbsc's LZP + QLFC formats are not this library's
     if (bh.block_size < kHeader + 4) return QZ_DATA_CORRUPT;
     const int m = (int)get32(input + kHeader);
     if (m < 1 || m >= bh.data_size || bh.index > m) return QZ_DATA_CORRUPT;
@@ -1048,6 +1054,7 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
             }
         });
     int dev_rc = 0;
+    size_t failed_from = batches.size();  // first sub-batch the device did not invert
     for (size_t j = 0; j < batches.size(); ++j) {
         std::vector<int> list;
         {
@@ -1056,12 +1063,22 @@ QZBSC_API int qzbsc_decompress_batch(const unsigned char* input, const int* in_s
             for (int b : batches[j])
                 if (!bad[b]) list.push_back(b);
         }
-        if (!list.empty() && !dev_rc) dev_rc = flush(guard->slot(), list);
+        if (!list.empty() && !dev_rc) {
+            dev_rc = flush(guard->slot(), list);
+            if (dev_rc) failed_from = j;
+        }
     }
     for (auto& t : pool) t.join();
     for (int b = 0; b < nblocks; ++b)
         if (bad[b] && !rcs[b]) rcs[b] = QZ_DATA_CORRUPT;
-    if (dev_rc) return dev_rc;
+    if (dev_rc) {
+        // the failed sub-batch and every later one were never inverted: their
+        // output still holds BWT bytes, so each of their blocks reports the error
+        for (size_t j = failed_from; j < batches.size(); ++j)
+            for (int b : batches[j])
+                if (!rcs[b]) rcs[b] = dev_rc;
+        return dev_rc;
+    }
     parallel_blocks(nblocks, [&](int b) {
         if (rcs[b]) return;
         uint8_t* out = output + ooff[b];

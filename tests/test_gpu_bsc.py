@@ -14,6 +14,9 @@ from qz_amd.bwt import BscError
 
 pytestmark = pytest.mark.gpu
 
+# BWT sorter | this library's coder id << 5 (libbsc's own blocks read 0x41)
+MODE_QZ = bsc.LIBBSC_BLOCKSORTER_BWT | (bsc.QZBSC_CODER_QZ_ADAPTIVE << 5)
+
 
 def _inputs():
     rng = np.random.default_rng(11)
@@ -40,7 +43,7 @@ def test_roundtrip_and_fields():
         if mode == 0:
             assert c[28:] == data
             continue
-        assert mode == 0x41
+        assert mode == MODE_QZ
         _, p, idx = O.bsc_bwt_encode(data)
         assert prim == p
         nidx = c[-1]
@@ -95,10 +98,10 @@ def test_lzp_and_coder_requests_roundtrip():
             assert bsc.decompress(blk) == bytes(data)
             mode = struct.unpack_from("<i", blk, 8)[0]
             if mode:
-                assert mode & 0xff == 0x41
+                assert mode & 0xff == MODE_QZ
     blk = bsc.compress_adaptive(rep)
     mode = struct.unpack_from("<i", blk, 8)[0]
-    assert mode == 0x41 | (128 << 8) | (16 << 16)
+    assert mode == MODE_QZ | (128 << 8) | (16 << 16)
     assert len(blk) < len(bsc.compress_adaptive_no_lzp(rep))
     for hs, ml in ((10, 4), (20, 32), (28, 255)):
         blk = bsc.compress_with_params(rep, hs, ml, 1, 2)
@@ -126,7 +129,7 @@ def test_large_block_mt_roundtrip():
     assert data.size == 400_000_000
     blk = bsc.compress_adaptive_mt_no_lzp(data)
     size, n, mode = struct.unpack_from("<iii", blk, 0)
-    assert size == len(blk) and n == data.size and mode == 0x41
+    assert size == len(blk) and n == data.size and mode == MODE_QZ
     assert blk[-1] == (n - 1) // O.sample_rate(n)
     assert len(blk) < data.size // 3
     back = bsc.decompress(blk)

@@ -190,16 +190,19 @@ def test_bad_parameters():
         assert r == -7 or r > 0, (args, r)
 
 
-def test_libbsc_payloads_are_rejected_not_misdecoded():
-    """This engine's adaptive model is its own (libbsc's model constants are
-    not restated): libbsc-written payloads must fail with -6, never decode
-    to wrong data silently."""
+def test_libbsc_coded_blocks_are_not_supported():
+    """libbsc's adaptive QLFC model is not restated here (its constants are
+    third-party source absent from the reference), so a libbsc block is
+    reported LIBBSC_NOT_SUPPORTED from its mode word (coder 2), and its bare
+    payload fails the coder's checks (-6) rather than decoding to wrong data."""
     L = qz_amd.lib()
     for blk in gold_blocks():
         n = struct.unpack_from("<i", blk, 4)[0]
+        a = np.frombuffer(blk, np.uint8).copy()
+        out = np.zeros(n, np.uint8)
+        assert L.bsc_decompress(a.ctypes.data_as(u8p), a.size, out.ctypes.data_as(u8p), n, 1) == -4
         nidx = blk[-1]
         pay = np.frombuffer(blk[28:len(blk) - 1 - 4 * nidx], np.uint8).copy()
-        out = np.zeros(n, np.uint8)
         rc = L.qzbsc_coder_decompress(pay.ctypes.data_as(u8p), pay.size, out.ctypes.data_as(u8p), n, 0)
         assert rc == -6
 
