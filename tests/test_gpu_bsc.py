@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from qz_amd import bsc, synth
+from qz_amd import bsc, bwt, synth
 from qz_amd.bwt import BscError
 
 pytestmark = pytest.mark.gpu
@@ -124,13 +124,41 @@ def test_lzp_block_corruption_fails():
 
 def test_large_block_mt_roundtrip():
     """qz's large-block MT mode (compress_adaptive_mt_no_lzp, ultra.rs:877-887
-    uses blocks up to 750 MB): one 400 MB block through the drop-in ABI."""
+    uses blocks up to 750 MB): one 400 MB block through the drop-in ABI. Its
+    BWT, primary index and trailer indexes equal the oracle's (SA-IS on the
+    host, run in a thread beside the GPU calls)."""
+    import threading
     data = synth.config2_sequences(2_700_000, seed=31)[:400_000_000]
     assert data.size == 400_000_000
+    ref = {}
+    th = threading.Thread(target=lambda: ref.update(zip(("u", "p", "idx"), O.bsc_bwt_encode(data))))
+    th.start()
+    print("large block: oracle started, GPU compress", flush=True)
+    blk = bsc.compress_adaptive_mt_no_lzp(data)
+    size, n, mode, prim = struct.unpack_from("<iiii", blk, 0)
+    assert size == len(blk) and n == data.size and mode == MODE_QZ
+    nidx = blk[-1]
+    assert nidx == (n - 1) // O.sample_rate(n)
+    trailer = list(struct.unpack_from(f"<{nidx}i", blk, len(blk) - 1 - 4 * nidx))
+    assert len(blk) < data.size // 3
+    back = bsc.decompress(blk)
+    assert back == data.tobytes()
+    del back
+    out, prims, _ = bwt.encode_batch([data])
+    th.join()
+    print("large block: oracle done", flush=True)
+    assert prim == ref["p"] == prims[0]
+    assert trailer == ref["idx"].tolist()
+    assert np.array_equal(out[0], ref["u"])
+
+
+def test_ultra_block_750mb_roundtrip():
+    """ultra's largest sequence block (750 MB, ultra.rs:44-55, 877-887)
+    through the drop-in ABI: header fields, sample count, round trip."""
+    data = synth.config2_sequences(5_000_000, seed=33)[:750_000_000]
+    assert data.size == 750_000_000
     blk = bsc.compress_adaptive_mt_no_lzp(data)
     size, n, mode = struct.unpack_from("<iii", blk, 0)
     assert size == len(blk) and n == data.size and mode == MODE_QZ
     assert blk[-1] == (n - 1) // O.sample_rate(n)
-    assert len(blk) < data.size // 3
-    back = bsc.decompress(blk)
-    assert back == data.tobytes()
+    assert bsc.decompress(blk) == data.tobytes()
