@@ -329,10 +329,12 @@ def archive_section(synth, reads: int, threads: int):
 
 
 def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, workers: int,
-                    lane_bytes: int = 32 << 20):
+                    lane_bytes: int = 32 << 20, unit_bytes: int = 64 << 20, policy: int = 0):
     """Poisson arrivals of compress (forward BWT) and decompress (inverse
     BWT) jobs, 70/30, each 1..40 blocks of 100 KB..25 MiB from the mixed
-    generators; continuous batching with HBM-footprint admission."""
+    generators; jobs cut into work units, continuous batching with
+    HBM-footprint admission within 288 GB. Request buffers are pinned host
+    memory; every job's result is compared with the same job run alone."""
     rng = np.random.default_rng(seed)
     pool, _ = synth.mixed_blocks(400, seed)
     templates = []
@@ -341,47 +343,67 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
         idx = rng.choice(len(pool), size=k, replace=False)
         blks = [pool[i] for i in idx]
         sizes = np.array([b.size for b in blks], dtype=np.int32)
-        inp = np.concatenate(blks)
         out, prim, _ = bwt.encode_batch(blks)
-        templates.append({"inp": inp, "sizes": sizes, "bwt": np.concatenate(out),
-                          "prim": np.array(prim, dtype=np.int32), "out": np.empty_like(inp)})
-    mean_bytes = float(np.mean([t["inp"].size for t in templates]))
+        inp = serve.PinnedBuffer(int(sizes.sum()))
+        np.concatenate(blks, out=inp.array)
+        bw = serve.PinnedBuffer(int(sizes.sum()))
+        np.concatenate(out, out=bw.array)
+        templates.append({"inp": inp, "sizes": sizes, "bwt": bw, "prim": np.array(prim, dtype=np.int32)})
+    mean_bytes = float(np.mean([t["sizes"].sum() for t in templates]))
+    # pinned output buffers: one as large as the largest template for the
+    # jobs run alone, and per rate one for every job, allocated before its
+    # arrivals start
+    big = int(max(t["sizes"].sum() for t in templates))
+    solo_out = serve.PinnedBuffer(big)
     res = {"workload": f"Poisson arrivals, 70% compress (forward BWT) / 30% decompress (inverse BWT), "
                        f"jobs of 1..40 blocks of 100 KB..25 MiB (mean {mean_bytes / 1e6:.1f} MB), "
                        f"{jobs_per_rate} jobs per rate, {workers} workers + a lane for jobs <= "
-                       f"{lane_bytes >> 20} MiB",
+                       f"{lane_bytes >> 20} MiB, work units of {unit_bytes >> 20} MiB, "
+                       f"{'oldest-job-first' if policy == 0 else 'fewest-bytes-left-first'} order, "
+                       "pinned request buffers",
            "rates": []}
-    budget = 200 << 30
-    with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers, small_lane_bytes=lane_bytes) as srv:
+    budget = 288 * 10**9
+    mismatches = 0
+
+    def submit(srv, t, kind, buf):
+        n = int(t["sizes"].sum())
+        o = buf.array[:n]
+        if kind == 0:
+            return srv.submit_raw(0, t["inp"].array, t["sizes"], o)
+        return srv.submit_raw(1, t["bwt"].array, t["sizes"], o, t["prim"])
+
+    def check(j, t, kind):
+        want = t["bwt"].array if kind == 0 else t["inp"].array
+        ok = np.array_equal(j.out, want) and (kind == 1 or j.prim_out.tolist() == t["prim"].tolist())
+        return 0 if ok else 1
+
+    with serve.Server(hbm_budget=budget, max_batch_bytes=1 << 30, workers=workers, small_lane_bytes=lane_bytes,
+                      unit_bytes=unit_bytes, policy=policy) as srv:
+        res["batch_cap_bytes"] = srv.batch_cap()
         # each job alone (both kinds), twice; the second run counts
         solo = {}
         for rep in range(2):
             for k, t in enumerate(templates):
                 for kind in (0, 1):
-                    if kind == 0:
-                        j = srv.submit_raw(0, t["inp"], t["sizes"], t["out"])
-                    else:
-                        j = srv.submit_raw(1, t["bwt"], t["sizes"], t["out"], t["prim"])
-                    srv.wait(j)
+                    j = srv.wait(submit(srv, t, kind, solo_out))
+                    mismatches += check(j, t, kind)
                     solo[(k, kind)] = j.latency_ms
         res["single_job_ms"] = round(float(np.median([solo[(k, 0)] for k in range(len(templates))])), 2)
+        res["single_job_p99_ms"] = round(float(np.percentile(list(solo.values()), 99)), 2)
         for rate in rates:
             arrivals = np.cumsum(rng.exponential(1.0 / rate, size=jobs_per_rate))
             picks = rng.integers(0, len(templates), size=jobs_per_rate)
             kinds = (rng.random(jobs_per_rate) < 0.3).astype(int)
+            outs = [serve.PinnedBuffer(int(templates[p]["sizes"].sum())) for p in picks]
             sub = []
             t0 = time.perf_counter()
-            for a, p, k in zip(arrivals, picks, kinds):
+            for i, (a, p, k) in enumerate(zip(arrivals, picks, kinds)):
                 dt = t0 + a - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
-                t = templates[p]
-                if k == 0:
-                    sub.append((srv.submit_raw(0, t["inp"], t["sizes"], t["out"]), t["inp"].size))
-                else:
-                    sub.append((srv.submit_raw(1, t["bwt"], t["sizes"], t["out"], t["prim"]), t["inp"].size))
+                sub.append(submit(srv, templates[p], int(k), outs[i]))
             lat, nbat, slow, alone = [], [], [], []
-            for (j, _), p, k in zip(sub, picks, kinds):
+            for j, p, k in zip(sub, picks, kinds):
                 srv.wait(j)
                 lat.append(j.latency_ms)
                 nbat.append(j.batch_jobs)
@@ -389,19 +411,36 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
                 alone.append(a)
                 slow.append(j.latency_ms / a)
             wall = time.perf_counter() - t0
-            nbytes = sum(b for _, b in sub)
+            # results are checked after the clock stops: comparing 100 MB
+            # per job on the host is not serving work
+            for j, p, k in zip(sub, picks, kinds):
+                mismatches += check(j, templates[p], int(k))
+            for o in outs:
+                o.free()
+            nbytes = sum(int(templates[p]["sizes"].sum()) for p in picks)
             res["rates"].append({"offered_jobs_per_s": rate, "jobs_per_s": round(len(sub) / wall, 2),
                                  "mb_s": round(nbytes / wall / 1e6, 1),
                                  "p50_ms": round(float(np.percentile(lat, 50)), 1),
                                  "p99_ms": round(float(np.percentile(lat, 99)), 1),
                                  "alone_p99_ms": round(float(np.percentile(alone, 99)), 1),
                                  "p99_vs_alone_p99": round(float(np.percentile(lat, 99) / np.percentile(alone, 99)), 2),
+                                 "p99_vs_median_single_job": round(float(np.percentile(lat, 99)) / res["single_job_ms"], 2),
                                  "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
                                  "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
                                  "mean_jobs_per_batch": round(float(np.mean(nbat)), 2)})
             log(f"serving rate {rate}: {res['rates'][-1]}")
         res["peak_inflight_bytes"] = srv.peak_bytes()
-    # correctness spot check: last template round trip through the server
+    for t in templates:
+        t["inp"].free()
+        t["bwt"].free()
+    solo_out.free()
+    # every served job (alone and in the mix) equals the same job's result
+    # from one direct batch call: mix invariance, checked on every job
+    res["results_checked"] = len(templates) * 4 + len(rates) * jobs_per_rate
+    res["results_mismatched"] = mismatches
+    if mismatches:
+        print(f"bench.py: {mismatches} served jobs differ from their results alone", file=sys.stderr)
+        sys.exit(1)
     return res
 
 
@@ -425,6 +464,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-lane-mib", type=int, default=128, help="small-job lane: jobs of at most this many MiB")
     ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-unit-mib", type=int, default=64, help="serving work units (MiB of whole blocks)")
+    ap.add_argument("--serve-policy", type=int, default=1, help="0 oldest job first, 1 fewest bytes left first")
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
     ap.add_argument("--archive-reads", type=int, default=1_000_000)
@@ -570,7 +611,7 @@ def main():
         log(f"trim before serving: {_lib().qzbsc_trim() / 1e9:.1f} GB released")
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         serving = serving_section(bwt, serve, synth, rates, args.serve_jobs, 5, args.serve_workers,
-                                  args.serve_lane_mib << 20)
+                                  args.serve_lane_mib << 20, args.serve_unit_mib << 20, args.serve_policy)
 
     streams = None
     if not args.no_streams:

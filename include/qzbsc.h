@@ -167,26 +167,46 @@ QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, u
  * it replaces qz's per-block rayon fan-out, mod.rs:402-404 and
  * decompress_impl.rs:117-120, for a long-running service).
  * create: HBM budget in bytes for the workspaces of batches in flight, a cap
- * on one batch's input bytes, and the worker count (one engine each).
+ * on one batch's input bytes, and the worker count (one engine each). Every
+ * worker reserves its workspaces for a full batch at creation; they never
+ * grow after that.
  * submit: kind 0 = forward BWT (compress side: fills out, primary_out[b]
  * and, if aux_out != NULL, 256 ints of sampled indexes per block), kind 1 =
- * inverse BWT (decompress side: needs primary_in). Buffers are host memory
- * and must stay valid until wait returns. Returns a job id (> 0) or < 0.
- * wait: blocks until the job is done; returns its status and, optionally,
- * its latency, its queueing time and the number of jobs in its batch. */
+ * inverse BWT (decompress side: needs primary_in). The job is cut into work
+ * units of whole blocks that workers batch with other jobs' units. Buffers
+ * are host memory (pinned buffers from qzbsc_host_alloc copy fastest) and
+ * must stay valid until wait returns. Returns a job id (> 0) or < 0
+ * (LIBBSC_NOT_ENOUGH_MEMORY for a block larger than the batch cap).
+ * wait: blocks until every unit of the job is done; returns its status and,
+ * optionally, its latency, its queueing time (until its first unit started)
+ * and the most jobs any of its batches held. A job can be waited on once. */
 QZBSC_API void* qzbsc_server_create(long long hbm_budget, long long max_batch_bytes, int workers);
 /* As qzbsc_server_create (which uses a 32 MiB lane), with the small-job lane
  * set explicitly: one more worker takes only jobs of at most
  * small_lane_bytes input bytes, so small requests do not queue behind
- * batches of large ones (0 = no lane). No counterpart in the reference. */
+ * batches of large ones (0 = no lane). Work units of 64 MiB, FIFO order. */
 QZBSC_API void* qzbsc_server_create_lanes(long long hbm_budget, long long max_batch_bytes, int workers,
                                           long long small_lane_bytes);
+/* As qzbsc_server_create_lanes, with the work-unit size (bytes of whole
+ * blocks per unit; 0 = the batch cap) and the scheduling order: policy 0 =
+ * oldest job first, 1 = the job with the fewest bytes not yet started first. */
+QZBSC_API void* qzbsc_server_create_opts(long long hbm_budget, long long max_batch_bytes, int workers,
+                                         long long small_lane_bytes, long long unit_bytes, int policy);
 QZBSC_API void qzbsc_server_destroy(void* srv);
 QZBSC_API int qzbsc_server_submit(void* srv, int kind, const unsigned char* in, unsigned char* out, const int* sizes,
                                   int nblocks, const int* primary_in, int* primary_out, int* aux_out);
 QZBSC_API int qzbsc_server_wait(void* srv, int job, double* latency_ms, double* queue_ms, int* batch_jobs);
 /* Largest footprint (bytes) the server had in flight. */
 QZBSC_API long long qzbsc_server_peak_bytes(void* srv);
+/* The batch cap the server settled on (input bytes per device batch). */
+QZBSC_API long long qzbsc_server_batch_cap(void* srv);
+/* Pinned (page-locked) host memory for request buffers, and its release.
+ * No counterpart in the reference. */
+QZBSC_API void* qzbsc_host_alloc(long long bytes);
+QZBSC_API void qzbsc_host_free(void* p);
+/* Free and total device memory (hipMemGetInfo), for admission checks and
+ * tests of resident memory. No counterpart in the reference. */
+QZBSC_API int qzbsc_device_memory(long long* free_bytes, long long* total_bytes);
 /* Device bytes one batch of `bytes` input in `blocks` blocks needs:
  * kind 0 forward BWT, kind 1 inverse BWT. */
 QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);

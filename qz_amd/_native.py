@@ -43,7 +43,13 @@ _SIGS = {
     "qzbsc_gpu_available": (C.c_int, []),
     "qzbsc_server_create": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int]),
     "qzbsc_server_create_lanes": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int, C.c_longlong]),
+    "qzbsc_server_create_opts": (C.c_void_p, [C.c_longlong, C.c_longlong, C.c_int, C.c_longlong, C.c_longlong,
+                                              C.c_int]),
     "qzbsc_server_destroy": (None, [C.c_void_p]),
+    "qzbsc_server_batch_cap": (C.c_longlong, [C.c_void_p]),
+    "qzbsc_host_alloc": (C.c_void_p, [C.c_longlong]),
+    "qzbsc_host_free": (None, [C.c_void_p]),
+    "qzbsc_device_memory": (C.c_int, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "qzbsc_server_submit": (C.c_int, [C.c_void_p, C.c_int, u8p, u8p, i32p, C.c_int, i32p, i32p, i32p]),
     "qzbsc_server_wait": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int)]),
