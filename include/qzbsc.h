@@ -140,6 +140,16 @@ QZBSC_API int qzbsc_dev_bwt_encode_batch(const void* d_in, void* d_out, const in
                                          int* primary, int* aux, float* gpu_ms);
 QZBSC_API int qzbsc_dev_bwt_decode_batch(const void* d_in, void* d_out, const int* sizes, int nblocks,
                                          const int* primary, float* gpu_ms);
+/* qzbsc_dev_bwt_decode_batch without any allocation, for callers that
+ * cannot allocate device memory (code under graph capture, tight serving
+ * loops). No counterpart in the reference. The workspace is a 64 MiB array
+ * every device the library runs on holds for the life of the process; a
+ * batch may hold at most 8 MiB (8,388,608 bytes) in at most 64 blocks, and
+ * larger batches return LIBBSC_NOT_ENOUGH_MEMORY. Runs on `stream` (a
+ * hipStream_t; NULL for the null stream) and returns after it finishes.
+ * Calls on one device share its array and are serialised in this process. */
+QZBSC_API int qzbsc_dev_bwt_decode_batch_noalloc(const void* d_in, void* d_out, const int* sizes, int nblocks,
+                                                 const int* primary, void* stream);
 
 /* Device QLFC front end of one BWT block (host buffers): the coder-segment
  * split and, per segment, its symbol table and per-run symbol, rank and

@@ -35,6 +35,7 @@ _SIGS = {
     "qzbsc_bwt_decode_batch": (C.c_int, [u8p, u8p, i32p, C.c_int, i32p]),
     "qzbsc_dev_bwt_encode_batch": (C.c_int, [C.c_void_p, C.c_void_p, i32p, C.c_int, i32p, i32p, f32p]),
     "qzbsc_dev_bwt_decode_batch": (C.c_int, [C.c_void_p, C.c_void_p, i32p, C.c_int, i32p, f32p]),
+    "qzbsc_dev_bwt_decode_batch_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, i32p, C.c_int, i32p, C.c_void_p]),
     "qzbsc_set_max_engines": (C.c_int, [C.c_int]),
     "qzbsc_qlfc_front": (C.c_int, [u8p, C.c_int, i32p, i32p, i32p, i32p, u8p, i32p, u8p, u8p, i32p]),
     "qzbsc_segment_header": (C.c_int, [u8p, C.c_int, i32p, u8p, i32p]),

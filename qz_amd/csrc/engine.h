@@ -69,6 +69,13 @@ class IbwtEngine {
 public:
     ~IbwtEngine() { release(); }
     int reserve(uint64_t n, uint32_t nblk);
+    // Places the workspace for batches of at most n bytes in nblk blocks in
+    // the device region [base, base + bytes) instead of allocating it (no
+    // U_/T_ staging: the caller passes device input and output), and runs
+    // every batch with eager launches (no graph). Returns
+    // LIBBSC_NOT_ENOUGH_MEMORY if the region is too small.
+    int bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk);
+    static size_t bound_bytes(uint64_t n, uint32_t nblk);
     void release();
     // off_[0..nblk] and prim_[0..nblk) must hold the batch; U/T are device
     // buffers of n bytes (U_/T_ serve as staging for host callers).
@@ -104,6 +111,7 @@ private:
     void* pinned_ = nullptr;
     uint64_t cap_tiles_ = 0;
     uint64_t cap_split_ = 0;
+    bool borrowed_ = false;        // workspace placed by bind(): not ours to free
     GraphCache graph_;             // the whole launch sequence as one graph
     std::vector<TileDesc> td_host_;
     std::vector<uint32_t> sb_host_;

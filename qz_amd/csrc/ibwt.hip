@@ -24,9 +24,11 @@
 //   4. each splitter walks its segment again and writes its symbols.
 #include "prims.h"
 #include "engine.h"
+#include "../../include/qzbsc.h"
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 namespace qz {
@@ -473,12 +475,78 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     return 0;
 }
 
+// The workspace of bind(): the same arrays reserve() allocates, but U_/T_,
+// carved from one region at 256-byte boundaries. With base == nullptr only
+// sizes the region (returns its bytes).
+static size_t carve(uint8_t* base, uint64_t n, uint32_t nblk, uint32_t** E, uint32_t** fst, uint8_t** fsym,
+                    uint32_t** fsig, uint32_t** hist, uint32_t** tmp, void** tiles, uint32_t** off, uint32_t** sbase,
+                    uint32_t** prim, uint32_t** nxt0, uint32_t** nxt1, uint32_t** len, uint32_t** d0, uint32_t** d1,
+                    uint32_t** flag, uint8_t** buf, uint32_t** resume)
+{
+    const uint64_t c = std::max<uint64_t>(n, 1024);
+    const uint32_t cb = std::max<uint32_t>(nblk, 1);
+    const uint64_t rows = c + cb, tiles_n = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
+    size_t at = 0;
+    auto take = [&](auto** p, uint64_t bytes) {
+        if (base) *p = reinterpret_cast<std::remove_pointer_t<decltype(p)>>(base + at);
+        at += (bytes + 255) & ~(uint64_t)255;
+    };
+    take(E, rows * 4);
+    take(fst, (uint64_t)cb * 256 * 4);
+    take(fsym, (uint64_t)cb * 256);
+    take(fsig, (uint64_t)cb * 4);
+    take(hist, tiles_n * 256 * 4);
+    take(tmp, (scan_tmp_words(tiles_n * 256) + 64) * 4);
+    take(tiles, tiles_n * sizeof(TileDesc));
+    take(off, (uint64_t)(cb + 1) * 4);
+    take(sbase, (uint64_t)(cb + 1) * 4);
+    take(prim, (uint64_t)cb * 4);
+    take(nxt0, nsplit * 4);
+    take(nxt1, nsplit * 4);
+    take(len, nsplit * 4);
+    take(d0, nsplit * 4);
+    take(d1, nsplit * 4);
+    take(flag, 64);
+    take(buf, seg_buf_bytes(rows, cb));
+    take(resume, nsplit * 4);
+    return at;
+}
+
+size_t IbwtEngine::bound_bytes(uint64_t n, uint32_t nblk)
+{
+    uint32_t *a, *b, *c, *d, *e, *f, *g, *h, *i, *j, *k, *l, *m, *o, *q;
+    uint8_t *y, *z;
+    void* t;
+    return carve(nullptr, n, nblk, &a, &b, &y, &c, &d, &e, &t, &f, &g, &h, &i, &j, &k, &l, &m, &o, &z, &q);
+}
+
+int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
+{
+    if (!base || bound_bytes(n, nblk) > bytes) return QZ_NOT_ENOUGH_MEMORY;
+    release();
+    carve(base, n, nblk, &E_, &fst_, &fsym_, &fsig_, &hist_, &tmp_, &tiles_, &off_, &sbase_, &prim_, &nxt0_, &nxt1_,
+          &len_, &d0_, &d1_, &flag_, &buf_, &resume_);
+    const uint64_t c = std::max<uint64_t>(n, 1024);
+    const uint32_t cb = std::max<uint32_t>(nblk, 1);
+    cap_ = c;
+    cap_blk_ = cb;
+    cap_tiles_ = c / kSortTile + cb + 1;
+    cap_split_ = (c + cb) / kMinSplit + cb + 1;
+    borrowed_ = true;
+    // host staging at its largest now, so later calls do not grow it
+    td_host_.reserve(cap_tiles_);
+    sb_host_.reserve((size_t)cb + 1);
+    return 0;
+}
+
 void IbwtEngine::release()
 {
     void* ps[] = {U_,   T_,   E_,    hist_, tmp_, tiles_,  off_, sbase_, prim_, nxt0_, nxt1_,
                   len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_};
-    for (void* p : ps)
-        if (p) (void)hipFree(p);
+    if (!borrowed_)
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+    borrowed_ = false;
     if (pinned_) (void)hipHostFree(pinned_);
     U_ = T_ = nullptr;
     E_ = nullptr;
@@ -576,9 +644,93 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         QZ_HIP(hipGetLastError());
         return 0;
     };
+    if (borrowed_) {  // bound workspace: eager launches only (capturing a graph could allocate)
+        QZ_TRY(prepare(s));
+        return launch(s);
+    }
     std::vector<uint64_t> key = {n, nblk, split, pool, (uint64_t)(uintptr_t)U, (uint64_t)(uintptr_t)T, cap_, cap_blk_};
     key.insert(key.end(), offs.begin(), offs.end());
     return graph_.run(key, s, prepare, launch);
 }
 
+// ---- the inverse BWT without allocation ---------------------------------
+//
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the workspace of batches up to kNoAllocBytes input
+// bytes in up to kNoAllocBlocks blocks lives in the file-scope array below,
+// which every device the library runs on holds until the process ends. Its
+// device address is taken once per device with hipGetSymbolAddress, and only
+// that address is handed to the engine and its kernels. Every word the
+// kernels read within a call is written earlier in the same call, so nothing
+// has to be cleared between calls. One device's array is shared by all calls
+// on that device: calls are serialised by a per-device lock, and calls that
+// overlap from another process or library instance on the same device would
+// race.
+constexpr uint64_t kNoAllocBytes = 8ull << 20;
+constexpr uint32_t kNoAllocBlocks = 64;
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+    std::mutex mu;
+    bool ready = false;
+    uint8_t* ptr = nullptr;  // this device's address of `scratch`
+    IbwtEngine eng;
+    std::vector<uint32_t> offs;
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+}  // namespace
+
+static int noalloc_setup(NoAllocDevice& D)
+{
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        fprintf(stderr, "qzbsc: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+        return QZ_GPU_ERROR;
+    }
+    D.ptr = (uint8_t*)p;
+    QZ_TRY(D.eng.bind(D.ptr, kScratchBytes, kNoAllocBytes, kNoAllocBlocks));
+    D.offs.reserve(kNoAllocBlocks + 1);
+    D.ready = true;
+    return 0;
+}
+
 }  // namespace qz
+
+using namespace qz;
+
+extern "C" {
+
+QZBSC_API int qzbsc_dev_bwt_decode_batch_noalloc(const void* d_in, void* d_out, const int* sizes, int nblocks,
+                                                 const int* primary, void* stream)
+{
+    static_assert(kNoAllocBlocks <= 256, "block count fits the bound");
+    if (!d_in || !d_out || !sizes || !primary || nblocks < 1) return QZ_BAD_PARAMETER;
+    if ((uint32_t)nblocks > kNoAllocBlocks) return QZ_NOT_ENOUGH_MEMORY;
+    uint64_t tot = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        if (sizes[b] < 1 || primary[b] < 1 || primary[b] > sizes[b]) return QZ_BAD_PARAMETER;
+        tot += (uint64_t)sizes[b];
+    }
+    if (tot > kNoAllocBytes) return QZ_NOT_ENOUGH_MEMORY;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return QZ_GPU_ERROR;
+    if (dev >= kMaxDevices) return QZ_NOT_SUPPORTED;
+    NoAllocDevice& D = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(D.mu);
+    if (!D.ready) QZ_TRY(noalloc_setup(D));
+    D.offs.resize((size_t)nblocks + 1);
+    D.offs[0] = 0;
+    for (int b = 0; b < nblocks; ++b) D.offs[b + 1] = D.offs[b] + (uint32_t)sizes[b];
+    const hipStream_t s = (hipStream_t)stream;
+    QZ_HIP(hipMemcpyAsync(D.eng.off_, D.offs.data(), D.offs.size() * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipMemcpyAsync(D.eng.prim_, primary, (size_t)nblocks * 4, hipMemcpyHostToDevice, s));
+    QZ_TRY(D.eng.decode_device((const uint8_t*)d_in, (uint8_t*)d_out, (uint32_t)tot, (uint32_t)nblocks, D.offs, s));
+    QZ_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"
