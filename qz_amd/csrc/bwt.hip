@@ -54,14 +54,26 @@ struct Grp {
 // most n/2 groups of size >= 2 exist, so n/2 entries always suffice), with
 // per-class counters. Before the next round k_bucket moves the pool into one
 // list ordered by class, each class a contiguous run, which the next round's
-// kernels read.
+// kernels read. The two counter sets (rounds alternate between them) fill
+// the pool from opposite ends: set A upwards from entry 0, set B downwards
+// from entry cap - 1. Round 0 can then emit into both at once (the second
+// set takes groups that already went through the first doubling round), and
+// as all live groups are disjoint and have >= 2 members, the two never meet.
 struct GroupLists {
     Grp* pool;
     uint32_t* top;        // pool fill counter
     uint32_t* cnt;        // kClasses per-class counters
     uint32_t cap;         // pool entries
     uint32_t* overflow;   // set if the pool would overflow (cannot happen by construction)
+    uint32_t down;        // 1: entry k of this set lives at pool[cap - 1 - k]
 };
+
+// Entry `at` of a set (counted in emission order).
+__device__ __forceinline__ void pool_put(const GroupLists& gl, uint32_t at, uint32_t start, uint32_t size)
+{
+    if (at < gl.cap) gl.pool[gl.down ? gl.cap - 1u - at : at] = Grp{start, size};
+    else *gl.overflow = 1u;
+}
 
 // Occupancy caps of the random-access kernels, as extra (unused) LDS per
 // workgroup: beyond a few workgroups per CU their scattered 4-byte loads
@@ -160,11 +172,7 @@ __device__ __forceinline__ void stage_flush_e(StageHdr& H, const uint2* E, const
         H.base = n ? atomicAdd(gl.top, n) : 0u;
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        const uint32_t at = H.base + k;
-        if (at < gl.cap) gl.pool[at] = Grp{E[k].x, E[k].y};
-        else *gl.overflow = 1u;
-    }
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) pool_put(gl, H.base + k, E[k].x, E[k].y);
 }
 
 template <int CAP>
@@ -188,10 +196,13 @@ __device__ __forceinline__ void stage_flush_t(StageT<CAP>& S_, const GroupLists&
 // next round's kernels read together. fill[] (zeroed) hands out the places
 // of each range's run within its class.
 constexpr uint32_t kBucketRange = 4096;
-__global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, uint32_t total,
-                                                const uint32_t* __restrict__ cnt, Grp* __restrict__ list,
-                                                uint32_t* __restrict__ fill)
+__global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, uint32_t total, uint32_t down,
+                                                uint32_t cap, const uint32_t* __restrict__ cnt,
+                                                Grp* __restrict__ list, uint32_t* __restrict__ fill)
 {
+    // entry i of the set (emission order); a downward set ends at pool[cap - 1]
+    const Grp* const at0 = down ? pool + cap - 1u : pool;
+    const int step = down ? -1 : 1;
     constexpr uint32_t W = 256 / QZ_WAVE, Q = kBucketRange / W;  // 1024 entries per wave
     __shared__ uint32_t co[kClasses];
     __shared__ uint32_t wb[W][kClasses];  // per wave: class counts, then write bases
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, ui
     for (int c = 0; c < kClasses; ++c) run[c] = 0;
     for (uint32_t b = q0; b < q0 + Q && b < total; b += QZ_WAVE) {  // wave-uniform
         const uint32_t i = b + lane;
-        const int cls = i < total ? grp_class(pool[i].size) : kClasses;
+        const int cls = i < total ? grp_class(at0[step * (int64_t)i].size) : kClasses;
 #pragma unroll
         for (int c = 0; c < kClasses; ++c) run[c] += (uint32_t)__popcll(__ballot(cls == c));
     }
@@ -239,7 +250,7 @@ __global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, ui
         Grp g = {0, 0};
         int cls = kClasses;
         if (i < total) {
-            g = pool[i];
+            g = at0[step * (int64_t)i];
             cls = grp_class(g.size);
         }
         uint32_t dst = 0;
@@ -701,6 +712,14 @@ __global__ __launch_bounds__(kThreads) void k_tile_lasthead(const uint32_t* __re
     if (threadIdx.x == 0) lh[blockIdx.x] = r;
 }
 
+// Whether round 0 finishes with the first doubling round fused in
+// (k_final0f): packed keys, and a 4-bit text to read the next k0 symbols
+// from. `fuse` is the host's request; the rest is decided on the device.
+__device__ __forceinline__ bool fuse_round1(const PackParams& pp, uint32_t fuse)
+{
+    return fuse != 0u && pp.rle == 0u && pp.radix <= 16u;
+}
+
 // Round-0 finish, one tile per workgroup: SA, ranks and the unsorted groups.
 // pre[t] = the last head slot before tile t (exclusive max-scan of lh). Wave
 // w takes the tile's quarter [1024w, 1024w+1024) in 16 coalesced steps and
@@ -712,8 +731,9 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                      uint32_t* __restrict__ rank, GroupLists gl,
                                                      const PackParams* __restrict__ ppd, uint32_t* __restrict__ dep,
-                                                     BlockMap bm, const uint32_t* __restrict__ brun)
+                                                     BlockMap bm, const uint32_t* __restrict__ brun, uint32_t fuse)
 {
+    if (fuse_round1(*ppd, fuse)) return;  // uniform: k_final0f finishes this batch
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);  // 1024 slots per wave
     __shared__ Stage S;
     __shared__ uint32_t qlast[kThreads / QZ_WAVE];
@@ -760,6 +780,269 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
         }
     }
     stage_flush(S, gl);
+}
+
+// ---- round 0 finish with the first doubling round fused in ----------------
+// With packed keys, the first doubling round sorts each round-0 group by
+// key2 = rank[s + k0] + 1. The packed code of the k0 symbols at s + k0
+// orders those suffixes exactly as their round-0 ranks do (equal codes <=>
+// the same round-0 group), so for a group that lies inside its radix tile
+// and has at most 64 members, the finishing workgroup sorts it right away
+// by that code, read from the 4-bit text: the group's members get the SA
+// slots and ranks of the first doubling round directly, and its subgroups
+// (2 k0 symbols deep) go to set B, which the second doubling round reads.
+// Every other group of size >= 2 goes to set A as in k_final0; the first
+// doubling round then sorts set A alone (its keys may point into groups
+// already refined here, which only orders them further: ranks stay the
+// start slots of groups in sorted order) and appends to set B.
+//
+// Phases: (1) group heads as in k_final0; slots of singletons and of groups
+// that start before the tile are final at once, the others wait in LDS; the
+// ends of in-tile groups of <= 64 members list them as fused, the ends of
+// other groups stage them for set A. (2) every waiting slot of an unfused
+// group gets its SA entry and rank; every slot of a fused group its code.
+// (3) waves sort the fused groups by code, by size class as k_sort_small
+// does, and stage their subgroups for set B.
+constexpr uint32_t kFuseMax = 64;
+constexpr uint32_t kFuseThreads = 512;
+constexpr uint32_t kWaitNone = 0xffffffffu;  // slot already final
+
+// The code of the k0 symbols at positions p, p+1, ... (radix r, code 0 at
+// positions >= lim, lim > p), from the 4-bit text nib (code of position q in
+// bits 4(q&7).. of word q>>3). Reads only words holding positions < lim.
+__device__ __forceinline__ uint32_t nib_code(const uint32_t* __restrict__ nib, uint32_t p, uint32_t lim, uint32_t k0,
+                                             uint32_t r)
+{
+    const uint32_t w0 = p >> 3, wl = (lim - 1u) >> 3;
+    uint32_t wv[5];  // k0 <= 32 codes from offset <= 7: at most 5 words
+#pragma unroll
+    for (uint32_t k = 0; k < 5; ++k) wv[k] = (w0 + k <= wl) ? nib[w0 + k] : 0u;
+    uint32_t key = 0;
+    for (uint32_t i = 0; i < k0; ++i) {
+        const uint32_t q = p + i;
+        const uint32_t k = (q >> 3) - w0;
+        const uint32_t w = k == 0 ? wv[0] : k == 1 ? wv[1] : k == 2 ? wv[2] : k == 3 ? wv[3] : wv[4];
+        key = key * r + ((q < lim) ? (w >> (4u * (q & 7u))) & 15u : 0u);
+    }
+    return key;
+}
+
+// Two-ended stage: set A entries from e[0] up, set B entries from e[kStage-1]
+// down (a tile's groups and subgroups are disjoint, so <= kStage in all).
+struct Stage2 {
+    uint32_t cnt[2][kClasses];
+    uint32_t n[2];
+    uint32_t base[2];
+    uint2 e[kStage];
+};
+
+__device__ __forceinline__ void stage2_push(Stage2& S, uint32_t set, uint32_t start, uint32_t size)
+{
+    atomicAdd(&S.cnt[set][grp_class(size)], 1u);
+    const uint32_t k = atomicAdd(&S.n[set], 1u);
+    if (k < (uint32_t)kStage) S.e[set ? (uint32_t)kStage - 1u - k : k] = make_uint2(start, size);
+}
+
+__device__ __forceinline__ void stage2_flush(Stage2& S, const GroupLists& glA, const GroupLists& glB)
+{
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t < (uint32_t)kClasses) {
+        if (S.cnt[0][t]) atomicAdd(&glA.cnt[t], S.cnt[0][t]);
+        if (S.cnt[1][t]) atomicAdd(&glB.cnt[t], S.cnt[1][t]);
+    }
+    if (t == 0) {
+        if (S.n[0] + S.n[1] > (uint32_t)kStage) *glA.overflow = 1u;
+        S.base[0] = S.n[0] ? atomicAdd(glA.top, S.n[0]) : 0u;
+        S.base[1] = S.n[1] ? atomicAdd(glB.top, S.n[1]) : 0u;
+    }
+    __syncthreads();
+    const uint32_t na = min(S.n[0], (uint32_t)kStage), nb = min(S.n[1], (uint32_t)kStage);
+    for (uint32_t k = t; k < na; k += blockDim.x) pool_put(glA, S.base[0] + k, S.e[k].x, S.e[k].y);
+    for (uint32_t k = t; k < nb; k += blockDim.x) {
+        const uint2 e = S.e[(uint32_t)kStage - 1u - k];
+        pool_put(glB, S.base[1] + k, e.x, e.y);
+    }
+}
+
+__global__ __launch_bounds__(kFuseThreads) void k_final0f(const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals,
+                                                          const TileDesc* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
+                                                          uint32_t* __restrict__ rank, GroupLists glA, GroupLists glB,
+                                                          const PackParams* __restrict__ ppd, BlockMap bm,
+                                                          const uint32_t* __restrict__ nib, uint32_t fuse)
+{
+    if (!fuse_round1(*ppd, fuse)) return;  // uniform: k_final0 finishes this batch
+    constexpr uint32_t T = kFuseThreads, W = T / QZ_WAVE;
+    constexpr uint32_t Q = kSortTile / W;      // slots per wave in phase 1
+    constexpr uint32_t PER = kSortTile / T;    // slots per thread in phase 2
+    __shared__ uint32_t sv[kSortTile];         // waiting slots: the suffix
+    __shared__ uint32_t sx[kSortTile];         // waiting slots: tile-local group head, then (fused) the code
+    __shared__ uint32_t fl[kSortTile / 32];    // fused groups, by tile-local head
+    __shared__ uint32_t fe[kStage];            // fused groups as (head << 8) | size, in any order
+    __shared__ uint32_t le[kStage];            // the same by size class
+    __shared__ uint32_t fcnt[kSmallClasses], fcur[kSmallClasses], fbase[kSmallClasses + 1], tb[kSmallClasses + 1];
+    __shared__ uint32_t nfe;
+    __shared__ uint32_t qlast[W];
+    __shared__ Stage2 S;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / QZ_WAVE;
+    if (tid < (uint32_t)kClasses) S.cnt[0][tid] = S.cnt[1][tid] = 0u;
+    if (tid < 2u) S.n[tid] = 0u;
+    if (tid < (uint32_t)kSmallClasses) fcnt[tid] = 0u;
+    if (tid == 0) nfe = 0u;
+    for (uint32_t k = tid; k < kSortTile / 32; k += T) fl[k] = 0u;
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t qb = w * Q;
+    uint32_t m = 0;
+    for (uint32_t i = qb + lane; i < qb + Q && i < td.len; i += QZ_WAVE)
+        if (slot_head(keys, td, i)) m = td.start + i;
+    for (int d = 1; d < QZ_WAVE; d <<= 1) {
+        const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
+        m = o > m ? o : m;
+    }
+    if (lane == 0) qlast[w] = m;
+    __syncthreads();
+    uint32_t carry = pre[blockIdx.x];
+    for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
+    const bool last_tile = td.tlocal + 1 == td.ntiles;
+    // phase 1
+    for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
+        const uint32_t i = b + lane;
+        const bool valid = i < td.len;
+        const uint32_t j = td.start + i;
+        const uint32_t k = valid ? keys[j] : 0u;
+        const bool head = valid && slot_head(keys, td, i);
+        uint32_t g = head ? j : 0u;
+        for (int d = 1; d < QZ_WAVE; d <<= 1) {
+            const uint32_t o = __shfl_up(g, d, QZ_WAVE);
+            if (lane >= (uint32_t)d && o > g) g = o;
+        }
+        if (carry > g) g = carry;
+        carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
+        if (valid) {
+            const uint32_t s = vals[j];
+            const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
+            if (g < td.start || (head && end)) {  // final now
+                sa[j] = s;
+                rank[s] = g;
+                sx[i] = kWaitNone;
+            } else {
+                sv[i] = s;
+                sx[i] = g - td.start;
+            }
+            if (end && j > g) {
+                const uint32_t size = j - g + 1;
+                if (g >= td.start && size <= kFuseMax) {
+                    const uint32_t gl = g - td.start;
+                    atomicOr(&fl[gl >> 5], 1u << (gl & 31u));
+                    atomicAdd(&fcnt[31 - __builtin_clz(size - 1)], 1u);
+                    const uint32_t at = atomicAdd(&nfe, 1u);
+                    if (at < (uint32_t)kStage) fe[at] = (gl << 8) | size;
+                } else {
+                    stage2_push(S, 0, g, size);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t o = 0, t = 0;
+        for (int c = 0; c < kSmallClasses; ++c) {
+            fbase[c] = o;
+            fcur[c] = o;
+            tb[c] = t;
+            o += fcnt[c];
+            const uint32_t per = (uint32_t)QZ_WAVE >> (c + 1);  // groups per wave task
+            t += (fcnt[c] + per - 1) / per;
+        }
+        fbase[kSmallClasses] = o;
+        tb[kSmallClasses] = t;
+    }
+    // phase 2: waiting slots (fused groups' codes read PER slots at a time)
+    {
+        const uint32_t r = ppd->radix, k0 = ppd->k0;
+        const uint32_t bend = bm.off[td.block + 1];
+        uint32_t pos[PER];
+        bool want[PER];
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q) {
+            const uint32_t i = tid + q * T;
+            want[q] = false;
+            pos[q] = 0;
+            if (i < td.len) {
+                const uint32_t x = sx[i];
+                if (x != kWaitNone) {
+                    if ((fl[x >> 5] >> (x & 31u)) & 1u) {
+                        want[q] = true;
+                        pos[q] = sv[i] + k0;
+                    } else {
+                        const uint32_t s = sv[i];
+                        sa[td.start + i] = s;
+                        rank[s] = td.start + x;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q)
+            if (want[q]) sx[tid + q * T] = pos[q] < bend ? nib_code(nib, pos[q], bend, k0, r) : 0u;
+    }
+    __syncthreads();  // class bases (thread 0) and codes are in place
+    for (uint32_t k = tid; k < min(nfe, (uint32_t)kStage); k += T) {
+        const uint32_t e = fe[k];
+        const uint32_t at = atomicAdd(&fcur[31 - __builtin_clz((e & 255u) - 1)], 1u);
+        le[at] = e;
+    }
+    __syncthreads();
+    // phase 3: fused groups by class, 64 / seg groups per wave task
+    for (uint32_t t = w; t < tb[kSmallClasses]; t += W) {  // wave-uniform
+        uint32_t c = 0;
+        while (c + 1 < (uint32_t)kSmallClasses && t >= tb[c + 1]) ++c;
+        const uint32_t seg = 2u << c;
+        const uint32_t gi = (t - tb[c]) * ((uint32_t)QZ_WAVE / seg) + lane / seg;
+        const uint32_t e = lane & (seg - 1u);
+        uint32_t gl = 0, size = 0;
+        if (gi < fcnt[c]) {
+            const uint32_t ent = le[fbase[c] + gi];
+            gl = ent >> 8;
+            size = ent & 255u;
+        }
+        const bool valid = e < size;
+        // sort by (code, suffix): pad lanes sort last even past a code of 2^32 - 1
+        uint32_t key = valid ? sx[gl + e] : 0xffffffffu;
+        uint32_t val = valid ? sv[gl + e] : 0xffffffffu;
+        for (uint32_t k = 2; k <= seg; k <<= 1) {
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                const uint32_t ok = __shfl_xor(key, (int)jj, QZ_WAVE);
+                const uint32_t ov = __shfl_xor(val, (int)jj, QZ_WAVE);
+                const bool up = (k == seg) || ((lane & k) == 0);
+                const bool lower = (lane & jj) == 0;
+                const bool less = ok < key || (ok == key && ov < val);
+                const bool more = ok > key || (ok == key && ov > val);
+                if ((lower == up) ? less : more) {
+                    key = ok;
+                    val = ov;
+                }
+            }
+        }
+        const uint32_t kprev = __shfl_up(key, 1, QZ_WAVE);
+        const uint32_t knext = __shfl_down(key, 1, QZ_WAVE);
+        const bool head = (e == 0) || kprev != key;
+        uint32_t hp = head ? e : 0u;
+        for (uint32_t d = 1; d < seg; d <<= 1) {
+            const uint32_t o = __shfl_up(hp, d, QZ_WAVE);
+            if (e >= d && o > hp) hp = o;
+        }
+        if (valid) {
+            const uint32_t g = td.start + gl;
+            sa[g + e] = val;
+            rank[val] = g + hp;
+            const bool end = (e + 1 == size) || knext != key;
+            if (end && e > hp) stage2_push(S, 1, g + hp, e - hp + 1);
+        }
+    }
+    stage2_flush(S, glA, glB);
 }
 
 // ---- small groups (<= 64), one wave holds 64 / 2^(c+1) groups of class c --
@@ -1428,6 +1711,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&rank_, c * 4));
     QZ_HIP(hipMalloc(&bufA_, tile_cap * 4));  // last head per radix tile
     QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(c, seg_words) * 4));
+    bufB_words_ = std::max<size_t>(c, seg_words);
     QZ_HIP(hipMalloc(&keys0_, c * 4));
     QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
@@ -1474,9 +1758,10 @@ void BwtEngine::release()
     pinned_ = nullptr;
     cap_ = 0;
     cap_blk_ = 0;
+    bufB_words_ = 0;
 }
 
-static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint64_t cap)
+static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint64_t cap, bool down)
 {
     GroupLists gl;
     gl.pool = (Grp*)pool;
@@ -1484,6 +1769,7 @@ static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint
     gl.top = cnt + kTopOff;
     gl.cap = (uint32_t)pool_cap(cap);
     gl.overflow = overflow;
+    gl.down = down ? 1u : 0u;
     return gl;
 }
 
@@ -1499,8 +1785,19 @@ static int rle_force()
 // replayed as a graph: clear the counters, count symbols, derive the packing
 // on the device, build the packed prefix keys, sort them (32 key bits),
 // then mark group heads and emit SA, ranks and the unsorted groups.
+// Word offset in bufB of the fused round's 4-bit text: past the round-0
+// histograms, 256-byte aligned.
+static size_t fuse_nib_off(uint32_t hist_words) { return ((size_t)hist_words + 63) & ~(size_t)63; }
+
+// QZBSC_FUSE=0 keeps the first doubling round out of round 0's finish.
+static bool fuse_enabled()
+{
+    const char* e = getenv("QZBSC_FUSE");
+    return !(e && e[0] == '0');
+}
+
 int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
-                              uint32_t hist_words, bool timed, hipStream_t s)
+                              uint32_t hist_words, bool timed, bool fuse, hipStream_t s)
 {
     const BlockMap bmap_{off_, btab_, nblk};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
@@ -1520,6 +1817,14 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
                         (const PackParams*)ppd_, (const uint32_t*)brun_, keys0_);
     QZ_HIP(hipGetLastError());
+    // the 4-bit text for the fused first doubling round, in bufB past the
+    // radix histograms (k_pack_nib leaves it alone for larger alphabets)
+    uint32_t* nib = bufB_ + fuse_nib_off(hist_words);
+    if (fuse) {
+        hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
+                           (const PackParams*)ppd_, nib);
+        QZ_HIP(hipGetLastError());
+    }
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
     QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
@@ -1532,11 +1837,19 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
-    const GroupLists gl = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_);
+    const GroupLists glA = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_, false);
+    const GroupLists glB = make_lists(groups_[0], cnt_ + kCntB, cnt_ + kOverflow, cap_, true);
+    // one of the two finishes the batch (the other returns at once): the
+    // choice depends on the key mode and alphabet, decided on the device
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
-                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, gl, (const PackParams*)ppd_, dep_[0], bmap_,
-                       (const uint32_t*)brun_);
+                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, glA, (const PackParams*)ppd_, dep_[0], bmap_,
+                       (const uint32_t*)brun_, fuse ? 1u : 0u);
     QZ_HIP(hipGetLastError());
+    if (fuse) {
+        hipLaunchKernelGGL(k_final0f, dim3(ntiles), dim3(kFuseThreads), 0, s, keys, vals, (const TileDesc*)tiles0_,
+                           bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)nib, 1u);
+        QZ_HIP(hipGetLastError());
+    }
     return 0;
 }
 
@@ -1550,8 +1863,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
     uint32_t* overflow = cnt_ + kOverflow;
     // emission targets: the pool with counter set A or B, alternating
-    const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_),
-                               make_lists(groups_[0], cnt_ + kCntB, overflow, cap_)};
+    const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_, false),
+                               make_lists(groups_[0], cnt_ + kCntB, overflow, cap_, true)};
     Grp* const list = (Grp*)groups_[1];
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
@@ -1559,6 +1872,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     const bool timed0 = tune_wants(0, n);  // the tuner is still timing: run eagerly with events
     uint32_t ntiles = 0;
     const uint32_t hist_words = count_tiles(offs_host_.data(), nblk, &ntiles, 1u << bits0);
+    // the fused first doubling round needs room for the 4-bit text in bufB
+    const bool fuse = fuse_enabled() && fuse_nib_off(hist_words) + (size_t)n / 8 + 2 <= bufB_words_;
     auto prepare0 = [&](hipStream_t st) -> int {
         uint32_t nt = 0;
         build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &nt, 1u << bits0);
@@ -1566,7 +1881,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         return 0;
     };
     auto launch0 = [&](hipStream_t st) -> int {
-        return enqueue_round0(text, n, nblk, bits0, ntiles, hist_words, timed0, st);
+        return enqueue_round0(text, n, nblk, bits0, ntiles, hist_words, timed0, fuse, st);
     };
     if (timed0) {
         g0_.drop();
@@ -1574,7 +1889,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         QZ_TRY(launch0(s));
     } else {
         std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_,
-                                     (uint64_t)(rle_force() + 1)};
+                                     (uint64_t)(rle_force() + 1), (uint64_t)fuse};
         key.insert(key.end(), offs_host_.begin(), offs_host_.end());
         QZ_TRY(g0_.run(key, s, prepare0, launch0));
     }
@@ -1594,7 +1909,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     for (;;) {
         // one copy brings back the current set's class counts and the
         // overflow word, which lies between the two sets in cnt_
-        const int lo = cur ? kOverflow : 0, hi = cur ? kCntB + kClasses : kOverflow + 1;
+        // (after round 0 both sets: the fused first round may have filled B)
+        const int lo = cur ? kOverflow : 0, hi = (cur || rounds_ == 1) ? kCntB + kClasses : kOverflow + 1;
         QZ_HIP(hipMemcpyAsync(hc, cnt_ + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToHost, s));
         if (rounds_ == 1)
             QZ_HIP(hipMemcpyAsync(hc + 64, &((const PackParams*)ppd_)->k0, 8, hipMemcpyDeviceToHost, s));  // k0, rle
@@ -1624,15 +1940,27 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             if (hipEventElapsedTime(&ms, tev_[2], tev_[3]) == hipSuccess) tune_report(1, large_bits, large_n, ms);
             large_timed = false;
         }
+        uint64_t fused = 0;  // groups round 0 already took through the first doubling round (set B)
+        if (rounds_ == 1)
+            for (int k = 0; k < kClasses; ++k) fused += ((volatile uint32_t*)hc)[kCntB + k];
         if (trace_) {
             if (rle)
                 fprintf(stderr, "qzbsc: round %d (per-group depths) groups", rounds_);
             else
                 fprintf(stderr, "qzbsc: round %d h=%u groups", rounds_, h);
             for (int k = 0; k < kClasses; ++k) fprintf(stderr, " %u", counts[k]);
+            if (fused) fprintf(stderr, " (+%llu for round 2)", (unsigned long long)fused);
             fprintf(stderr, "\n");
         }
-        if (total == 0) break;
+        if (total == 0) {
+            if (!fused) break;
+            // every group round 0 left was fused: the first doubling round
+            // has nothing to sort, and set B holds the second round's groups
+            ++rounds_;
+            h = (h > 0x40000000u) ? 0x80000000u : h * 2;
+            cur ^= 1;
+            continue;
+        }
         if (rounds_ > 40) {
             fprintf(stderr, "qzbsc: suffix sort did not converge\n");
             return QZ_GPU_ERROR;
@@ -1649,12 +1977,13 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             }
             cl.cnt = C.cnt;
             QZ_HIP(hipMemsetAsync(cnt_ + kFill, 0, kClasses * 4, s));
-            hipLaunchKernelGGL(k_bucket, dim3(qz_div_up(o, kBucketRange)), dim3(256), 0, s, C.pool, o, C.cnt, list,
-                               cnt_ + kFill);
+            hipLaunchKernelGGL(k_bucket, dim3(qz_div_up(o, kBucketRange)), dim3(256), 0, s, C.pool, o, C.down, C.cap,
+                               C.cnt, list, cnt_ + kFill);
             QZ_HIP(hipGetLastError());
         }
         // this round emits into the pool again, with the other counter set
-        QZ_HIP(hipMemsetAsync(N.cnt, 0, (kClasses + 1) * 4, s));
+        // (after round 0 that set may already hold fused groups: append)
+        if (rounds_ != 1) QZ_HIP(hipMemsetAsync(N.cnt, 0, (kClasses + 1) * 4, s));
 
         SmallPlan plan;
         plan.wave_base[0] = 0;

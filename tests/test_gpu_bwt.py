@@ -122,6 +122,44 @@ def test_round0_key_modes_give_the_same_bwt(gpu, monkeypatch, mode):
         _check([blk], gpu)
 
 
+def _fuse_batches():
+    """Batches whose whole alphabet fits the 4-bit text (sigma + 1 <= 16), the
+    case where round 0 also does the first doubling round for small groups
+    inside one radix tile. The radix (sigma + 1) ** k0 reaches 2 ** 32 for
+    sigma 1, 3 and 15, so the largest code is 0xffffffff there."""
+    rng = np.random.default_rng(88)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    unit = rng.integers(0, 4, 50_000)
+    dna = [synth.sequence_stream(60_000, 9_000, 150, 36, 0.002, 0.0001),
+           acgt[rng.integers(0, 4, 5_000)], np.frombuffer(b"ACGT" * 20, np.uint8),
+           acgt[np.tile(unit, 40)],                 # 40 copies: fused groups of 40, some across tiles
+           acgt[np.tile(unit[:10_000], 64)],        # 64 copies: the largest fused size
+           acgt[np.tile(unit[:7_000], 65)],         # 65 copies: just past it
+           np.frombuffer(b"A", np.uint8), np.frombuffer(b"CA", np.uint8)]
+    acg = np.frombuffer(b"ACG", np.uint8)
+    three = [acg[rng.integers(0, 3, 200_000)], acg[np.tile(rng.integers(0, 3, 3_000), 50)], acg[rng.integers(0, 3, 17)]]
+    fifteen = np.frombuffer(b"ABCDEFGHIJKLMNO", np.uint8)
+    hexa = [fifteen[rng.integers(0, 15, 300_000)], fifteen[np.tile(rng.integers(0, 15, 2_000), 30)]]
+    ones = [np.full(100_000, 65, np.uint8), np.full(5_000, 65, np.uint8), np.full(70, 65, np.uint8)]
+    return {"dna": dna, "sigma3": three, "sigma15": hexa, "sigma1": ones}
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("kind", ["dna", "sigma3", "sigma15", "sigma1"])
+def test_fused_first_round_matches_oracle(gpu, monkeypatch, fuse, kind):
+    """Round 0's finish with the first doubling round fused in (and, for
+    comparison, without): the oracle's BWT, primary and samples on every
+    block, batched and alone. A one-symbol block would take run-length keys,
+    which never fuse, so packed keys are forced for it."""
+    monkeypatch.setenv("QZBSC_FUSE", fuse)
+    if kind == "sigma1":
+        monkeypatch.setenv("QZBSC_RLE", "0")
+    blocks = _fuse_batches()[kind]
+    print("fuse", fuse, kind, [b.size for b in blocks], flush=True)
+    _check(blocks, gpu)
+    _check([blocks[0]], gpu)
+
+
 def _hip_runtime():
     """The HIP runtime that the library has already loaded in this process."""
     import ctypes as C
