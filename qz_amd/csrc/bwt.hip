@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace qz {
@@ -804,7 +805,6 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
 // (3) waves sort the fused groups by code, by size class as k_sort_small
 // does, and stage their subgroups for set B.
 constexpr uint32_t kFuseMax = 64;
-constexpr uint32_t kFuseThreads = 512;
 constexpr uint32_t kWaitNone = 0xffffffffu;  // slot already final
 
 // The code of the k0 symbols at positions p, p+1, ... (radix r, code 0 at
@@ -865,19 +865,20 @@ __device__ __forceinline__ void stage2_flush(Stage2& S, const GroupLists& glA, c
     }
 }
 
-__global__ __launch_bounds__(kFuseThreads) void k_final0f(const uint32_t* __restrict__ keys,
-                                                          const uint32_t* __restrict__ vals,
-                                                          const TileDesc* __restrict__ tiles,
-                                                          const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
-                                                          uint32_t* __restrict__ rank, GroupLists glA, GroupLists glB,
-                                                          const PackParams* __restrict__ ppd, BlockMap bm,
-                                                          const uint32_t* __restrict__ nib, uint32_t fuse)
+// T threads per tile; SVL: waiting slots keep their suffix in LDS (else it
+// is read again from vals, trading 16 KB of LDS for a coalesced re-read).
+template <uint32_t T, bool SVL>
+__global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                               const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ pre,
+                                               uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
+                                               GroupLists glA, GroupLists glB, const PackParams* __restrict__ ppd,
+                                               BlockMap bm, const uint32_t* __restrict__ nib, uint32_t fuse)
 {
     if (!fuse_round1(*ppd, fuse)) return;  // uniform: k_final0 finishes this batch
-    constexpr uint32_t T = kFuseThreads, W = T / QZ_WAVE;
+    constexpr uint32_t W = T / QZ_WAVE;
     constexpr uint32_t Q = kSortTile / W;      // slots per wave in phase 1
     constexpr uint32_t PER = kSortTile / T;    // slots per thread in phase 2
-    __shared__ uint32_t sv[kSortTile];         // waiting slots: the suffix
+    __shared__ uint32_t sv[SVL ? kSortTile : 1];  // waiting slots: the suffix
     __shared__ uint32_t sx[kSortTile];         // waiting slots: tile-local group head, then (fused) the code
     __shared__ uint32_t fl[kSortTile / 32];    // fused groups, by tile-local head
     __shared__ uint32_t fe[kStage];            // fused groups as (head << 8) | size, in any order
@@ -928,7 +929,7 @@ __global__ __launch_bounds__(kFuseThreads) void k_final0f(const uint32_t* __rest
                 rank[s] = g;
                 sx[i] = kWaitNone;
             } else {
-                sv[i] = s;
+                if (SVL) sv[i] = s;
                 sx[i] = g - td.start;
             }
             if (end && j > g) {
@@ -973,11 +974,11 @@ __global__ __launch_bounds__(kFuseThreads) void k_final0f(const uint32_t* __rest
             if (i < td.len) {
                 const uint32_t x = sx[i];
                 if (x != kWaitNone) {
+                    const uint32_t s = SVL ? sv[i] : vals[td.start + i];
                     if ((fl[x >> 5] >> (x & 31u)) & 1u) {
                         want[q] = true;
-                        pos[q] = sv[i] + k0;
+                        pos[q] = s + k0;
                     } else {
-                        const uint32_t s = sv[i];
                         sa[td.start + i] = s;
                         rank[s] = td.start + x;
                     }
@@ -1011,7 +1012,7 @@ __global__ __launch_bounds__(kFuseThreads) void k_final0f(const uint32_t* __rest
         const bool valid = e < size;
         // sort by (code, suffix): pad lanes sort last even past a code of 2^32 - 1
         uint32_t key = valid ? sx[gl + e] : 0xffffffffu;
-        uint32_t val = valid ? sv[gl + e] : 0xffffffffu;
+        uint32_t val = valid ? (SVL ? sv[gl + e] : vals[td.start + gl + e]) : 0xffffffffu;
         for (uint32_t k = 2; k <= seg; k <<= 1) {
             for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
                 const uint32_t ok = __shfl_xor(key, (int)jj, QZ_WAVE);
@@ -1796,8 +1797,19 @@ static bool fuse_enabled()
     return !(e && e[0] == '0');
 }
 
+// Launch shape of the fused finish, QZBSC_FUSE_SHAPE=<threads>[s]: 256, 512
+// or 1024 threads per tile, "s" for suffixes kept in LDS. Returns the code
+// enqueue_round0 takes (0 = the default).
+static int fuse_shape()
+{
+    const char* e = getenv("QZBSC_FUSE_SHAPE");
+    if (!e) return 0;
+    const std::string v = e;
+    return v == "256" ? 1 : v == "256s" ? 2 : v == "512" ? 3 : v == "1024" ? 4 : v == "1024s" ? 5 : 0;
+}
+
 int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
-                              uint32_t hist_words, bool timed, bool fuse, hipStream_t s)
+                              uint32_t hist_words, bool timed, bool fuse, int fshape, hipStream_t s)
 {
     const BlockMap bmap_{off_, btab_, nblk};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
@@ -1846,8 +1858,18 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
                        (const uint32_t*)brun_, fuse ? 1u : 0u);
     QZ_HIP(hipGetLastError());
     if (fuse) {
-        hipLaunchKernelGGL(k_final0f, dim3(ntiles), dim3(kFuseThreads), 0, s, keys, vals, (const TileDesc*)tiles0_,
-                           bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)nib, 1u);
+#define QZ_FINAL0F(T, SVL)                                                                                      \
+    hipLaunchKernelGGL((k_final0f<T, SVL>), dim3(ntiles), dim3(T), 0, s, keys, vals, (const TileDesc*)tiles0_, \
+                       bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)nib, 1u)
+        switch (fshape) {
+            case 1: QZ_FINAL0F(256, false); break;
+            case 2: QZ_FINAL0F(256, true); break;
+            case 3: QZ_FINAL0F(512, false); break;
+            case 4: QZ_FINAL0F(1024, false); break;
+            case 5: QZ_FINAL0F(1024, true); break;
+            default: QZ_FINAL0F(512, true); break;
+        }
+#undef QZ_FINAL0F
         QZ_HIP(hipGetLastError());
     }
     return 0;
@@ -1874,6 +1896,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     const uint32_t hist_words = count_tiles(offs_host_.data(), nblk, &ntiles, 1u << bits0);
     // the fused first doubling round needs room for the 4-bit text in bufB
     const bool fuse = fuse_enabled() && fuse_nib_off(hist_words) + (size_t)n / 8 + 2 <= bufB_words_;
+    const int fshape = fuse_shape();
     auto prepare0 = [&](hipStream_t st) -> int {
         uint32_t nt = 0;
         build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &nt, 1u << bits0);
@@ -1881,7 +1904,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         return 0;
     };
     auto launch0 = [&](hipStream_t st) -> int {
-        return enqueue_round0(text, n, nblk, bits0, ntiles, hist_words, timed0, fuse, st);
+        return enqueue_round0(text, n, nblk, bits0, ntiles, hist_words, timed0, fuse, fshape, st);
     };
     if (timed0) {
         g0_.drop();
@@ -1889,7 +1912,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         QZ_TRY(launch0(s));
     } else {
         std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_,
-                                     (uint64_t)(rle_force() + 1), (uint64_t)fuse};
+                                     (uint64_t)(rle_force() + 1), (uint64_t)fuse, (uint64_t)fshape};
         key.insert(key.end(), offs_host_.begin(), offs_host_.end());
         QZ_TRY(g0_.run(key, s, prepare0, launch0));
     }
