@@ -714,11 +714,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_lasthead(const uint32_t* __re
 }
 
 // Whether round 0 finishes with the first doubling round fused in
-// (k_final0f): packed keys, and a 4-bit text to read the next k0 symbols
-// from. `fuse` is the host's request; the rest is decided on the device.
+// (k_final0f): the host asks for it (fuse != 0) and the batch has packed
+// keys, a choice made on the device.
 __device__ __forceinline__ bool fuse_round1(const PackParams& pp, uint32_t fuse)
 {
-    return fuse != 0u && pp.rle == 0u && pp.radix <= 16u;
+    return fuse != 0u && pp.rle == 0u;
 }
 
 // Round-0 finish, one tile per workgroup: SA, ranks and the unsorted groups.
@@ -785,13 +785,15 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
 
 // ---- round 0 finish with the first doubling round fused in ----------------
 // With packed keys, the first doubling round sorts each round-0 group by
-// key2 = rank[s + k0] + 1. The packed code of the k0 symbols at s + k0
-// orders those suffixes exactly as their round-0 ranks do (equal codes <=>
-// the same round-0 group), so for a group that lies inside its radix tile
-// and has at most 64 members, the finishing workgroup sorts it right away
-// by that code, read from the 4-bit text: the group's members get the SA
-// slots and ranks of the first doubling round directly, and its subgroups
-// (2 k0 symbols deep) go to set B, which the second doubling round reads.
+// key2 = rank[s + k0] + 1. The round-0 key of position s + k0 (the packed
+// code of its k0 symbols, 0 past the block end) orders those suffixes
+// exactly as their round-0 ranks do (equal codes <=> the same round-0
+// group), so for a group that lies inside its radix tile and has at most 64
+// members, the finishing workgroup sorts it right away by that key, read
+// from the positional round-0 keys, which the radix sort leaves intact:
+// the group's members get the SA slots and ranks of the first doubling
+// round directly, and its subgroups (2 k0 symbols deep) go to set B, which
+// the second doubling round reads.
 // Every other group of size >= 2 goes to set A as in k_final0; the first
 // doubling round then sorts set A alone (its keys may point into groups
 // already refined here, which only orders them further: ranks stay the
@@ -806,26 +808,6 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
 // does, and stage their subgroups for set B.
 constexpr uint32_t kFuseMax = 64;
 constexpr uint32_t kWaitNone = 0xffffffffu;  // slot already final
-
-// The code of the k0 symbols at positions p, p+1, ... (radix r, code 0 at
-// positions >= lim, lim > p), from the 4-bit text nib (code of position q in
-// bits 4(q&7).. of word q>>3). Reads only words holding positions < lim.
-__device__ __forceinline__ uint32_t nib_code(const uint32_t* __restrict__ nib, uint32_t p, uint32_t lim, uint32_t k0,
-                                             uint32_t r)
-{
-    const uint32_t w0 = p >> 3, wl = (lim - 1u) >> 3;
-    uint32_t wv[5];  // k0 <= 32 codes from offset <= 7: at most 5 words
-#pragma unroll
-    for (uint32_t k = 0; k < 5; ++k) wv[k] = (w0 + k <= wl) ? nib[w0 + k] : 0u;
-    uint32_t key = 0;
-    for (uint32_t i = 0; i < k0; ++i) {
-        const uint32_t q = p + i;
-        const uint32_t k = (q >> 3) - w0;
-        const uint32_t w = k == 0 ? wv[0] : k == 1 ? wv[1] : k == 2 ? wv[2] : k == 3 ? wv[3] : wv[4];
-        key = key * r + ((q < lim) ? (w >> (4u * (q & 7u))) & 15u : 0u);
-    }
-    return key;
-}
 
 // Two-ended stage: set A entries from e[0] up, set B entries from e[kStage-1]
 // down (a tile's groups and subgroups are disjoint, so <= kStage in all).
@@ -867,12 +849,13 @@ __device__ __forceinline__ void stage2_flush(Stage2& S, const GroupLists& glA, c
 
 // T threads per tile; SVL: waiting slots keep their suffix in LDS (else it
 // is read again from vals, trading 16 KB of LDS for a coalesced re-read).
+// kpos: the round-0 key of every position.
 template <uint32_t T, bool SVL>
 __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ pre,
                                                uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
                                                GroupLists glA, GroupLists glB, const PackParams* __restrict__ ppd,
-                                               BlockMap bm, const uint32_t* __restrict__ nib, uint32_t fuse)
+                                               BlockMap bm, const uint32_t* __restrict__ kpos, uint32_t fuse)
 {
     if (!fuse_round1(*ppd, fuse)) return;  // uniform: k_final0 finishes this batch
     constexpr uint32_t W = T / QZ_WAVE;
@@ -960,9 +943,9 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
         fbase[kSmallClasses] = o;
         tb[kSmallClasses] = t;
     }
-    // phase 2: waiting slots (fused groups' codes read PER slots at a time)
+    // phase 2: waiting slots (fused groups' keys gathered PER slots at a time)
     {
-        const uint32_t r = ppd->radix, k0 = ppd->k0;
+        const uint32_t k0 = ppd->k0;
         const uint32_t bend = bm.off[td.block + 1];
         uint32_t pos[PER];
         bool want[PER];
@@ -987,7 +970,7 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
         }
 #pragma unroll
         for (uint32_t q = 0; q < PER; ++q)
-            if (want[q]) sx[tid + q * T] = pos[q] < bend ? nib_code(nib, pos[q], bend, k0, r) : 0u;
+            if (want[q]) sx[tid + q * T] = pos[q] < bend ? kpos[pos[q]] : 0u;
     }
     __syncthreads();  // class bases (thread 0) and codes are in place
     for (uint32_t k = tid; k < min(nfe, (uint32_t)kStage); k += T) {
@@ -1010,7 +993,7 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
             size = ent & 255u;
         }
         const bool valid = e < size;
-        // sort by (code, suffix): pad lanes sort last even past a code of 2^32 - 1
+        // sort by (key, suffix): pad lanes sort last even past a key of 2^32 - 1
         uint32_t key = valid ? sx[gl + e] : 0xffffffffu;
         uint32_t val = valid ? (SVL ? sv[gl + e] : vals[td.start + gl + e]) : 0xffffffffu;
         for (uint32_t k = 2; k <= seg; k <<= 1) {
@@ -1657,13 +1640,15 @@ int tune_state(int which, int* choice, double* ns_per_elem)
 template <typename K>
 static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
                          uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s, bool* alt,
-                         bool iota_vals = false)
+                         bool iota_vals = false, const K* kin = nullptr)
 {
     if (bits == 11)
-        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
+        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
+                                     kin);
     if (bits == 9)
-        return seg_radix_sort<K, 9>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
-    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals);
+        return seg_radix_sort<K, 9>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
+                                    kin);
+    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals, kin);
 }
 
 static int bitlen(uint64_t v)
@@ -1712,7 +1697,6 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&rank_, c * 4));
     QZ_HIP(hipMalloc(&bufA_, tile_cap * 4));  // last head per radix tile
     QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(c, seg_words) * 4));
-    bufB_words_ = std::max<size_t>(c, seg_words);
     QZ_HIP(hipMalloc(&keys0_, c * 4));
     QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
@@ -1759,7 +1743,6 @@ void BwtEngine::release()
     pinned_ = nullptr;
     cap_ = 0;
     cap_blk_ = 0;
-    bufB_words_ = 0;
 }
 
 static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint64_t cap, bool down)
@@ -1786,10 +1769,6 @@ static int rle_force()
 // replayed as a graph: clear the counters, count symbols, derive the packing
 // on the device, build the packed prefix keys, sort them (32 key bits),
 // then mark group heads and emit SA, ranks and the unsorted groups.
-// Word offset in bufB of the fused round's 4-bit text: past the round-0
-// histograms, 256-byte aligned.
-static size_t fuse_nib_off(uint32_t hist_words) { return ((size_t)hist_words + 63) & ~(size_t)63; }
-
 // QZBSC_FUSE=0 keeps the first doubling round out of round 0's finish.
 static bool fuse_enabled()
 {
@@ -1799,7 +1778,8 @@ static bool fuse_enabled()
 
 // Launch shape of the fused finish, QZBSC_FUSE_SHAPE=<threads>[s]: 256, 512
 // or 1024 threads per tile, "s" for suffixes kept in LDS. Returns the code
-// enqueue_round0 takes (0 = the default).
+// enqueue_round0 takes (0 = the default, 512 threads with suffixes in LDS,
+// the fastest on the MI355X).
 static int fuse_shape()
 {
     const char* e = getenv("QZBSC_FUSE_SHAPE");
@@ -1809,7 +1789,7 @@ static int fuse_shape()
 }
 
 int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
-                              uint32_t hist_words, bool timed, bool fuse, int fshape, hipStream_t s)
+                              uint32_t hist_words, bool timed, uint32_t fuse, int fshape, hipStream_t s)
 {
     const BlockMap bmap_{off_, btab_, nblk};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
@@ -1826,21 +1806,17 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
                        (const uint32_t*)brun_);
     QZ_HIP(hipGetLastError());
     const uint32_t g = qz_div_up(n, kThreads);
+    // positional keys in dep_[1] (free until the first doubling round):
+    // the radix sort reads them in its first pass and leaves them intact
+    // for the fused first doubling round
+    uint32_t* kpos = dep_[1];
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
-                        (const PackParams*)ppd_, (const uint32_t*)brun_, keys0_);
+                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos);
     QZ_HIP(hipGetLastError());
-    // the 4-bit text for the fused first doubling round, in bufB past the
-    // radix histograms (k_pack_nib leaves it alone for larger alphabets)
-    uint32_t* nib = bufB_ + fuse_nib_off(hist_words);
-    if (fuse) {
-        hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
-                           (const PackParams*)ppd_, nib);
-        QZ_HIP(hipGetLastError());
-    }
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
     QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
-                                   32, bufB_, tmp_, s, &alt, true));
+                                   32, bufB_, tmp_, s, &alt, true, kpos));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;
@@ -1855,12 +1831,12 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     // choice depends on the key mode and alphabet, decided on the device
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
                        (const TileDesc*)tiles0_, bufA_, sa_, rank_, glA, (const PackParams*)ppd_, dep_[0], bmap_,
-                       (const uint32_t*)brun_, fuse ? 1u : 0u);
+                       (const uint32_t*)brun_, fuse);
     QZ_HIP(hipGetLastError());
     if (fuse) {
 #define QZ_FINAL0F(T, SVL)                                                                                      \
     hipLaunchKernelGGL((k_final0f<T, SVL>), dim3(ntiles), dim3(T), 0, s, keys, vals, (const TileDesc*)tiles0_, \
-                       bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)nib, 1u)
+                       bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)kpos, fuse)
         switch (fshape) {
             case 1: QZ_FINAL0F(256, false); break;
             case 2: QZ_FINAL0F(256, true); break;
@@ -1895,8 +1871,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     uint32_t ntiles = 0;
     const uint32_t hist_words = count_tiles(offs_host_.data(), nblk, &ntiles, 1u << bits0);
     // the fused first doubling round needs room for the 4-bit text in bufB
-    const bool fuse = fuse_enabled() && fuse_nib_off(hist_words) + (size_t)n / 8 + 2 <= bufB_words_;
     const int fshape = fuse_shape();
+    const uint32_t fuse = fuse_enabled() ? 1u : 0u;
     auto prepare0 = [&](hipStream_t st) -> int {
         uint32_t nt = 0;
         build_tiles(offs_host_.data(), nblk, tiles_host_.data(), &nt, 1u << bits0);

@@ -33,7 +33,7 @@ public:
 private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
     int enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
-                       uint32_t hist_words, bool timed, bool fuse, int fshape, hipStream_t s);
+                       uint32_t hist_words, bool timed, uint32_t fuse, int fshape, hipStream_t s);
     void* tiles0_ = nullptr;       // round-0 radix tiles (device; read by the round-0 graph)
     void* ppd_ = nullptr;          // symbol packing of round 0 (device)
     uint32_t* btab_ = nullptr;     // position bucket -> block (device), see BlockMap
@@ -41,7 +41,6 @@ private:
     GraphCache g0_;                // round 0 as one graph
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
-    size_t bufB_words_ = 0;
     uint32_t* keys0_ = nullptr;
     uint32_t* keys1_ = nullptr;
     uint32_t* vals0_ = nullptr;

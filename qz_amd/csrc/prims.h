@@ -297,22 +297,25 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 // Sorts by key bits [0, key_bits) with BITS-bit digits. `tiles` must have
 // been built with radix 1 << BITS (hist_words from build_tiles). hist holds
 // hist_words, stmp the scan scratch (scan_tmp_words(hist_words)). Result in (k0,v0) for even pass counts.
+// kin (optional): the first pass reads its keys from kin instead of k0,
+// which then serves only as a buffer, so kin is left as it was.
 template <typename K, int BITS>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
                           uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s,
-                          bool* in_alt, bool iota_vals = false)
+                          bool* in_alt, bool iota_vals = false, const K* kin = nullptr)
 {
     *in_alt = false;
     K *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     int passes = 0;
     for (int sh = 0; sh < key_bits; sh += BITS, ++passes) {
-        hipLaunchKernelGGL((k_seg_hist<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, tiles, sh, hist);
+        const K* src = (passes == 0 && kin) ? kin : ka;
+        hipLaunchKernelGGL((k_seg_hist<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, tiles, sh, hist);
         QZ_HIP(hipGetLastError());
         QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
         // iota_vals: the first pass takes each key's position as its value
         const uint32_t* vin = (iota_vals && passes == 0) ? nullptr : va;
-        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, ka, vin, kb, vb, tiles,
+        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, vin, kb, vb, tiles,
                            sh, hist);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
