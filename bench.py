@@ -25,7 +25,10 @@ Further sections, each in the same JSON line (and each with a --no-* flag):
   * streams — config 3 (binned qualities) and the config-2 header stream as
     their own batches;
   * archive — FASTQ -> .qz -> FASTQ end to end (BSC qualities, and qz's
-    quality_ctx choice), and the batch block calls against one call per block.
+    quality_ctx choice), and the batch block calls against one call per block;
+  * random_access — the device's ceilings for random 4-byte gathers and
+    stores over 100 MB and 6 GB arrays, beside a streaming copy: the shapes
+    the suffix sort and the inverse are bound by.
 """
 from __future__ import annotations
 
@@ -53,8 +56,8 @@ def log(*a):
 
 def measured_traffic(total: int):
     """HBM bytes per launch of the forward and inverse launch sequences, from
-    the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised under profiles/
-    (copy-calibrated), when they were taken on this workload size."""
+    the rocprofv3 request-count runs summarised under profiles/, when they
+    were taken on this workload size."""
     p = ROOT / "profiles" / "traffic_seq58x25MiB.json"
     try:
         d = json.loads(p.read_text())
@@ -97,6 +100,31 @@ def cpu_baseline(blocks, threads: int):
         "per_core_mb_s": round(nbytes / dt / 1e6 / len(sample), 3),
         "nproc": os.cpu_count(),
     }
+
+
+def random_access_section(lib, array_sizes=(100_000_000, 6_000_000_000), items=1 << 28, iters=5):
+    """The device's own ceilings for the access shapes the suffix sort and the
+    inverse are bound by: uniform random 4-byte gathers (index and output
+    streamed) and random 4-byte stores over arrays of one block's rank array
+    (100 MB) and of a whole batch's (6 GB), beside a streaming copy."""
+    import ctypes as C
+    rows = []
+    for nbytes in array_sizes:
+        for mode, shape in ((0, "gather"), (1, "scatter"), (2, "copy")):
+            ms = C.c_double(0.0)
+            rc = lib.qzbsc_calib_access(mode, nbytes, items, iters, C.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"qzbsc_calib_access({mode}, {nbytes}) returned {rc}")
+            row = {"array_bytes": nbytes, "shape": shape, "ms": round(ms.value, 4)}
+            if mode == 2:
+                row["gb_s"] = round(2 * nbytes / ms.value / 1e6, 1)
+            else:
+                row["items"] = items
+                row["g_items_per_s"] = round(items / ms.value / 1e6, 3)
+            rows.append(row)
+    return {"workload": f"{items} uniform random 4-byte accesses per launch (gather: dst[i] = src[idx[i]], "
+                        "scatter: dst[idx[i]] = i) and a 16-byte copy of the array; mean of "
+                        f"{iters} launches after one warm-up", "rows": rows}
 
 
 def uneven_section(torch, bwt, synth, count: int, seed: int, cap: int, lo: int = 100_000, hi: int | None = None):
@@ -469,6 +497,7 @@ def main():
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
     ap.add_argument("--archive-reads", type=int, default=1_000_000)
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the random-access ceiling section")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -539,8 +568,6 @@ def main():
     if not args.no_per_block:
         blat = []
         offs = np.concatenate([[0], np.cumsum(sizes)])
-        for b in range(len(blocks)):  # warm
-            pass
         t1 = time.perf_counter()
         for b in range(len(blocks)):
             ts = time.perf_counter()
@@ -578,6 +605,11 @@ def main():
         "bwt_ibwt_frac": round((ALG_BYTES_FWD + ALG_BYTES_INV) * total / (fwd + inv) / 1e9 / HBM_PEAK_GBS, 5),
     }
 
+    ceiling = None
+    if not args.no_ceiling:
+        ceiling = random_access_section(lib)
+        log(f"random-access ceiling: {ceiling}")
+
     graphs = None
     if not args.no_graph:
         graphs = graph_section(torch, bwt, synth)
@@ -587,8 +619,9 @@ def main():
     if tr:
         roof["traffic"] = tr["forward"]["hbm_bytes"]
         roof["inverse"]["traffic"] = tr["inverse"]["hbm_bytes"]
-        roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
-                                "workload (profiles/traffic_seq58x25MiB.json), scaled by a 256 MiB copy: "
+        roof["traffic_note"] = ("HBM bytes per launch from rocprofv3 --pmc runs of this workload "
+                                "(profiles/traffic_seq58x25MiB.json): TCC read and write requests resolved by "
+                                "request size, the rule checked on a 6 GB copy kernel: "
                                 f"{tr['forward']['hbm_bytes_per_input_byte']} B per input byte forward, "
                                 f"{tr['inverse']['hbm_bytes_per_input_byte']} inverse")
 
@@ -657,6 +690,7 @@ def main():
         "per_block": per_block,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "random_access": ceiling,
         "graph_replay": graphs,
         "uneven_batch": uneven,
         "serving": serving,

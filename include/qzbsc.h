@@ -273,6 +273,15 @@ QZBSC_API long long qzbsc_trim(void);
 /* 1 if a HIP device is usable, else 0. */
 QZBSC_API int qzbsc_gpu_available(void);
 
+/* Memory-system calibration (no counterpart in the reference; not on the
+ * block path): the mean ms per launch of `iters` launches of one access
+ * shape over a device array of array_bytes bytes (< 16 GiB). mode 0: items
+ * random 4-byte gathers dst[i] = src[idx[i]] with idx and dst streamed;
+ * mode 1: items random 4-byte stores dst[idx[i]] = i; mode 2: a 16-byte
+ * vector copy of the whole array (items unused). Indexes are uniform over
+ * the array. Allocates and frees its own buffers. */
+QZBSC_API int qzbsc_calib_access(int mode, long long array_bytes, long long items, int iters, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

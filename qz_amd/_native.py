@@ -63,6 +63,7 @@ _SIGS = {
     "qzbsc_lzp_decompress": (C.c_int, [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int]),
     "qzbsc_quality_ctx_compress": (C.c_longlong, [u8p, i32p, u8p, i32p, C.c_int, u8p, C.c_longlong]),
     "qzbsc_quality_ctx_decompress": (C.c_int, [u8p, C.c_longlong, u8p, i32p, C.c_int, u8p, i32p]),
+    "qzbsc_calib_access": (C.c_int, [C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None
