@@ -22,7 +22,7 @@ Further sections, each in the same JSON line (and each with a --no-* flag):
   * serving — Poisson arrivals of compress / decompress jobs through the
     serving loop (continuous batching, footprint admission, small-job lane):
     p50 / p99 latency against each job alone;
-  * streams — config 3 (binned qualities) and the config-2 header stream as
+  * streams — binned qualities (1.5e9 B) and Illumina names (10M reads) as
     their own batches;
   * archive — FASTQ -> .qz -> FASTQ end to end (BSC qualities, and qz's
     quality_ctx choice), and the batch block calls against one call per block;
@@ -247,13 +247,12 @@ def graph_section(torch, bwt, synth, sizes=(100_000, 1_000_000, 4_000_000), jobs
 
 def streams_section(torch, bwt, synth, reads: int, steps: int):
     """The other streams of the configurations, each as one batch of 25 MiB
-    blocks from HBM, forward + inverse BWT like the headline: config 3
-    (binned qualities, 150 per read) and the config-2 header stream
-    (Illumina names)."""
+    blocks from HBM, forward + inverse BWT like the headline: binned
+    qualities (150 per read) and the names of the same reads."""
     dev = torch.device("cuda:0")
     rows = []
-    for name, gen in (("config 3 qualities", lambda: synth.quality_stream(reads, 150, 3)),
-                      ("config 2 headers", lambda: synth.header_stream(reads, 4))):
+    for name, gen in (("binned qualities 1.5e9 B", lambda: synth.quality_stream(reads, 150, 3)),
+                      ("Illumina names", lambda: synth.header_stream(reads, 4))):
         stream = gen()
         sizes = np.array([b.size for b in synth.cut_blocks(stream)], dtype=np.int32)
         d_in = torch.from_numpy(stream).to(dev)
@@ -516,7 +515,7 @@ def main():
     lib = qz_amd.lib()
 
     t = time.perf_counter()
-    stream = synth.config2_sequences(args.reads, seed=2)
+    stream = synth.reads_50mbp_stream(args.reads, seed=2)
     blocks = synth.cut_blocks(stream)
     sizes = np.array([b.size for b in blocks], dtype=np.int32)
     total = int(sizes.sum())

@@ -72,9 +72,16 @@ def _ascii_uint(v: np.ndarray, width: int):
     return d, keep
 
 
+_COMMENT = np.frombuffer(b" 1:N:0:", dtype=np.uint8)
+_INDEX = np.frombuffer(b"GATCAGAT+AGATCTCG", dtype=np.uint8)
+_INDEX_BASES = np.array([i for i in range(17) if i != 8])
+
+
 def header_stream(n_reads: int, seed: int, reads_per_tile: int = 4000) -> np.ndarray:
-    """varint(len) + "@A00123:456:HGK7WDSXY:LANE:TILE:X:Y" per read; tiles
-    increment every `reads_per_tile` reads, x and y are uniform below 30000."""
+    """varint(len) + "@A00123:456:HGK7WDSXY:LANE:TILE:X:Y 1:N:0:I7+I5" per
+    read (about 64 bytes); tiles increment every `reads_per_tile` reads, x and
+    y are uniform below 30000, and the dual sample index reads GATCAGAT +
+    AGATCTCG except for one substituted base in 1% of reads."""
     rng = np.random.default_rng(seed)
     parts = []
     chunk = 1 << 20
@@ -96,6 +103,13 @@ def header_stream(n_reads: int, seed: int, reads_per_tile: int = 4000) -> np.nda
             cols += [d, colon]
             keeps += [kp, on]
         cols, keeps = cols[:-1], keeps[:-1]
+        # Illumina read comment: read 1, not filtered, control 0, dual index
+        idx = np.broadcast_to(_INDEX, (k, _INDEX.size)).copy()
+        hit = np.flatnonzero(rng.random(k) < 0.01)
+        pos = _INDEX_BASES[rng.integers(0, _INDEX_BASES.size, size=hit.size)]
+        idx[hit, pos] = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, size=hit.size)]
+        cols += [np.broadcast_to(_COMMENT, (k, _COMMENT.size)), idx]
+        keeps += [np.ones((k, _COMMENT.size), bool), np.ones((k, _INDEX.size), bool)]
         body = np.concatenate(cols, axis=1)
         keep = np.concatenate(keeps, axis=1)
         length = keep.sum(axis=1).astype(np.uint8)  # < 128: one varint byte
@@ -140,12 +154,12 @@ def quality_stream(n_reads: int, read_len: int, seed: int, p_repeat: float = 0.8
     return out.reshape(-1)
 
 
-def config1_block(seed: int = 1) -> np.ndarray:
+def reads_15mb_block(seed: int = 1) -> np.ndarray:
     """1 Mbp genome, 100K x 150 bp reads, 0.1% substitutions: one 15 MB block."""
     return sequence_stream(1_000_000, 100_000, 150, seed, 0.001)
 
 
-def config2_sequences(n_reads: int, seed: int = 2) -> np.ndarray:
+def reads_50mbp_stream(n_reads: int, seed: int = 2) -> np.ndarray:
     """50 Mbp genome, 150 bp reads, 0.2% substitutions, 0.01% N."""
     return sequence_stream(50_000_000, n_reads, 150, seed, 0.002, 0.0001)
 
@@ -189,8 +203,8 @@ def mixed_blocks(count: int, seed: int, lo: int = 100_000, hi: int = BLOCK, tota
     need = [min(pool_cap, max(int(sizes[kinds == k].sum()), int(sizes[kinds == k].max(initial=0)))) for k in range(3)]
     need = [max(n, int(sizes[kinds == k].max(initial=0))) for k, n in enumerate(need)]
     pools = [
-        config2_sequences(need[0] // 150 + 2, seed + 11) if need[0] else np.zeros(0, np.uint8),
-        header_stream(need[1] // 38 + 2, seed + 12) if need[1] else np.zeros(0, np.uint8),
+        reads_50mbp_stream(need[0] // 150 + 2, seed + 11) if need[0] else np.zeros(0, np.uint8),
+        header_stream(need[1] // 60 + 2, seed + 12) if need[1] else np.zeros(0, np.uint8),
         quality_stream(need[2] // 150 + 2, 150, seed + 13) if need[2] else np.zeros(0, np.uint8),
     ]
     blocks = []

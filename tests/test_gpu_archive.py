@@ -135,7 +135,7 @@ def test_batch_forms_across_device_sub_batches():
     with LZP."""
     rng = np.random.default_rng(14)
     blocks = [
-        synth.config2_sequences(140_000, 5)[:20_000_000],
+        synth.reads_50mbp_stream(140_000, 5)[:20_000_000],
         rng.integers(65, 69, 5_000, dtype=np.uint8),
         synth.quality_stream(140_000, 150, 6)[:20_000_000],
         np.frombuffer(b"ACGT", np.uint8),

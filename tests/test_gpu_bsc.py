@@ -54,7 +54,7 @@ def test_roundtrip_and_fields():
 
 
 def test_ratio_and_mt_identity():
-    data = synth.config1_block(3)[:3_000_000].tobytes()
+    data = synth.reads_15mb_block(3)[:3_000_000].tobytes()
     a = bsc.compress_adaptive_no_lzp(data)
     b = bsc.compress_adaptive_mt_no_lzp(data)
     assert a == b
@@ -128,7 +128,7 @@ def test_large_block_mt_roundtrip():
     BWT, primary index and trailer indexes equal the oracle's (SA-IS on the
     host, run in a thread beside the GPU calls)."""
     import threading
-    data = synth.config2_sequences(2_700_000, seed=31)[:400_000_000]
+    data = synth.reads_50mbp_stream(2_700_000, seed=31)[:400_000_000]
     assert data.size == 400_000_000
     ref = {}
     th = threading.Thread(target=lambda: ref.update(zip(("u", "p", "idx"), O.bsc_bwt_encode(data))))
@@ -155,7 +155,7 @@ def test_large_block_mt_roundtrip():
 def test_ultra_block_750mb_roundtrip():
     """ultra's largest sequence block (750 MB, ultra.rs:44-55, 877-887)
     through the drop-in ABI: header fields, sample count, round trip."""
-    data = synth.config2_sequences(5_000_000, seed=33)[:750_000_000]
+    data = synth.reads_50mbp_stream(5_000_000, seed=33)[:750_000_000]
     assert data.size == 750_000_000
     blk = bsc.compress_adaptive_mt_no_lzp(data)
     size, n, mode = struct.unpack_from("<iii", blk, 0)

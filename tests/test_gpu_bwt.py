@@ -66,9 +66,9 @@ def test_bwt_mixed_streams(gpu):
     _check(blocks, gpu)
 
 
-def test_bwt_config1_block(gpu):
+def test_bwt_reads_15mb_block(gpu):
     """One 15,000,000-byte sequence block (1 Mbp genome, 100K x 150 bp)."""
-    blk = synth.config1_block(seed=1)
+    blk = synth.reads_15mb_block(seed=1)
     assert blk.size == 15_000_000
     _check([blk], gpu)
 

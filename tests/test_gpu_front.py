@@ -30,7 +30,7 @@ def gpu_front(bwt):
 
 def _blocks():
     rng = np.random.default_rng(17)
-    out = [synth.config1_block(5)[:3_000_000], synth.header_stream(60_000, 6), synth.quality_stream(20_000, 150, 7),
+    out = [synth.reads_15mb_block(5)[:3_000_000], synth.header_stream(60_000, 6), synth.quality_stream(20_000, 150, 7),
            rng.integers(0, 256, 700_000).astype(np.uint8), np.full(600_000, 7, np.uint8),
            np.concatenate([np.full(900_000, 65, np.uint8), np.full(10, 66, np.uint8)]),
            rng.integers(0, 4, 40_000).astype(np.uint8), np.frombuffer(b"AB" * 50, np.uint8)]

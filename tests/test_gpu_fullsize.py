@@ -1,8 +1,9 @@
 """Full-size workloads on the GPU, through the batch C-ABI.
 
-* config 2: 1.5e9 B of sequence reads in 58 x 25 MiB blocks, one batch;
-* config 3: 1.5e9 B of binned qualities in 25 MiB blocks, one batch;
-* the config-2 header stream (Illumina names, about 0.6 GB).
+* 1.5e9 B of sequence reads (50 Mbp genome, 10M x 150 bp) in 58 x 25 MiB
+  blocks, one batch;
+* 1.5e9 B of binned qualities in 25 MiB blocks, one batch;
+* the names of the same 10M reads (Illumina style, about 0.64 GB).
 
 Every block is checked by size-independent properties: the BWT is a
 permutation of the block (byte histogram kept), the primary index lies in
@@ -40,13 +41,13 @@ def _check(name, stream, oracle_blocks=(0, -1)):
         print(f"{name}: block {b} matches the oracle", flush=True)
 
 
-def test_config2_sequences_full_size():
-    _check("config 2 sequences", synth.config2_sequences(10_000_000, seed=2))
+def test_reads_1_5gb_full_size():
+    _check("reads 1.5e9 B", synth.reads_50mbp_stream(10_000_000, seed=2))
 
 
-def test_config3_qualities_full_size():
-    _check("config 3 qualities", synth.quality_stream(10_000_000, 150, 3), oracle_blocks=(0,))
+def test_binned_qualities_1_5gb_full_size():
+    _check("binned qualities 1.5e9 B", synth.quality_stream(10_000_000, 150, 3), oracle_blocks=(0,))
 
 
-def test_config2_headers_full_size():
-    _check("config 2 headers", synth.header_stream(10_000_000, 4), oracle_blocks=(0,))
+def test_illumina_names_full_size():
+    _check("Illumina names", synth.header_stream(10_000_000, 4), oracle_blocks=(0,))

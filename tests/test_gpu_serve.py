@@ -102,7 +102,7 @@ def test_small_job_lane_skips_the_queue():
     """One regular worker busy with a 100 MB job (one unit): a small job
     submitted right after goes to the small-job lane and finishes first, with
     the same result as alone."""
-    big = synth.cut_blocks(synth.config2_sequences(700_000, seed=8))[:4]
+    big = synth.cut_blocks(synth.reads_50mbp_stream(700_000, seed=8))[:4]
     small = [synth.sequence_stream(100_000, 1_000, 150, 9, 0.002)]
     print("lane test:", [b.size for b in big], small[0].size, flush=True)
     with serve.Server(hbm_budget=64 << 30, max_batch_bytes=1 << 30, workers=1, small_lane_bytes=32 << 20,

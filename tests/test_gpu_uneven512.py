@@ -30,23 +30,23 @@ def _sub_batches(sizes):
     return groups
 
 
-def test_config4_uneven_512_blocks():
+def test_uneven_512_blocks():
     blocks, kinds = synth.mixed_blocks(512, seed=4)
     sizes = [int(b.size) for b in blocks]
-    print(f"config 4: {len(blocks)} blocks, {sum(sizes)} B, {min(sizes)}..{max(sizes)} B", flush=True)
+    print(f"uneven 512: {len(blocks)} blocks, {sum(sizes)} B, {min(sizes)}..{max(sizes)} B", flush=True)
     assert len(blocks) == 512 and min(sizes) >= 100_000 and max(sizes) <= synth.BLOCK
     out, prim, aux = [None] * 512, [0] * 512, [None] * 512
     for g in _sub_batches(sizes):
         o, p, a = bwt.encode_batch([blocks[b] for b in g])
         for i, b in enumerate(g):
             out[b], prim[b], aux[b] = o[i], p[i], a[i]
-    print("config 4: forward done", flush=True)
+    print("uneven 512: forward done", flush=True)
     back = [None] * 512
     for g in _sub_batches(sizes):
         r = bwt.decode_batch([out[b] for b in g], [prim[b] for b in g])
         for i, b in enumerate(g):
             back[b] = r[i]
-    print("config 4: inverse done", flush=True)
+    print("uneven 512: inverse done", flush=True)
     for b, blk in enumerate(blocks):
         assert np.array_equal(np.bincount(out[b], minlength=256), np.bincount(blk, minlength=256)), b
         assert prim[b] == aux[b][0] and aux[b].size == (blk.size - 1) // O.sample_rate(blk.size) + 1, b
@@ -56,7 +56,7 @@ def test_config4_uneven_512_blocks():
         picks.add(int(np.flatnonzero(kinds == k)[0]))
     for b in sorted(picks):
         u, p, a = O.bwt_aux(blocks[b])
-        print(f"config 4: oracle block {b} kind {int(kinds[b])} {sizes[b]} B", flush=True)
+        print(f"uneven 512: oracle block {b} kind {int(kinds[b])} {sizes[b]} B", flush=True)
         assert np.array_equal(out[b], u) and prim[b] == p and np.array_equal(aux[b], a), f"oracle block {b}"
     for b in sorted(picks)[:3]:
         o1, p1, a1 = bwt.encode_batch([blocks[b]])

@@ -115,7 +115,7 @@ def _coder_rt(bwt, features=0):
 def test_coder_roundtrip_streams():
     from qz_amd import synth
     rng = np.random.default_rng(3)
-    for data in [synth.config1_block(1)[:2_000_000], synth.header_stream(20_000, 4),
+    for data in [synth.reads_15mb_block(1)[:2_000_000], synth.header_stream(20_000, 4),
                  synth.quality_stream(10_000, 150, 5), np.full(700_000, 65, np.uint8),
                  rng.integers(0, 3, 100_000).astype(np.uint8)]:
         u, p, _ = O.bwt_aux(data)
@@ -145,7 +145,7 @@ def test_coder_incompressible_is_rejected():
 
 def test_coder_corrupt_segment_fails():
     from qz_amd import synth
-    u, _, _ = O.bwt_aux(synth.config1_block(2)[:300_000])
+    u, _, _ = O.bwt_aux(synth.reads_15mb_block(2)[:300_000])
     L = qz_amd.lib()
     out = np.zeros(u.size + 4096, np.uint8)
     r = L.qzbsc_coder_compress(u.ctypes.data_as(u8p), u.size, out.ctypes.data_as(u8p), out.size, 0)
