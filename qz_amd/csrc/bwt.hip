@@ -7,8 +7,10 @@
 //
 // Layout in HBM (positions are u32; a batch holds < 2^31 bytes):
 //   text[N]      input bytes of all blocks, block b at [off[b], off[b+1])
-//   sa[N]        suffix array, global positions; block b's suffixes occupy
-//                SA slots [off[b], off[b+1])
+//   sa[N]        suffix array; block b's suffixes occupy SA slots
+//                [off[b], off[b+1]); an entry is the suffix's global
+//                position, or (char mode, see SaVal) its block-local position
+//                and the code of the symbol before it
 //   rank[N]      per text position, the SA slot where its current group
 //                starts (Larsson-Sadakane ranks); the inverse SA at the end
 //   key2[N]      per SA slot, the doubling key gathered for this round
@@ -110,6 +112,17 @@ struct BlockMap {
         return lo;
     }
 };
+
+// SA entries. Plain: the suffix's global position. Char mode (cb = 4, a batch
+// of at most 15 distinct bytes in blocks below 2^28 bytes): the block-local
+// position << 4 | the code of the byte before the suffix, so every sort that
+// moves a suffix also moves its BWT symbol and the output is a streaming pass
+// (the code of a block's first position is unused: the primary row emits
+// none). An entry's block is the block of its SA slot.
+__device__ __forceinline__ uint32_t sa_pos(uint32_t v, uint32_t cb, uint32_t block_off)
+{
+    return cb ? (v >> cb) + block_off : v;
+}
 
 // Host side of BlockMap::find: tab[k] = last block b with off[b] <= k << kTabShift.
 static void build_block_tab(const std::vector<uint32_t>& offs, uint32_t n, std::vector<uint32_t>& tab)
@@ -276,11 +289,12 @@ __device__ __forceinline__ void stage_flush(Stage& S, const GroupLists& gl) { st
 
 // The doubling key of suffix s: 1 + rank of s + d (d = the depth of s's
 // group, the symbols all its members share), 0 past the block end.
-__device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const BlockMap& bm, uint32_t s,
-                                              uint32_t d)
+__device__ __forceinline__ uint32_t round_key(const uint32_t* __restrict__ rank, const BlockMap& bm, uint32_t slot,
+                                              uint32_t v, uint32_t cb, uint32_t d)
 {
-    const uint32_t end = bm.off[bm.find(s) + 1];
-    const uint64_t p = (uint64_t)s + d;
+    const uint32_t b = bm.find(slot);
+    const uint32_t end = bm.off[b + 1];
+    const uint64_t p = (uint64_t)sa_pos(v, cb, bm.off[b]) + d;
     return (p < end) ? rank[(uint32_t)p] + 1u : 0u;
 }
 
@@ -351,6 +365,7 @@ struct PackParams {
     uint32_t radix;      // sigma + 1
     uint32_t k0;         // symbols per 32-bit key (packed keys)
     uint32_t rle;        // 1: some block has run-length keys (per-group depths)
+    uint32_t cb;         // SA entries carry this many bits of preceding-symbol code (0 or 4)
     uint32_t sb, lb, nseg, segbits, cap;
     int32_t force;       // -1 per block by its runs, 0 no block, 1 every block
 };
@@ -619,7 +634,8 @@ __global__ __launch_bounds__(kThreads) void k_block_runs(const uint8_t* __restri
 // Symbol packing and the key mode (force: -1 per block by its runs, 0 never
 // run-length keys, 1 always).
 __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp,
-                                                        int force, BlockMap bm, const uint32_t* __restrict__ brun)
+                                                        int force, BlockMap bm, const uint32_t* __restrict__ brun,
+                                                        uint32_t small_blocks)
 {
     __shared__ uint32_t any_rle;
     if (threadIdx.x == 0) any_rle = 0;
@@ -653,6 +669,8 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         }
         pp->radix = (uint32_t)radix;
         pp->k0 = k ? k : 1;
+        // char mode: codes 1..15 and block-local positions below 2^28
+        pp->cb = (radix <= 16 && small_blocks) ? 4u : 0u;
         const bool rle = force >= 0 ? force != 0 : any_rle != 0;
         uint32_t sb = 0;
         while ((1ull << sb) < radix) ++sb;  // codes 0..sigma
@@ -666,6 +684,38 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         pp->segbits = segbits;
         pp->cap = (1u << lb) - 1u;
         pp->force = force;
+    }
+}
+
+// The values round 0's radix sort carries: every position's SA entry (char
+// mode: block-local position << 4 | code of the byte before it; else the
+// global position), four consecutive positions per thread, stored as one
+// 16-byte vector when whole.
+__global__ __launch_bounds__(kThreads) void k_init_vals(const uint8_t* __restrict__ text, uint32_t n, BlockMap bm,
+                                                        const PackParams* __restrict__ ppd, uint32_t* __restrict__ vals)
+{
+    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * 4u;
+    if (i0 >= n) return;
+    const uint32_t cb = ppd->cb;
+    uint32_t b = bm.find(i0);
+    uint32_t o = bm.off[b], e = bm.off[b + 1];
+    uint32_t v[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t i = i0 + q;
+        v[q] = 0;
+        if (i >= n) continue;
+        if (i >= e) {
+            ++b;
+            o = e;
+            e = bm.off[b + 1];
+        }
+        v[q] = cb ? ((i - o) << 4) | (i > o ? (uint32_t)ppd->code[text[i - 1]] : 0u) : i;
+    }
+    if (i0 + 4u <= n) {
+        *reinterpret_cast<uint4*>(vals + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+        for (uint32_t q = 0; i0 + q < n; ++q) vals[i0 + q] = v[q];
     }
 }
 
@@ -756,6 +806,7 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
     const bool last_tile = td.tlocal + 1 == td.ntiles;
     const bool track = ppd->rle != 0;  // groups carry depths
     const bool rleb = track && block_rle(*ppd, bm, brun, td.block);
+    const uint32_t cb = ppd->cb, bo = bm.off[td.block];
     for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
         const uint32_t i = b + lane;
         const bool valid = i < td.len;
@@ -772,7 +823,7 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
         if (valid) {
             const uint32_t s = vals[j];
             sa[j] = s;
-            rank[s] = g;
+            rank[sa_pos(s, cb, bo)] = g;
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
             if (end && j > g) {
                 stage_push(S, g, j - g + 1);
@@ -890,6 +941,7 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
     uint32_t carry = pre[blockIdx.x];
     for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
     const bool last_tile = td.tlocal + 1 == td.ntiles;
+    const uint32_t cb = ppd->cb, bo = bm.off[td.block];
     // phase 1
     for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
         const uint32_t i = b + lane;
@@ -909,7 +961,7 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
             if (g < td.start || (head && end)) {  // final now
                 sa[j] = s;
-                rank[s] = g;
+                rank[sa_pos(s, cb, bo)] = g;
                 sx[i] = kWaitNone;
             } else {
                 if (SVL) sv[i] = s;
@@ -960,10 +1012,10 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
                     const uint32_t s = SVL ? sv[i] : vals[td.start + i];
                     if ((fl[x >> 5] >> (x & 31u)) & 1u) {
                         want[q] = true;
-                        pos[q] = s + k0;
+                        pos[q] = sa_pos(s, cb, bo) + k0;
                     } else {
                         sa[td.start + i] = s;
-                        rank[s] = td.start + x;
+                        rank[sa_pos(s, cb, bo)] = td.start + x;
                     }
                 }
             }
@@ -1021,7 +1073,7 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
         if (valid) {
             const uint32_t g = td.start + gl;
             sa[g + e] = val;
-            rank[val] = g + hp;
+            rank[sa_pos(val, cb, bo)] = g + hp;
             const bool end = (e + 1 == size) || knext != key;
             if (end && e > hp) stage2_push(S, 1, g + hp, e - hp + 1);
         }
@@ -1060,7 +1112,7 @@ struct CurLists {
 __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPlan plan,
                                                            const uint32_t* __restrict__ sa,
                                                            const uint32_t* __restrict__ rank,
-                                                           BlockMap bm, uint32_t h,
+                                                           BlockMap bm, uint32_t cb, uint32_t h,
                                                            const uint32_t* __restrict__ dep,
                                                            uint32_t* __restrict__ key2)
 {
@@ -1071,7 +1123,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPla
     const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
     if (valid) {
         const uint32_t slot = start + e;
-        key2[slot] = round_key(rank, bm, sa[slot], dep ? dep[start] : h);
+        key2[slot] = round_key(rank, bm, slot, sa[slot], cb, dep ? dep[start] : h);
     }
 }
 
@@ -1080,7 +1132,7 @@ constexpr int kSmallTasks = 16;  // wave tasks per wave in k_sort_small (<= 4096
 __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan plan, const uint32_t* __restrict__ key2,
                                                          uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
                                                          GroupLists gl, const uint32_t* __restrict__ dep_in,
-                                                         uint32_t* __restrict__ dep_out)
+                                                         uint32_t* __restrict__ dep_out, BlockMap bm, uint32_t cb)
 {
     __shared__ Stage S;
     stage_init(S);
@@ -1123,7 +1175,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
         }
         if (valid) {
             sa[start + e] = val;
-            if (hp) rank[val] = start + hp;  // hp == 0: still the group's start
+            if (hp) rank[sa_pos(val, cb, cb ? bm.off[bm.find(start)] : 0u)] = start + hp;  // hp == 0: still the group's start
             const bool end = (e + 1 == size) || knext != key;
             if (end && e > hp) {
                 stage_push(S, start + hp, e - hp + 1);
@@ -1140,30 +1192,30 @@ __global__ __launch_bounds__(QZ_WAVE) void k_gather_wave(const Grp* __restrict__
                                                          const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
-                                                         uint32_t* __restrict__ key2)
+                                                         BlockMap bm, uint32_t cb, uint32_t h,
+                                                         const uint32_t* __restrict__ dep, uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
     const uint32_t d = dep ? dep[g.start] : h;
     for (uint32_t i = threadIdx.x; i < g.size; i += QZ_WAVE) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, bm, sa[slot], d);
+        key2[slot] = round_key(rank, bm, slot, sa[slot], cb, d);
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather_med(const Grp* __restrict__ list, const uint32_t* __restrict__ cnt,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
-                                                         uint32_t* __restrict__ key2)
+                                                         BlockMap bm, uint32_t cb, uint32_t h,
+                                                         const uint32_t* __restrict__ dep, uint32_t* __restrict__ key2)
 {
     if (blockIdx.x >= *cnt) return;
     const Grp g = list[blockIdx.x];
     const uint32_t d = dep ? dep[g.start] : h;
     for (uint32_t i = threadIdx.x; i < g.size; i += kThreads) {
         const uint32_t slot = g.start + i;
-        key2[slot] = round_key(rank, bm, sa[slot], d);
+        key2[slot] = round_key(rank, bm, slot, sa[slot], cb, d);
     }
 }
 
@@ -1306,7 +1358,7 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
                                                               const uint32_t* __restrict__ key2,
                                                               uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
                                                               GroupLists gl, const uint32_t* __restrict__ dep_in,
-                                                              uint32_t* __restrict__ dep_out)
+                                                              uint32_t* __restrict__ dep_out, BlockMap bm, uint32_t cb)
 {
     constexpr uint32_t T = WAVES * QZ_WAVE;
     constexpr uint32_t MAXN = WAVES == 1 ? kWaveMax : kMedMax;
@@ -1367,11 +1419,12 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
         if (i < g.size && H[i] < excl) H[i] = excl;
     }
     __syncthreads();
+    const uint32_t bo = cb ? bm.off[bm.find(g.start)] : 0u;
     for (uint32_t i = tid; i < g.size; i += T) {
         const uint32_t hp = H[i];
         const uint32_t v = Vf[i];
         sa[g.start + i] = v;
-        if (hp) rank[v] = g.start + hp;  // hp == 0: still the group's start
+        if (hp) rank[sa_pos(v, cb, bo)] = g.start + hp;  // hp == 0: still the group's start
         const bool end = (i + 1 == g.size) || Kf[i + 1] != Kf[i];
         if (end && i > hp) {
             stage_push_e(S, E, g.start + hp, i - hp + 1, MAXN / 2);
@@ -1389,8 +1442,9 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
                                                          const uint32_t* __restrict__ lbase,
                                                          const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t h, const uint32_t* __restrict__ dep,
-                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                                                         BlockMap bm, uint32_t cb, uint32_t h,
+                                                         const uint32_t* __restrict__ dep, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals)
 {
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
@@ -1402,7 +1456,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
         const uint32_t j = td.start + i;
         const uint32_t s = sa[g0 + (j - b0)];
-        const uint64_t p = (uint64_t)s + d;
+        const uint64_t p = (uint64_t)sa_pos(s, cb, bstart) + d;
         keys[j] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
         vals[j] = s;
     }
@@ -1420,7 +1474,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
                                                           const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                           uint32_t* __restrict__ rank, GroupLists gl, BlockMap bm,
                                                           const uint32_t* __restrict__ dep_in,
-                                                          uint32_t* __restrict__ dep_out)
+                                                          uint32_t* __restrict__ dep_out, uint32_t cb)
 {
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);
     __shared__ Stage S;
@@ -1428,7 +1482,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
     stage_init(S);
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
-    const uint32_t bstart = dep_out ? bm.off[bm.find(g0)] : 0u;  // keys are block-local: rank + 1 - bstart
+    const uint32_t bstart = (dep_out || cb) ? bm.off[bm.find(g0)] : 0u;  // keys are block-local: rank + 1 - bstart
     const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
     const uint32_t qb = w * Q;
     uint32_t m = 0;
@@ -1461,7 +1515,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
             const uint32_t slot = g0 + (j - b0);
             const uint32_t hs = g0 + (g - b0);  // slot of the subgroup head
             sa[slot] = s;
-            if (hs != g0) rank[s] = hs;  // the first subgroup keeps the group's start
+            if (hs != g0) rank[sa_pos(s, cb, bstart)] = hs;  // the first subgroup keeps the group's start
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
             if (end && slot > hs) {
                 stage_push(S, hs, slot - hs + 1);
@@ -1481,7 +1535,7 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
 __global__ __launch_bounds__(kThreads) void k_pack_nib(const uint8_t* __restrict__ text, uint32_t n,
                                                        const PackParams* __restrict__ ppd, uint32_t* __restrict__ nib)
 {
-    if (ppd->radix > 16u) return;  // uniform over the grid
+    if (ppd->radix > 16u || ppd->cb) return;  // uniform over the grid (char mode needs no text)
     __shared__ uint8_t code[256];
     code[threadIdx.x] = (uint8_t)ppd->code[threadIdx.x];
     __syncthreads();
@@ -1510,10 +1564,12 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict_
                                                       BlockMap bm,
                                                       uint32_t n, uint8_t* __restrict__ out)
 {
-    // code -> byte for the packed form (k_pack_nib ran with the same test)
+    // code -> byte for the 4-bit forms (k_pack_nib ran with the same test)
     __shared__ uint8_t dec[16];
-    const bool packed = ppd->radix <= 16u;
-    if (packed) {
+    const uint32_t cb = ppd->cb;
+    const bool small = ppd->radix <= 16u;
+    const bool packed = small && !cb;
+    if (small) {
         const uint32_t cd = ppd->code[threadIdx.x];
         if (cd != 0u && cd < 16u) dec[cd] = (uint8_t)threadIdx.x;
     }
@@ -1524,6 +1580,10 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict_
     const uint32_t o = bm.off[b], e = bm.off[b + 1];
     const uint32_t local = j - o;
     const uint32_t p = rank[o] - o + 1u;
+    if (cb) {  // char mode: the symbol travelled with the suffix (a streaming pass)
+        out[j] = local == 0 ? text[e - 1u] : dec[(local < p ? sa[j - 1] : sa[j]) & 15u];
+        return;
+    }
     const uint32_t x = local == 0 ? e - 1u : (local < p ? sa[j - 1] : sa[j]) - 1u;
     out[j] = packed ? dec[(nib[x >> 3] >> (4u * (x & 7u))) & 15u] : text[x];
 }
@@ -1802,10 +1862,16 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_block_runs, dim3(qz_div_up(n, kThreads * kRunSample)), dim3(kThreads), 0, s, text, n, bmap_,
                        brun_);
     QZ_HIP(hipGetLastError());
+    uint32_t maxb = 0;
+    for (uint32_t b = 0; b < nblk; ++b) maxb = std::max(maxb, offs_host_[b + 1] - offs_host_[b]);
+    const uint32_t small_blocks = maxb < (1u << 28) ? 1u : 0u;  // block-local positions fit 28 bits
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force(), bmap_,
-                       (const uint32_t*)brun_);
+                       (const uint32_t*)brun_, small_blocks);
     QZ_HIP(hipGetLastError());
-    const uint32_t g = qz_div_up(n, kThreads);
+    // the SA entries round 0 carries (char mode or positions) in vals0
+    hipLaunchKernelGGL(k_init_vals, dim3(qz_div_up(n, kThreads * 4u)), dim3(kThreads), 0, s, text, n, bmap_,
+                       (const PackParams*)ppd_, vals0_);
+    QZ_HIP(hipGetLastError());
     // positional keys in dep_[1] (free until the first doubling round):
     // the radix sort reads them in its first pass and leaves them intact
     // for the fused first doubling round
@@ -1816,7 +1882,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
     QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
-                                   32, bufB_, tmp_, s, &alt, true, kpos));
+                                   32, bufB_, tmp_, s, &alt, false, kpos));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     uint32_t* keys = alt ? keys1_ : keys0_;
     uint32_t* vals = alt ? vals1_ : vals0_;
@@ -1900,6 +1966,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     const int lkey_bits = bitlen(maxb);  // block-local large-group keys are <= block size
     uint32_t h = 0;  // the doubling depth of packed keys: k0, read back with the first counts
     bool rle = false;  // run-length keys: every group carries its own depth (dep_)
+    uint32_t cb = 0;   // SA entries in char mode (4) or global positions (0)
     int cur = 0;
     bool large_timed = false;
     int large_bits = 8;
@@ -1912,11 +1979,12 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         const int lo = cur ? kOverflow : 0, hi = (cur || rounds_ == 1) ? kCntB + kClasses : kOverflow + 1;
         QZ_HIP(hipMemcpyAsync(hc, cnt_ + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToHost, s));
         if (rounds_ == 1)
-            QZ_HIP(hipMemcpyAsync(hc + 64, &((const PackParams*)ppd_)->k0, 8, hipMemcpyDeviceToHost, s));  // k0, rle
+            QZ_HIP(hipMemcpyAsync(hc + 64, &((const PackParams*)ppd_)->k0, 12, hipMemcpyDeviceToHost, s));  // k0, rle, cb
         QZ_HIP(hipStreamSynchronize(s));
         if (rounds_ == 1) {
             h = ((volatile uint32_t*)hc)[64];
             rle = ((volatile uint32_t*)hc)[65] != 0;
+            cb = ((volatile uint32_t*)hc)[66];
             k0_ = h;
         }
         const int cofs = (cur ? kCntB : 0) - lo;
@@ -2040,38 +2108,38 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         // phase A: gather every key of the round before any rank changes
         if (small_waves) {
             hipLaunchKernelGGL(k_gather_small, dim3(small_blocks), dim3(kThreads), occ_lds(kOccGatherSmall), s, cl,
-                               plan, sa_, rank_, bmap_, h, dep_in, bufB_);
+                               plan, sa_, rank_, bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
             hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave,
-                               sa_, rank_, bmap_, h, dep_in, bufB_);
+                               sa_, rank_, bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
             hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, sa_,
-                               rank_, bmap_, h, dep_in, bufB_);
+                               rank_, bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
             hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, sa_, rank_,
-                               bmap_, h, dep_in, lk0, vals0_);
+                               bmap_, cb, h, dep_in, lk0, vals0_);
             QZ_HIP(hipGetLastError());
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
         if (small_waves) {
             hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), occ_lds(kOccSortSmall), s, cl, plan,
-                               bufB_, sa_, rank_, N, dep_in, dep_out);
+                               bufB_, sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
             hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave, bufB_,
-                               sa_, rank_, N, dep_in, dep_out);
+                               sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
             hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, bufB_,
-                               sa_, rank_, N, dep_in, dep_out);
+                               sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
@@ -2091,7 +2159,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
             hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), occ_lds(kOccFinalLarge), s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
-                               bufA_, sa_, rank_, N, bmap_, dep_in, dep_out);
+                               bufA_, sa_, rank_, N, bmap_, dep_in, dep_out, cb);
             QZ_HIP(hipGetLastError());
         }
         ++rounds_;
