@@ -420,11 +420,18 @@ __device__ __forceinline__ void store_keys(uint32_t* __restrict__ keys, uint32_t
             if (i0 + q < n) keys[i0 + q] = out[q];
     }
 }
+// The SA entry of position i of the block starting at bo; prev is the byte
+// before i (unused at a block start).
+__device__ __forceinline__ uint32_t sa_entry(uint32_t i, uint32_t bo, uint32_t cb, const uint16_t* code, uint8_t prev)
+{
+    return cb ? ((i - bo) << 4) | (i > bo ? (uint32_t)code[prev] : 0u) : i;
+}
+
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
                                                           BlockMap bm,
                                                           const PackParams* __restrict__ ppd,
                                                           const uint32_t* __restrict__ brun,
-                                                          uint32_t* __restrict__ keys)
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     // the workgroup's 4096 positions and the 511 a run-length key may look
     // ahead: W window bytes, W remaining run lengths, W block-start flags;
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
     __shared__ uint16_t code[256];
     __shared__ __align__(16) uint8_t smem[kSmem];
     code[threadIdx.x] = ppd->code[threadIdx.x];
-    const uint32_t radix = ppd->radix, k0 = ppd->k0;
+    const uint32_t radix = ppd->radix, k0 = ppd->k0, cb = ppd->cb;
     __syncthreads();
     const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kKeyItems;
     if (ppd->rle) {  // uniform: the whole grid takes this branch or none
@@ -484,18 +491,21 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         }
         __syncthreads();
         if (i0 >= n) return;
-        uint32_t end = 0, out[kKeyItems];
+        uint32_t end = 0, bo = 0, out[kKeyItems], vo[kKeyItems];
         bool rleb = false;
 #pragma unroll
         for (int q = 0; q < kKeyItems; ++q) {
             const uint32_t i = i0 + q;
             out[q] = 0;
+            vo[q] = 0;
             if (i < n) {
                 if (i >= end) {
                     const uint32_t b = bm.find(i);
+                    bo = bm.off[b];
                     end = bm.off[b + 1];
                     rleb = block_rle(pp, bm, brun, b);
                 }
+                vo[q] = sa_entry(i, bo, cb, code, i > bo ? (i - 1 >= base ? win[i - 1 - base] : text[i - 1]) : 0);
                 if (!rleb) {  // a packed key from the window (k0 <= 32 < kRleLook)
                     uint32_t key = 0;
                     for (uint32_t j = 0; j < k0; ++j) {
@@ -530,6 +540,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
             }
         }
         store_keys(keys, i0, n, out);
+        store_keys(vals, i0, n, vo);
         return;
     }
     // packed keys: the workgroup stages its text window in LDS (16-byte
@@ -556,14 +567,17 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
     if (i0 < n) {
         uint32_t top = 1;  // weight of the leading code
         for (uint32_t j = 1; j < k0; ++j) top *= radix;
-        uint32_t end = 0, key = 0;
-        uint32_t out[kKeyItems];
+        uint32_t end = 0, bo = 0, key = 0;
+        uint32_t out[kKeyItems], vo[kKeyItems];
 #pragma unroll
         for (int q = 0; q < kKeyItems; ++q) {
             const uint32_t i = i0 + q;
+            vo[q] = 0;
             if (i < n) {
                 if (i >= end) {  // first position or a new block: full key
-                    end = bm.off[bm.find(i) + 1];
+                    const uint32_t b = bm.find(i);
+                    bo = bm.off[b];
+                    end = bm.off[b + 1];
                     key = 0;
                     for (uint32_t j = 0; j < k0; ++j) {
                         const uint32_t p = i + j;
@@ -574,9 +588,11 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
                     const uint32_t p = i + k0 - 1;
                     key = (key - lead * top) * radix + ((p < end) ? code[win[p - base]] : 0u);
                 }
+                vo[q] = sa_entry(i, bo, cb, code, i > bo ? (i - 1 >= base ? win[i - 1 - base] : text[i - 1]) : 0);
             }
             out[q] = key;
         }
+        store_keys(vals, i0, n, vo);
         uint4* dst = reinterpret_cast<uint4*>(kl + threadIdx.x * kKeyItems);
 #pragma unroll
         for (int v = 0; v < kKeyItems / 4; ++v)
@@ -684,38 +700,6 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         pp->segbits = segbits;
         pp->cap = (1u << lb) - 1u;
         pp->force = force;
-    }
-}
-
-// The values round 0's radix sort carries: every position's SA entry (char
-// mode: block-local position << 4 | code of the byte before it; else the
-// global position), four consecutive positions per thread, stored as one
-// 16-byte vector when whole.
-__global__ __launch_bounds__(kThreads) void k_init_vals(const uint8_t* __restrict__ text, uint32_t n, BlockMap bm,
-                                                        const PackParams* __restrict__ ppd, uint32_t* __restrict__ vals)
-{
-    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * 4u;
-    if (i0 >= n) return;
-    const uint32_t cb = ppd->cb;
-    uint32_t b = bm.find(i0);
-    uint32_t o = bm.off[b], e = bm.off[b + 1];
-    uint32_t v[4];
-#pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-        const uint32_t i = i0 + q;
-        v[q] = 0;
-        if (i >= n) continue;
-        if (i >= e) {
-            ++b;
-            o = e;
-            e = bm.off[b + 1];
-        }
-        v[q] = cb ? ((i - o) << 4) | (i > o ? (uint32_t)ppd->code[text[i - 1]] : 0u) : i;
-    }
-    if (i0 + 4u <= n) {
-        *reinterpret_cast<uint4*>(vals + i0) = make_uint4(v[0], v[1], v[2], v[3]);
-    } else {
-        for (uint32_t q = 0; i0 + q < n; ++q) vals[i0 + q] = v[q];
     }
 }
 
@@ -1564,12 +1548,11 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict_
                                                       BlockMap bm,
                                                       uint32_t n, uint8_t* __restrict__ out)
 {
-    // code -> byte for the 4-bit forms (k_pack_nib ran with the same test)
+    if (ppd->cb) return;  // uniform: char mode is k_bwt_out_chars
+    // code -> byte for the packed form (k_pack_nib ran with the same test)
     __shared__ uint8_t dec[16];
-    const uint32_t cb = ppd->cb;
-    const bool small = ppd->radix <= 16u;
-    const bool packed = small && !cb;
-    if (small) {
+    const bool packed = ppd->radix <= 16u;
+    if (packed) {
         const uint32_t cd = ppd->code[threadIdx.x];
         if (cd != 0u && cd < 16u) dec[cd] = (uint8_t)threadIdx.x;
     }
@@ -1580,12 +1563,72 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out(const uint8_t* __restrict_
     const uint32_t o = bm.off[b], e = bm.off[b + 1];
     const uint32_t local = j - o;
     const uint32_t p = rank[o] - o + 1u;
-    if (cb) {  // char mode: the symbol travelled with the suffix (a streaming pass)
-        out[j] = local == 0 ? text[e - 1u] : dec[(local < p ? sa[j - 1] : sa[j]) & 15u];
-        return;
-    }
     const uint32_t x = local == 0 ? e - 1u : (local < p ? sa[j - 1] : sa[j]) - 1u;
     out[j] = packed ? dec[(nib[x >> 3] >> (4u * (x & 7u))) & 15u] : text[x];
+}
+
+// The BWT in char mode: every SA entry holds the code of the symbol before
+// its suffix, so the output is one streaming pass, 16 slots per thread (16-byte
+// loads of entries, one 16-byte store when the output is aligned).
+constexpr uint32_t kOutItems = 16;
+__global__ __launch_bounds__(kThreads) void k_bwt_out_chars(const uint8_t* __restrict__ text,
+                                                            const PackParams* __restrict__ ppd,
+                                                            const uint32_t* __restrict__ sa,
+                                                            const uint32_t* __restrict__ rank, BlockMap bm, uint32_t n,
+                                                            uint8_t* __restrict__ out)
+{
+    if (!ppd->cb) return;  // uniform: k_bwt_out serves the batch
+    __shared__ uint8_t dec[16];
+    {
+        const uint32_t cd = ppd->code[threadIdx.x];
+        if (cd != 0u && cd < 16u) dec[cd] = (uint8_t)threadIdx.x;
+    }
+    __syncthreads();
+    const uint32_t j0 = (blockIdx.x * kThreads + threadIdx.x) * kOutItems;
+    if (j0 >= n) return;
+    uint32_t b = bm.find(j0);
+    uint32_t o = bm.off[b], e = bm.off[b + 1];
+    uint32_t p = rank[o] - o + 1u;  // the block's primary index
+    uint32_t prev = j0 > o ? sa[j0 - 1] : 0u;
+    uint32_t ent[kOutItems];
+    if (j0 + kOutItems <= n) {
+        const uint4* v = reinterpret_cast<const uint4*>(sa + j0);
+#pragma unroll
+        for (uint32_t k = 0; k < kOutItems / 4; ++k) {
+            const uint4 x = v[k];
+            ent[4 * k] = x.x;
+            ent[4 * k + 1] = x.y;
+            ent[4 * k + 2] = x.z;
+            ent[4 * k + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < kOutItems; ++q) ent[q] = j0 + q < n ? sa[j0 + q] : 0u;
+    }
+    uint32_t w[kOutItems / 4] = {0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t q = 0; q < kOutItems; ++q) {
+        const uint32_t j = j0 + q;
+        uint32_t c = 0;
+        if (j < n) {
+            if (j >= e) {  // a new block starts inside this thread's slots
+                ++b;
+                o = e;
+                e = bm.off[b + 1];
+                p = rank[o] - o + 1u;
+            }
+            const uint32_t local = j - o;
+            c = local == 0 ? text[e - 1u] : dec[(local < p ? prev : ent[q]) & 15u];
+            prev = ent[q];
+        }
+        w[q / 4] |= c << (8 * (q % 4));
+    }
+    uint8_t* dst = out + j0;
+    if (j0 + kOutItems <= n && ((uintptr_t)dst & 15u) == 0) {
+        *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else {
+        for (uint32_t q = 0; q < kOutItems && j0 + q < n; ++q) dst[q] = (uint8_t)(w[q / 4] >> (8 * (q % 4)));
+    }
 }
 
 // Per block: primary index and sampled indexes I[k] = 1 + rank(k*r).
@@ -1868,16 +1911,12 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force(), bmap_,
                        (const uint32_t*)brun_, small_blocks);
     QZ_HIP(hipGetLastError());
-    // the SA entries round 0 carries (char mode or positions) in vals0
-    hipLaunchKernelGGL(k_init_vals, dim3(qz_div_up(n, kThreads * 4u)), dim3(kThreads), 0, s, text, n, bmap_,
-                       (const PackParams*)ppd_, vals0_);
-    QZ_HIP(hipGetLastError());
     // positional keys in dep_[1] (free until the first doubling round):
     // the radix sort reads them in its first pass and leaves them intact
     // for the fused first doubling round
     uint32_t* kpos = dep_[1];
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
-                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos);
+                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos, vals0_);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
@@ -2183,6 +2222,9 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text,
                        (const uint32_t*)keys0_, (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
+    QZ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bwt_out_chars, dim3(qz_div_up(n, kThreads * kOutItems)), dim3(kThreads), 0, s, text,
+                       (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
     QZ_HIP(hipGetLastError());
