@@ -223,12 +223,12 @@ QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
 
 /* Radix digit-width tuner (no counterpart in the reference): which = 0 for
  * the round-0 suffix sort (candidates 8 and 11 bits), 1 for the sort of
- * groups larger than 4096 (8, 9 and 11 bits). The first sorts of >= 4M
+ * groups larger than 4096 (8 and 9 bits). The first sorts of >= 4M
  * elements rotate through the candidates, timed with HIP events; then the
  * fastest per element is kept for the process (QZBSC_RADIX_BITS=8|9|11
- * forces one). Writes the choice (0 = undecided) and the mean ns per element
- * of 8-, 9- and 11-bit digits into ns_per_elem[3]; returns the number of
- * trials. */
+ * forces one; the large-group sort takes at most 9). Writes the choice
+ * (0 = undecided) and the mean ns per element of 8-, 9- and 11-bit digits
+ * into ns_per_elem[3]; returns the number of trials. */
 QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns_per_elem);
 
 /* HIP graph replay counts of this process (no reference counterpart).

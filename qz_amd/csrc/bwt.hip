@@ -765,8 +765,7 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
                                                      const TileDesc* __restrict__ tiles,
                                                      const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
                                                      uint32_t* __restrict__ rank, GroupLists gl,
-                                                     const PackParams* __restrict__ ppd, uint32_t* __restrict__ dep,
-                                                     BlockMap bm, const uint32_t* __restrict__ brun, uint32_t fuse)
+                                                     const PackParams* __restrict__ ppd, BlockMap bm, uint32_t fuse)
 {
     if (fuse_round1(*ppd, fuse)) return;  // uniform: k_final0f finishes this batch
     constexpr uint32_t Q = kSortTile / (kThreads / QZ_WAVE);  // 1024 slots per wave
@@ -788,8 +787,6 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
     uint32_t carry = pre[blockIdx.x];
     for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
     const bool last_tile = td.tlocal + 1 == td.ntiles;
-    const bool track = ppd->rle != 0;  // groups carry depths
-    const bool rleb = track && block_rle(*ppd, bm, brun, td.block);
     const uint32_t cb = ppd->cb, bo = bm.off[td.block];
     for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
         const uint32_t i = b + lane;
@@ -809,13 +806,24 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
             sa[j] = s;
             rank[sa_pos(s, cb, bo)] = g;
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
-            if (end && j > g) {
-                stage_push(S, g, j - g + 1);
-                if (track) dep[g] = key_depth(k, *ppd, rleb);
-            }
+            if (end && j > g) stage_push(S, g, j - g + 1);
         }
     }
     stage_flush(S, gl);
+}
+
+// Run-length keys: the depth of every group round 0 left (set A, pool
+// entries [0, total)), from its round-0 key, which the sorted keys still
+// hold at the group's start slot.
+__global__ __launch_bounds__(kThreads) void k_init_depths(const Grp* __restrict__ pool, uint32_t total,
+                                                          const uint32_t* __restrict__ keys,
+                                                          const PackParams* __restrict__ ppd, BlockMap bm,
+                                                          const uint32_t* __restrict__ brun, uint32_t* __restrict__ dep)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t g = pool[i].start;
+    dep[g] = key_depth(keys[g], *ppd, block_rle(*ppd, bm, brun, bm.find(g)));
 }
 
 // ---- round 0 finish with the first doubling round fused in ----------------
@@ -1419,40 +1427,35 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
 }
 
 // ---- large groups (> 4096): block-segmented radix sort of key2, one -----
-// segment per group (tiles never span two groups), flat index j in
-// [lbase[g], lbase[g+1]) standing for slot list[g].start + j - lbase[g].
+// segment per group. The groups of a round are taken in slot order and
+// sorted where they lie: keys at their own SA slots in one buffer, the SA
+// entries in sa, the other side of the ping-pong slot-indexed as well (tiles
+// never span two groups; td.block is the group's index in lg, td.start a
+// slot).
 __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restrict__ tiles,
-                                                         const Grp* __restrict__ list,
-                                                         const uint32_t* __restrict__ lbase,
-                                                         const uint32_t* __restrict__ sa,
-                                                         const uint32_t* __restrict__ rank,
-                                                         BlockMap bm, uint32_t cb, uint32_t h,
-                                                         const uint32_t* __restrict__ dep, uint32_t* __restrict__ keys,
-                                                         uint32_t* __restrict__ vals)
+                                                         const Grp* __restrict__ lg, const uint32_t* __restrict__ sa,
+                                                         const uint32_t* __restrict__ rank, BlockMap bm, uint32_t cb,
+                                                         uint32_t h, const uint32_t* __restrict__ dep,
+                                                         uint32_t* __restrict__ keys)
 {
     const TileDesc td = tiles[blockIdx.x];
-    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
+    const uint32_t g0 = lg[td.block].start;
     // a group lies in one block, whose text positions and SA slots are the
     // same range: keys relative to its start need only bits(block size)
     const uint32_t blk = bm.find(g0);
     const uint32_t bstart = bm.off[blk], bend = bm.off[blk + 1];
     const uint32_t d = dep ? dep[g0] : h;
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
-        const uint32_t j = td.start + i;
-        const uint32_t s = sa[g0 + (j - b0)];
-        const uint64_t p = (uint64_t)sa_pos(s, cb, bstart) + d;
-        keys[j] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
-        vals[j] = s;
+        const uint32_t slot = td.start + i;
+        const uint64_t p = (uint64_t)sa_pos(sa[slot], cb, bstart) + d;
+        keys[slot] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
     }
 }
 
-// Large-group finish, one radix tile per workgroup (tiles never span two
-// groups; td.block is the group's index in the list). Same scheme as
-// k_final0 over the sorted keys array, whose index j maps to SA slot
-// g0 + (j - b0) of group g0 = list[td.block].start, b0 = lbase[td.block].
+// Large-group finish, one radix tile per workgroup, over the sorted keys and
+// entries at the groups' own slots. Same scheme as k_final0.
 __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __restrict__ tiles,
-                                                          const Grp* __restrict__ list,
-                                                          const uint32_t* __restrict__ lbase,
+                                                          const Grp* __restrict__ lg,
                                                           const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ vals,
                                                           const uint32_t* __restrict__ pre, uint32_t* __restrict__ sa,
@@ -1465,8 +1468,8 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
     __shared__ uint32_t qlast[kThreads / QZ_WAVE];
     stage_init(S);
     const TileDesc td = tiles[blockIdx.x];
-    const uint32_t g0 = list[td.block].start, b0 = lbase[td.block];
-    const uint32_t bstart = (dep_out || cb) ? bm.off[bm.find(g0)] : 0u;  // keys are block-local: rank + 1 - bstart
+    const uint32_t g0 = lg[td.block].start;
+    const uint32_t bstart = bm.off[bm.find(g0)];  // keys are block-local: rank + 1 - bstart
     const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
     const uint32_t qb = w * Q;
     uint32_t m = 0;
@@ -1496,14 +1499,12 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
         carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
         if (valid) {
             const uint32_t s = vals[j];
-            const uint32_t slot = g0 + (j - b0);
-            const uint32_t hs = g0 + (g - b0);  // slot of the subgroup head
-            sa[slot] = s;
-            if (hs != g0) rank[sa_pos(s, cb, bstart)] = hs;  // the first subgroup keeps the group's start
+            sa[j] = s;
+            if (g != g0) rank[sa_pos(s, cb, bstart)] = g;  // the first subgroup keeps the group's start
             const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
-            if (end && slot > hs) {
-                stage_push(S, hs, slot - hs + 1);
-                if (dep_out) dep_out[hs] = sub_depth(dep_in, g0, k ? k - 1u + bstart + 1u : 0u);
+            if (end && j > g) {
+                stage_push(S, g, j - g + 1);
+                if (dep_out) dep_out[g] = sub_depth(dep_in, g0, k ? k - 1u + bstart + 1u : 0u);
             }
         }
     }
@@ -1653,11 +1654,13 @@ __global__ void k_bwt_index(const uint32_t* __restrict__ rank, BlockMap bm,
 // block-local keys need at most 27 bits for blocks of up to 128 MiB (3
 // passes where 8-bit digits take 4; round 0's 32-bit keys gain nothing from
 // it). Every width gives the same order, so the tuner rotates the candidates
-// ({8, 11} for round 0, {8, 9, 11} for large groups) on the first sorts of at
-// least kTuneMin elements, times them with HIP events, and keeps the fastest
-// per element for the process. QZBSC_RADIX_BITS=8|9|11 forces one width for
-// both sorts.
+// ({8, 11} for round 0, {8, 9} for large groups, whose histograms have a
+// buffer sized for 9-bit digits) on the first sorts of at least kTuneMin
+// elements, times them with HIP events, and keeps the fastest per element for
+// the process. QZBSC_RADIX_BITS=8|9|11 forces one width for both sorts (at
+// most 9 for large groups).
 constexpr int kWidths[3] = {8, 9, 11};
+constexpr int kLargeRadixBits = 9;  // the large-group sort's widest digit (its histograms have their own buffer)
 struct RadixTuner {
     std::mutex mu;
     int choice = 0;
@@ -1674,8 +1677,8 @@ static std::once_flag g_tune_once;
 static void tune_init()
 {
     std::call_once(g_tune_once, [] {
-        g_tune[1].ncand = 3;
-        for (int k = 0; k < 3; ++k) g_tune[1].cand[k] = kWidths[k];
+        g_tune[1].ncand = 2;
+        for (int k = 0; k < 2; ++k) g_tune[1].cand[k] = kWidths[k];
     });
 }
 
@@ -1743,16 +1746,20 @@ int tune_state(int which, int* choice, double* ns_per_elem)
 template <typename K>
 static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
                          uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s, bool* alt,
-                         bool iota_vals = false, const K* kin = nullptr)
+                         bool iota_vals = false, const K* kin = nullptr, const int32_t* seg_delta = nullptr)
 {
     if (bits == 11)
         return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
-                                     kin);
+                                     kin, seg_delta);
     if (bits == 9)
         return seg_radix_sort<K, 9>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
-                                    kin);
-    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals, kin);
+                                    kin, seg_delta);
+    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals, kin,
+                                seg_delta);
 }
+
+// Radix passes of a key_bits-bit sort with bits-bit digits.
+static int radix_passes(int key_bits, int bits) { return (key_bits + bits - 1) / bits; }
 
 static int bitlen(uint64_t v)
 {
@@ -1775,11 +1782,24 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     const size_t tmpw = scan_tmp_words(seg_words) + 1024;
     const uint64_t groups = pool_cap(c);
     const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
-    // text, out; sa, rank, keys0/1, vals0/1, dep0/1 (4 B each); bufB (key2 or radix histograms); bufA per tile
-    return 2 * (c + 64) + 8 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
-           3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 +
-           2 * groups * sizeof(Grp) + (c / (kMedMax + 1) + 2) * 4 + 2 * tile_cap * sizeof(TileDesc) +
+    const uint64_t nlg = c / (kMedMax + 1) + 2;  // large groups of a round, at most
+    // text, out; sa, rank, keys0, vals0 (4 B each); dep0/1 (4 B each, allocated for run-length batches
+    // only, counted always); bufB (doubling keys or round-0 histograms); bufA per tile; large-group
+    // histograms (9-bit digits at most), their list, offsets and shifts
+    return 2 * (c + 64) + 4 * 4 * c + 2 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
+           3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 + 2 * groups * sizeof(Grp) +
+           nlg * (sizeof(Grp) + 4) + tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc) +
            sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
+}
+
+// Depth arrays, allocated for the engine's first batch with run-length keys
+// and kept from then on.
+int BwtEngine::ensure_depths()
+{
+    if (dep_[0]) return 0;
+    QZ_HIP(hipMalloc(&dep_[0], cap_ * 4));
+    QZ_HIP(hipMalloc(&dep_[1], cap_ * 4));
+    return 0;
 }
 
 int BwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -1801,11 +1821,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&bufA_, tile_cap * 4));  // last head per radix tile
     QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(c, seg_words) * 4));
     QZ_HIP(hipMalloc(&keys0_, c * 4));
-    QZ_HIP(hipMalloc(&keys1_, c * 4));
     QZ_HIP(hipMalloc(&vals0_, c * 4));
-    QZ_HIP(hipMalloc(&vals1_, c * 4));
-    QZ_HIP(hipMalloc(&dep_[0], c * 4));
-    QZ_HIP(hipMalloc(&dep_[1], c * 4));
+    QZ_HIP(hipMalloc(&lhist_, tile_cap * (1u << kLargeRadixBits) * 4));
     QZ_HIP(hipMalloc(&tmp_, tmpw * 4));
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
@@ -1813,7 +1830,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&brun_, 2 * ((size_t)cb + 1) * 4));
     QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
     for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
-    QZ_HIP(hipMalloc(&lbase_, (c / (kMedMax + 1) + 2) * 4));
+    QZ_HIP(hipMalloc(&lgrp_, (c / (kMedMax + 1) + 2) * sizeof(Grp)));
+    QZ_HIP(hipMalloc(&ldelta_, (c / (kMedMax + 1) + 2) * 4));
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
@@ -1829,16 +1847,19 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
 
 void BwtEngine::release()
 {
-    void* ps[] = {text_, out_, sa_, rank_, bufA_, bufB_, keys0_, keys1_, vals0_, vals1_, dep_[0], dep_[1], tmp_, off_,
-                  idx_, hist_, brun_, cnt_, groups_[0], groups_[1], lbase_, tiles_, tiles0_, ppd_, btab_};
+    void* ps[] = {text_,      out_,       sa_,    rank_,  bufA_,   bufB_, keys0_, vals0_, dep_[0], dep_[1], tmp_,
+                  off_,       idx_,       hist_,  brun_,  cnt_,    groups_[0], groups_[1], lgrp_,   ldelta_,
+                  lhist_,     tiles_,     tiles0_, ppd_,  btab_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
     text_ = out_ = nullptr;
-    sa_ = rank_ = bufA_ = bufB_ = vals0_ = vals1_ = tmp_ = off_ = hist_ = brun_ = cnt_ = lbase_ = nullptr;
-    keys0_ = keys1_ = nullptr;
+    sa_ = rank_ = bufA_ = bufB_ = vals0_ = tmp_ = off_ = hist_ = brun_ = cnt_ = lhist_ = nullptr;
+    keys0_ = nullptr;
     dep_[0] = dep_[1] = nullptr;
+    lgrp_ = nullptr;
+    ldelta_ = nullptr;
     groups_[0] = groups_[1] = nullptr;
     tiles_ = tiles0_ = ppd_ = nullptr;
     btab_ = nullptr;
@@ -1911,20 +1932,30 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force(), bmap_,
                        (const uint32_t*)brun_, small_blocks);
     QZ_HIP(hipGetLastError());
-    // positional keys in dep_[1] (free until the first doubling round):
-    // the radix sort reads them in its first pass and leaves them intact
-    // for the fused first doubling round
-    uint32_t* kpos = dep_[1];
+    // positional keys in the group list buffer (free until the first
+    // doubling round): the radix sort reads them in its first pass and
+    // leaves them intact for the fused first doubling round
+    uint32_t* kpos = (uint32_t*)groups_[1];
+    // the radix ping-pong: (keys0, vals0) and (group pool, sa), both free
+    // until the finish writes sa, ranks and groups; the last pass lands in
+    // (keys0, vals0), so with an odd pass count the SA entries start in sa
+    const bool odd = (radix_passes(32, bits) & 1) != 0;
+    uint32_t* pk = (uint32_t*)groups_[0];
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
-                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos, vals0_);
+                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos, odd ? sa_ : vals0_);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
-    QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, keys1_, vals1_, (const TileDesc*)tiles0_, ntiles, hist_words,
-                                   32, bufB_, tmp_, s, &alt, false, kpos));
+    if (odd)
+        QZ_TRY(seg_sort_bits<uint32_t>(bits, pk, sa_, keys0_, vals0_, (const TileDesc*)tiles0_, ntiles, hist_words, 32,
+                                       bufB_, tmp_, s, &alt, false, kpos));
+    else
+        QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, pk, sa_, (const TileDesc*)tiles0_, ntiles, hist_words, 32,
+                                       bufB_, tmp_, s, &alt, false, kpos));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
-    uint32_t* keys = alt ? keys1_ : keys0_;
-    uint32_t* vals = alt ? vals1_ : vals0_;
+    if (alt != odd) return QZ_GPU_ERROR;  // cannot happen: the last pass writes (keys0, vals0)
+    uint32_t* keys = keys0_;
+    uint32_t* vals = vals0_;
     // group starts: last head per tile, an exclusive max-scan over tiles,
     // then tile-local scans inside the finishing kernel
     hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
@@ -1935,8 +1966,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     // one of the two finishes the batch (the other returns at once): the
     // choice depends on the key mode and alphabet, decided on the device
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
-                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, glA, (const PackParams*)ppd_, dep_[0], bmap_,
-                       (const uint32_t*)brun_, fuse);
+                       (const TileDesc*)tiles0_, bufA_, sa_, rank_, glA, (const PackParams*)ppd_, bmap_, fuse);
     QZ_HIP(hipGetLastError());
     if (fuse) {
 #define QZ_FINAL0F(T, SVL)                                                                                      \
@@ -2073,6 +2103,14 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         const GroupLists& C = gls[cur];
         const GroupLists& N = gls[cur ^ 1];
+        if (rounds_ == 1 && rle) {
+            // run-length keys: per-group depths, from round 0's groups (set A)
+            QZ_TRY(ensure_depths());
+            hipLaunchKernelGGL(k_init_depths, dim3(qz_div_up(total, kThreads)), dim3(kThreads), 0, s,
+                               (const Grp*)C.pool, (uint32_t)total, keys0_, (const PackParams*)ppd_, bmap_,
+                               (const uint32_t*)brun_, dep_[0]);
+            QZ_HIP(hipGetLastError());
+        }
         // the last round's groups: pool -> list ordered by class
         CurLists cl;
         {
@@ -2104,7 +2142,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         const uint32_t nmed = counts[kClsMed];
         const uint32_t nlarge = counts[kClsLarge];
 
-        // large groups: host-side prefix of sizes and per-group tiles
+        // large groups: in slot order, host-side prefix of sizes, per-group
+        // tiles at the groups' own slots
         uint32_t M = 0, lt = 0, lhist = 0;
         int lbits = 8;
         std::vector<Grp> lg;
@@ -2113,6 +2152,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             lg.resize(nlarge);
             QZ_HIP(hipMemcpyAsync(lg.data(), cl.list[kClsLarge], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
             QZ_HIP(hipStreamSynchronize(s));
+            std::sort(lg.begin(), lg.end(), [](const Grp& a, const Grp& b) { return a.start < b.start; });
             lb.resize(nlarge + 1);
             for (uint32_t i = 0; i < nlarge; ++i) {
                 lb[i] = M;
@@ -2131,14 +2171,19 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                         mx, (unsigned long long)sz[0], (unsigned long long)sz[1], (unsigned long long)sz[2],
                         (unsigned long long)sz[3], (unsigned long long)sz[4]);
             }
-            lbits = tune_pick(1, M);
+            lbits = std::min(tune_pick(1, M), kLargeRadixBits);
             lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt, 1u << lbits);
-            QZ_HIP(hipMemcpyAsync(lbase_, lb.data(), (nlarge + 1) * 4, hipMemcpyHostToDevice, s));
+            // tiles at the groups' slots; the sort writes each group shifted
+            // from its packed offset to its slot start
+            std::vector<int32_t> delta(nlarge);
+            for (uint32_t i = 0; i < nlarge; ++i) delta[i] = (int32_t)(lg[i].start - lb[i]);
+            for (uint32_t t = 0; t < lt; ++t) tiles_host_[t].start += (uint32_t)delta[tiles_host_[t].block];
+            QZ_HIP(hipMemcpyAsync(lgrp_, lg.data(), nlarge * sizeof(Grp), hipMemcpyHostToDevice, s));
+            QZ_HIP(hipMemcpyAsync(ldelta_, delta.data(), nlarge * 4, hipMemcpyHostToDevice, s));
             QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+            QZ_HIP(hipStreamSynchronize(s));  // delta and lg leave scope before the copies would be read otherwise
         }
         const TileDesc* ltiles = (const TileDesc*)tiles_;
-        uint32_t* lk0 = keys0_;
-        uint32_t* lk1 = keys1_;
         // group depths: read from this round's buffer (written when the
         // groups were emitted), new subgroups' depths go to the other one
         const uint32_t* dep_in = rle ? dep_[cur] : nullptr;
@@ -2161,8 +2206,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (M) {
-            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, cl.list[kClsLarge], lbase_, sa_, rank_,
-                               bmap_, cb, h, dep_in, lk0, vals0_);
+            hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, (const Grp*)lgrp_, sa_, rank_,
+                               bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
@@ -2184,21 +2229,21 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         if (M) {
             QZ_HIP(hipEventRecord(tev_[2], s));
             bool alt = false;
-            // histograms in bufB: this round's keys there are consumed by the
-            // small/wave/medium sorts, which precede this sort on the stream
-            QZ_TRY(seg_sort_bits<uint32_t>(lbits, lk0, vals0_, lk1, vals1_, ltiles, lt, lhist, lkey_bits, bufB_, tmp_,
-                                           s, &alt));
+            // keys at the groups' slots in bufB (beside the other classes'
+            // keys), entries in sa; the other side is (keys0, vals0)
+            QZ_TRY(seg_sort_bits<uint32_t>(lbits, bufB_, sa_, keys0_, vals0_, ltiles, lt, lhist, lkey_bits, lhist_, tmp_,
+                                           s, &alt, false, nullptr, ldelta_));
             QZ_HIP(hipEventRecord(tev_[3], s));
             large_timed = true;
             large_bits = lbits;
             large_n = M;
-            uint32_t* keys = alt ? lk1 : lk0;
-            uint32_t* vals = alt ? vals1_ : vals0_;
+            const uint32_t* keys = alt ? keys0_ : bufB_;
+            const uint32_t* vals = alt ? vals0_ : sa_;
             hipLaunchKernelGGL(k_tile_lasthead, dim3(lt), dim3(kThreads), 0, s, keys, ltiles, bufA_);
             QZ_HIP(hipGetLastError());
             QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, lt, tmp_, s)));
-            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), occ_lds(kOccFinalLarge), s, ltiles, cl.list[kClsLarge], lbase_, keys, vals,
-                               bufA_, sa_, rank_, N, bmap_, dep_in, dep_out, cb);
+            hipLaunchKernelGGL(k_final_large, dim3(lt), dim3(kThreads), occ_lds(kOccFinalLarge), s, ltiles,
+                               (const Grp*)lgrp_, keys, vals, bufA_, sa_, rank_, N, bmap_, dep_in, dep_out, cb);
             QZ_HIP(hipGetLastError());
         }
         ++rounds_;

@@ -41,17 +41,19 @@ private:
     GraphCache g0_;                // round 0 as one graph
     uint32_t* bufA_ = nullptr;
     uint32_t* bufB_ = nullptr;
-    uint32_t* keys0_ = nullptr;
-    uint32_t* keys1_ = nullptr;
+    uint32_t* keys0_ = nullptr;    // radix keys, one side of the ping-pong (the other: group pool / sa)
     uint32_t* vals0_ = nullptr;
-    uint32_t* vals1_ = nullptr;
-    uint32_t* dep_[2] = {nullptr, nullptr};  // depth of each unsorted group at its start slot, per round parity
+    uint32_t* dep_[2] = {nullptr, nullptr};  // depth of each unsorted group at its start slot, per round
+                                             // parity; allocated for the first batch with run-length keys
+    uint32_t* lhist_ = nullptr;    // histograms of the large-group sort
+    void* lgrp_ = nullptr;         // large groups of a round in slot order (device)
+    int32_t* ldelta_ = nullptr;    // per large group: its slot start minus its packed offset
+    int ensure_depths();
     uint32_t* tmp_ = nullptr;
     uint32_t* hist_ = nullptr;
     uint32_t* brun_ = nullptr;     // run boundaries per block (round-0 key mode)
     uint32_t* cnt_ = nullptr;      // group-list counters (two sets) + overflow flag
     void* groups_[2] = {nullptr, nullptr};
-    uint32_t* lbase_ = nullptr;    // prefix of large-group sizes
     void* tiles_ = nullptr;        // block-aligned radix tiles (device)
     std::vector<TileDesc> tiles_host_;
   public:

@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
                                                               const uint32_t* __restrict__ vin,
                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                               const TileDesc* __restrict__ tiles, int shift,
-                                                              const uint32_t* __restrict__ offs)
+                                                              const uint32_t* __restrict__ offs,
+                                                              const int32_t* __restrict__ seg_delta)
 {
     constexpr uint32_t D = 1u << BITS;
     constexpr uint32_t R = D / kSortThreads;  // digits per thread in the prefix (1 or 8)
@@ -215,10 +216,13 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     __shared__ uint32_t wtot[W];
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
+    // seg_delta (optional): per segment, where its output lies relative to
+    // the packed order the histograms count in
+    const uint32_t delta = seg_delta ? (uint32_t)seg_delta[td.block] : 0u;
     for (uint32_t d = tid; d < D; d += kSortThreads) {
 #pragma unroll
         for (int w = 0; w < W; ++w) wcnt[w][d] = 0;
-        gbase[d] = offs[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal];
+        gbase[d] = offs[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] + delta;
     }
     __syncthreads();
     K kk[IT];
@@ -299,10 +303,14 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 // hist_words, stmp the scan scratch (scan_tmp_words(hist_words)). Result in (k0,v0) for even pass counts.
 // kin (optional): the first pass reads its keys from kin instead of k0,
 // which then serves only as a buffer, so kin is left as it was.
+// seg_delta (optional): segment b's elements are at tile positions (tiles'
+// start) and are sorted into positions shifted by seg_delta[b] from the
+// packed order of the histograms (segments laid out with gaps).
 template <typename K, int BITS>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
                           uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s,
-                          bool* in_alt, bool iota_vals = false, const K* kin = nullptr)
+                          bool* in_alt, bool iota_vals = false, const K* kin = nullptr,
+                          const int32_t* seg_delta = nullptr)
 {
     *in_alt = false;
     K *ka = k0, *kb = k1;
@@ -316,7 +324,7 @@ static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDe
         // iota_vals: the first pass takes each key's position as its value
         const uint32_t* vin = (iota_vals && passes == 0) ? nullptr : va;
         hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, vin, kb, vb, tiles,
-                           sh, hist);
+                           sh, hist, seg_delta);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
         uint32_t* tv = va; va = vb; vb = tv;

@@ -209,3 +209,30 @@ def test_device_entry_at_unaligned_pointers(gpu):
             finally:
                 for d in bufs:
                     hip.hipFree(d)
+
+
+def test_forward_workspace_per_input_byte(gpu):
+    """The forward workspace measured on the device (free memory before and
+    after the first call of a trimmed pool): at most 32 bytes per input byte
+    for reads (packed keys), 40 with run-length keys (qualities: the depth
+    arrays are allocated for such batches only); never above the footprint
+    the admission model charges, up to 16 MiB of allocation rounding over
+    the workspace's two dozen buffers. The result is still the oracle's."""
+    from qz_amd import serve
+    lib = gpu.lib()
+    n = 64 << 20
+    cases = [("reads", synth.reads_50mbp_stream(500_000, seed=9)[:n], 32.0),
+             ("qualities", synth.quality_stream(450_000, 150, 9)[:n], 40.0)]
+    for name, blk, limit in cases:
+        lib.qzbsc_trim()
+        free0, _ = serve.device_memory()
+        out, prim, _ = gpu.bwt.encode_batch([blk])
+        free1, _ = serve.device_memory()
+        per = (free0 - free1) / blk.size
+        model = serve.footprint(0, blk.size, 1) / blk.size
+        print(f"{name}: {per:.2f} B per input byte resident, model {model:.2f}", flush=True)
+        assert per <= limit and free0 - free1 <= serve.footprint(0, blk.size, 1) + (16 << 20)
+        if name == "reads":
+            u, p, _ = O.bwt_aux(blk[:4 << 20])
+            o2, p2, _ = gpu.bwt.encode_batch([blk[:4 << 20]])
+            assert np.array_equal(o2[0], u) and p2[0] == p
