@@ -110,6 +110,8 @@ private:
     uint32_t* flag_ = nullptr;
     uint8_t* buf_ = nullptr;       // first symbols of every segment
     uint32_t* resume_ = nullptr;   // row after those symbols
+    uint32_t* rbase_ = nullptr;    // per block: its first ruler of the list ranking
+    uint32_t* rul_ = nullptr;      // rulers' next / rows / distances (five arrays)
     void* pinned_ = nullptr;
     uint64_t cap_tiles_ = 0;
     uint64_t cap_split_ = 0;
@@ -117,6 +119,7 @@ private:
     GraphCache graph_;             // the whole launch sequence as one graph
     std::vector<TileDesc> td_host_;
     std::vector<uint32_t> sb_host_;
+    std::vector<uint32_t> rb_host_;
 };
 
 }  // namespace qz

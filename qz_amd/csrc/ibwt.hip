@@ -77,7 +77,12 @@ __global__ __launch_bounds__(kIThreads) void k_ib_hist(const uint8_t* __restrict
 }
 
 // hist has been exclusive-scanned: entry = off[b] + #(< c in block) + #(c in
-// earlier tiles of the block).
+// earlier tiles of the block). Each wave ranks its own 1024 consecutive
+// symbols (16 steps of 64) against wave-private counters with one ballot per
+// symbol bit, so ranking needs no workgroup barrier; one barrier then turns
+// the four waves' counts into each wave's base per symbol, and the row
+// entries are written. Stable: order within a tile is wave, then step, then
+// lane, which is BWT index order.
 __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict__ U,
                                                         const TileDesc* __restrict__ tiles,
                                                         const uint32_t* __restrict__ hist,
@@ -86,21 +91,24 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
                                                         uint32_t split, uint32_t* __restrict__ E)
 {
     constexpr int W = kIThreads / QZ_WAVE;
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wh[W][256];
+    constexpr int IT = kSortTile / kIThreads;  // 16 steps of 64 per wave
+    __shared__ uint32_t cnt[W][256];
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
-    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE;
+    const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = lane_id();
     const uint32_t ob = off[td.block];
     const uint32_t p = prim[td.block];
     const uint32_t rb = ob + td.block;  // row base of this block
-    run[tid] = hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] - ob;
-    for (uint32_t it = 0; it < td.len; it += kIThreads) {
-        const uint32_t i = it + tid;
+#pragma unroll
+    for (int w = 0; w < W; ++w) cnt[w][tid] = 0;
+    __syncthreads();
+    uint8_t cs[IT];
+    uint32_t rk[IT];
+    const uint32_t wbase = wave * (IT * QZ_WAVE);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t i = wbase + it * QZ_WAVE + lane;
         const bool valid = i < td.len;
         const uint32_t c = valid ? U[td.start + i] : 0u;
-#pragma unroll
-        for (int w = 0; w < W; ++w) wh[w][tid] = 0;
-        __syncthreads();
         uint64_t peer = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -109,26 +117,34 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
             peer &= bit ? m : ~m;
         }
         const uint32_t below = popc_below(peer);
-        if (valid && below == 0) wh[wave][c] = (uint32_t)__popcll(peer);
-        __syncthreads();
-        {
-            uint32_t r = run[tid];
+        // a wave's LDS reads and writes complete in order: the next step
+        // reads what this one wrote
+        const uint32_t before = cnt[wave][c];
+        if (valid && below == 0) cnt[wave][c] = before + (uint32_t)__popcll(peer);
+        cs[it] = (uint8_t)c;
+        rk[it] = valid ? before + below : 0xffffffffu;
+    }
+    __syncthreads();
+    {
+        // symbol tid: each wave's base = the tile's run + the earlier waves' counts
+        uint32_t r = hist[(size_t)td.hbase + (size_t)tid * td.ntiles + td.tlocal] - ob;
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint32_t t = wh[w][tid];
-                wh[w][tid] = r;
-                r += t;
-            }
-            run[tid] = r;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t t = cnt[w][tid];
+            cnt[w][tid] = r;
+            r += t;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t u = td.start - ob + i;          // local BWT index
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        if (rk[it] != 0xffffffffu) {
+            const uint32_t i = wbase + it * QZ_WAVE + lane;
+            const uint32_t u = td.start - ob + i;  // local BWT index
             const uint32_t row = u < p ? u : u + 1u;
-            const uint32_t lf = 1u + wh[wave][c] + below;  // local row
+            const uint32_t lf = 1u + cnt[wave][cs[it]] + rk[it];  // local row
             E[rb + row] = (row % split == 0) ? (lf | kSplitFlag) : lf;
         }
-        __syncthreads();
     }
 }
 
@@ -340,6 +356,71 @@ __global__ __launch_bounds__(kIThreads) void k_ib_jump(const uint32_t* __restric
     if (n2 != kNil) *any = 1u;
 }
 
+// Phase 3 by a ruling set: every kRule-th splitter of a block (its first,
+// the head of the block's chain, included) is a ruler. Each ruler walks its
+// chain to the next ruler, recording for every splitter it passes its ruler
+// (owner) and the rows from the ruler to that splitter (pre); rnxt / rlen
+// hold the next ruler (kNil at the chain end) and the rows up to it. Pointer
+// jumping then ranks the rulers alone, and every splitter's distance follows
+// from its ruler's. Rulers are numbered per block: block b's lie at
+// rbase[b] .. rbase[b + 1).
+constexpr uint32_t kRule = 16;
+
+__global__ __launch_bounds__(kIThreads) void k_ib_rule_walk(const uint32_t* __restrict__ nxt,
+                                                             const uint32_t* __restrict__ len,
+                                                             const uint32_t* __restrict__ sbase,
+                                                             const uint32_t* __restrict__ rbase, uint32_t nblk,
+                                                             uint32_t nrule, uint32_t* __restrict__ owner,
+                                                             uint32_t* __restrict__ pre, uint32_t* __restrict__ rnxt,
+                                                             uint32_t* __restrict__ rlen)
+{
+    const uint32_t q = blockIdx.x * kIThreads + threadIdx.x;
+    if (q >= nrule) return;
+    const uint32_t b = find_block_u32(rbase, nblk, q);
+    const uint32_t sb = sbase[b], r0 = rbase[b];
+    const uint32_t sid = sb + (q - r0) * kRule;
+    uint32_t acc = len[sid];
+    uint32_t t = nxt[sid];
+    // chains stay inside their block, and every chain ends (kNil)
+    while (t != kNil && (t - sb) % kRule != 0) {
+        owner[t] = q;
+        pre[t] = acc;
+        acc += len[t];
+        t = nxt[t];
+    }
+    rnxt[q] = (t == kNil) ? kNil : r0 + (t - sb) / kRule;
+    rlen[q] = acc;
+}
+
+// dist[s]: the rows from splitter s to the end of its chain, from its ruler's.
+// A valid block's LF is one cycle through row 0, so every splitter is on its
+// block's chain and has a ruler; a BWT whose LF splits into several cycles
+// (a corrupt block) leaves some splitters unvisited, and the owner check
+// keeps their reads inside the ruler arrays (the writes that follow clamp
+// their positions to the block).
+__global__ __launch_bounds__(kIThreads) void k_ib_rule_apply(const uint32_t* __restrict__ sbase,
+                                                              const uint32_t* __restrict__ rbase, uint32_t nblk,
+                                                              uint32_t nsplit, uint32_t nrule,
+                                                              const uint32_t* __restrict__ owner,
+                                                              const uint32_t* __restrict__ pre,
+                                                              const uint32_t* __restrict__ rdist,
+                                                              uint32_t* __restrict__ dist)
+{
+    const uint32_t s = blockIdx.x * kIThreads + threadIdx.x;
+    if (s >= nsplit) return;
+    const uint32_t b = find_block_u32(sbase, nblk, s);
+    const uint32_t local = s - sbase[b];
+    if (local % kRule == 0) {
+        dist[s] = rdist[rbase[b] + local / kRule];
+    } else {
+        const uint32_t o = owner[s];
+        dist[s] = o < nrule ? rdist[o] - pre[s] : 0u;
+    }
+}
+
+// Rulers of a batch: block b has its splitters / kRule, rounded up.
+static inline uint64_t rule_cap(uint64_t nsplit, uint32_t nblk) { return nsplit / kRule + nblk + 2; }
+
 // Phase 4a: place the buffered symbols, eight per thread: step k of a
 // segment starting at chain step t0 writes T[n-1-t0-k]; thread q of a
 // splitter takes steps 8q .. 8q+7 from one 8-byte word of the buffer.
@@ -360,11 +441,33 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
     const uint32_t b = find_block_u32(sbase, nblk, sid);
     const uint32_t ob = off[b], n = off[b + 1] - ob;
     const uint32_t t0 = n + 1u - dist[sid] + k0;
+    if (t0 >= n) return;  // steps past the block's last symbol (the sentinel's)
     const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
-    const uint32_t m = l - k0 < 8u ? l - k0 : 8u;
-    for (uint32_t j = 0; j < m; ++j) {
-        const uint32_t t = t0 + j;
-        if (t < n) T[ob + n - 1u - t] = (uint8_t)(w >> (8 * j));
+    uint32_t m = l - k0 < 8u ? l - k0 : 8u;
+    m = m < n - t0 ? m : n - t0;
+    // step j goes to T[n-1-t0-j]: the m bytes land at [lo, lo + m) in
+    // reverse order; write them with naturally aligned stores of 8, 4, 2
+    // and 1 bytes (at most four)
+    uint64_t r = 0;
+    for (uint32_t j = 0; j < m; ++j) r |= ((w >> (8 * j)) & 0xffull) << (8 * (m - 1u - j));
+    uint8_t* dst = T + ob + n - t0 - m;
+    uint32_t k = 0;
+    while (k < m) {
+        const uintptr_t a = (uintptr_t)(dst + k);
+        const uint32_t left = m - k;
+        if ((a & 7u) == 0 && left >= 8u) {
+            *reinterpret_cast<uint64_t*>(dst + k) = r >> (8 * k);
+            k += 8;
+        } else if ((a & 3u) == 0 && left >= 4u) {
+            *reinterpret_cast<uint32_t*>(dst + k) = (uint32_t)(r >> (8 * k));
+            k += 4;
+        } else if ((a & 1u) == 0 && left >= 2u) {
+            *reinterpret_cast<uint16_t*>(dst + k) = (uint16_t)(r >> (8 * k));
+            k += 2;
+        } else {
+            dst[k] = (uint8_t)(r >> (8 * k));
+            k += 1;
+        }
     }
 }
 
@@ -432,7 +535,8 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
     return 2 * (c + 64) + rows * 4 + (uint64_t)cb * (256 * 5 + 4) + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
-           tiles * sizeof(TileDesc) + 3 * (cb + 1) * 4 + 6 * nsplit * 4 + seg_buf_bytes(rows, cb) + 64;
+           tiles * sizeof(TileDesc) + 4 * (cb + 1) * 4 + 6 * nsplit * 4 + 5 * rule_cap(nsplit, cb) * 4 +
+           seg_buf_bytes(rows, cb) + 64;
 }
 
 int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
@@ -458,6 +562,8 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&tiles_, tiles * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
     QZ_HIP(hipMalloc(&sbase_, (cb + 1) * 4));
+    QZ_HIP(hipMalloc(&rbase_, (cb + 1) * 4));
+    QZ_HIP(hipMalloc(&rul_, 5 * rule_cap(nsplit, cb) * 4));
     QZ_HIP(hipMalloc(&prim_, cb * 4));
     QZ_HIP(hipMalloc(&nxt0_, nsplit * 4));
     QZ_HIP(hipMalloc(&nxt1_, nsplit * 4));
@@ -481,7 +587,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 static size_t carve(uint8_t* base, uint64_t n, uint32_t nblk, uint32_t** E, uint32_t** fst, uint8_t** fsym,
                     uint32_t** fsig, uint32_t** hist, uint32_t** tmp, void** tiles, uint32_t** off, uint32_t** sbase,
                     uint32_t** prim, uint32_t** nxt0, uint32_t** nxt1, uint32_t** len, uint32_t** d0, uint32_t** d1,
-                    uint32_t** flag, uint8_t** buf, uint32_t** resume)
+                    uint32_t** flag, uint8_t** buf, uint32_t** resume, uint32_t** rbase, uint32_t** rul)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
@@ -509,15 +615,17 @@ static size_t carve(uint8_t* base, uint64_t n, uint32_t nblk, uint32_t** E, uint
     take(flag, 64);
     take(buf, seg_buf_bytes(rows, cb));
     take(resume, nsplit * 4);
+    take(rbase, (uint64_t)(cb + 1) * 4);
+    take(rul, 5 * rule_cap(nsplit, cb) * 4);
     return at;
 }
 
 size_t IbwtEngine::bound_bytes(uint64_t n, uint32_t nblk)
 {
-    uint32_t *a, *b, *c, *d, *e, *f, *g, *h, *i, *j, *k, *l, *m, *o, *q;
+    uint32_t *a, *b, *c, *d, *e, *f, *g, *h, *i, *j, *k, *l, *m, *o, *q, *r, *u;
     uint8_t *y, *z;
     void* t;
-    return carve(nullptr, n, nblk, &a, &b, &y, &c, &d, &e, &t, &f, &g, &h, &i, &j, &k, &l, &m, &o, &z, &q);
+    return carve(nullptr, n, nblk, &a, &b, &y, &c, &d, &e, &t, &f, &g, &h, &i, &j, &k, &l, &m, &o, &z, &q, &r, &u);
 }
 
 int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
@@ -525,7 +633,7 @@ int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
     if (!base || bound_bytes(n, nblk) > bytes) return QZ_NOT_ENOUGH_MEMORY;
     release();
     carve(base, n, nblk, &E_, &fst_, &fsym_, &fsig_, &hist_, &tmp_, &tiles_, &off_, &sbase_, &prim_, &nxt0_, &nxt1_,
-          &len_, &d0_, &d1_, &flag_, &buf_, &resume_);
+          &len_, &d0_, &d1_, &flag_, &buf_, &resume_, &rbase_, &rul_);
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     cap_ = c;
@@ -542,7 +650,7 @@ int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
 void IbwtEngine::release()
 {
     void* ps[] = {U_,   T_,   E_,    hist_, tmp_, tiles_,  off_, sbase_, prim_, nxt0_, nxt1_,
-                  len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_};
+                  len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_, rbase_, rul_};
     if (!borrowed_)
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -553,6 +661,7 @@ void IbwtEngine::release()
     fst_ = fsig_ = nullptr;
     fsym_ = nullptr;
     hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = resume_ = nullptr;
+    rbase_ = rul_ = nullptr;
     buf_ = nullptr;
     tiles_ = nullptr;
     pinned_ = nullptr;
@@ -581,11 +690,13 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     // descriptors are built when the graph is not replayed)
     uint32_t ntiles = 0;
     const uint32_t hbase = count_tiles(offs.data(), nblk, &ntiles, 256);
-    uint32_t nsplit = 0, chain = 1;  // chain: the most splitters of one block
+    uint32_t nsplit = 0, nrule = 0, chain = 1;  // chain: the most rulers of one block
     for (uint32_t b = 0; b < nblk; ++b) {
         const uint32_t m = (offs[b + 1] - offs[b]) / split + 1;
+        const uint32_t r = (m + kRule - 1) / kRule;
         nsplit += m;
-        chain = m > chain ? m : chain;
+        nrule += r;
+        chain = r > chain ? r : chain;
     }
     if (ntiles > cap_tiles_ || nsplit > cap_split_) return QZ_BAD_PARAMETER;
     auto prepare = [&](hipStream_t st) -> int {
@@ -600,8 +711,16 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
             acc += (offs[b + 1] - offs[b]) / split + 1;
         }
         sb_host_[nblk] = acc;
+        rb_host_.resize(nblk + 1);
+        acc = 0;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            rb_host_[b] = acc;
+            acc += ((offs[b + 1] - offs[b]) / split + 1 + kRule - 1) / kRule;
+        }
+        rb_host_[nblk] = acc;
         QZ_HIP(hipMemcpyAsync(tiles_, td_host_.data(), (size_t)nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
         QZ_HIP(hipMemcpyAsync(sbase_, sb_host_.data(), sb_host_.size() * 4, hipMemcpyHostToDevice, st));
+        QZ_HIP(hipMemcpyAsync(rbase_, rb_host_.data(), rb_host_.size() * 4, hipMemcpyHostToDevice, st));
         return 0;
     };
     auto launch = [&](hipStream_t st) -> int {
@@ -621,26 +740,36 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
                            nsplit, split, pool, nxt0_, len_, buf_, resume_);
         QZ_HIP(hipGetLastError());
-        // list ranking; the first round reads the segment lengths directly,
-        // later rounds alternate between d0_ and d1_
-        const uint32_t* din = len_;
-        uint32_t *na = nxt0_, *nb_ = nxt1_, *dout = d0_;
-        // a block's splitters form one chain (its LF cycle), so
+        // list ranking by a ruling set: rulers walk to the next ruler
+        // (owner / pre in nxt1_ / d1_), pointer jumping ranks the rulers,
+        // then every splitter's distance goes to d0_
+        const uint64_t rc = rule_cap(cap_split_, cap_blk_);
+        uint32_t *rA = rul_, *rB = rul_ + rc, *rL = rul_ + 2 * rc, *rD1 = rul_ + 3 * rc, *rD2 = rul_ + 4 * rc;
+        const uint32_t gr = qz_div_up(nrule, kIThreads);
+        hipLaunchKernelGGL(k_ib_rule_walk, dim3(gr), dim3(kIThreads), 0, st, nxt0_, len_, sbase_, rbase_, nblk, nrule,
+                           nxt1_, d1_, rA, rL);
+        QZ_HIP(hipGetLastError());
+        // a block's rulers form one chain (its LF cycle), so
         // ceil(log2(longest chain)) + 1 rounds always suffice
+        const uint32_t* din = rL;
+        uint32_t *na = rA, *nb_ = rB, *dout = rD1;
         int rounds = 1;
         while ((1u << rounds) < chain && rounds < 31) ++rounds;
         for (int r = 0; r <= rounds; ++r) {
-            hipLaunchKernelGGL(k_ib_jump, dim3(gs), dim3(kIThreads), 0, st, na, din, nb_, dout, nsplit, flag_);
+            hipLaunchKernelGGL(k_ib_jump, dim3(gr), dim3(kIThreads), 0, st, na, din, nb_, dout, nrule, flag_);
             QZ_HIP(hipGetLastError());
             std::swap(na, nb_);
             din = dout;
-            dout = (dout == d0_) ? d1_ : d0_;
+            dout = (dout == rD1) ? rD2 : rD1;
         }
+        hipLaunchKernelGGL(k_ib_rule_apply, dim3(gs), dim3(kIThreads), 0, st, sbase_, rbase_, nblk, nsplit, nrule,
+                           nxt1_, d1_, din, d0_);
+        QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_copy, dim3(qz_div_up((uint64_t)nsplit * (seg_buf(split) / 8), kIThreads)),
-                           dim3(kIThreads), 0, st, off_, sbase_, nblk, nsplit, split, len_, din, buf_, T);
+                           dim3(kIThreads), 0, st, off_, sbase_, nblk, nsplit, split, len_, d0_, buf_, T);
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
-                           nsplit, split, pool, len_, din, resume_, T);
+                           nsplit, split, pool, len_, d0_, resume_, T);
         QZ_HIP(hipGetLastError());
         return 0;
     };
