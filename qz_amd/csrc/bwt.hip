@@ -564,11 +564,12 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
         }
     }
     __syncthreads();
+    uint32_t vo[kKeyItems];
     if (i0 < n) {
         uint32_t top = 1;  // weight of the leading code
         for (uint32_t j = 1; j < k0; ++j) top *= radix;
         uint32_t end = 0, bo = 0, key = 0;
-        uint32_t out[kKeyItems], vo[kKeyItems];
+        uint32_t out[kKeyItems];
 #pragma unroll
         for (int q = 0; q < kKeyItems; ++q) {
             const uint32_t i = i0 + q;
@@ -592,18 +593,31 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
             }
             out[q] = key;
         }
-        store_keys(vals, i0, n, vo);
         uint4* dst = reinterpret_cast<uint4*>(kl + threadIdx.x * kKeyItems);
 #pragma unroll
         for (int v = 0; v < kKeyItems / 4; ++v)
             dst[v] = make_uint4(out[4 * v], out[4 * v + 1], out[4 * v + 2], out[4 * v + 3]);
     }
-    __syncthreads();
+    // keys, then SA entries, each staged in LDS and stored as contiguous
+    // 16-byte vectors
     const uint32_t cnt = min(n - base, (uint32_t)(kThreads * kKeyItems));
     const uint4* src = reinterpret_cast<const uint4*>(kl);
-    uint4* kv = reinterpret_cast<uint4*>(keys + base);
-    for (uint32_t v = threadIdx.x; v < cnt / 4u; v += kThreads) kv[v] = src[v];
-    for (uint32_t i = (cnt & ~3u) + threadIdx.x; i < cnt; i += kThreads) keys[base + i] = kl[i];
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t* o = pass ? vals : keys;
+        if (pass) {
+            __syncthreads();  // the keys are out of LDS
+            if (i0 < n) {
+                uint4* dst = reinterpret_cast<uint4*>(kl + threadIdx.x * kKeyItems);
+#pragma unroll
+                for (int v = 0; v < kKeyItems / 4; ++v)
+                    dst[v] = make_uint4(vo[4 * v], vo[4 * v + 1], vo[4 * v + 2], vo[4 * v + 3]);
+            }
+        }
+        __syncthreads();
+        uint4* ov = reinterpret_cast<uint4*>(o + base);
+        for (uint32_t v = threadIdx.x; v < cnt / 4u; v += kThreads) ov[v] = src[v];
+        for (uint32_t i = (cnt & ~3u) + threadIdx.x; i < cnt; i += kThreads) o[base + i] = kl[i];
+    }
 }
 
 // Symbol packing from the byte histogram (one workgroup of 256): present
@@ -2055,6 +2069,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             rle = ((volatile uint32_t*)hc)[65] != 0;
             cb = ((volatile uint32_t*)hc)[66];
             k0_ = h;
+            cb_ = cb;
         }
         const int cofs = (cur ? kCntB : 0) - lo;
         uint32_t counts[kClasses];
@@ -2260,17 +2275,20 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
     build_block_tab(offs_host_, n, btab_host_);
     QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
     QZ_TRY(sort_suffixes(text, n, nblk, s));
-    // the sort is done with its keys: keys0_ (4 bytes per position) holds the
-    // 4-bit text when the alphabet allows it
-    hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
-                       (const PackParams*)ppd_, keys0_);
-    QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text,
-                       (const uint32_t*)keys0_, (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
-    QZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bwt_out_chars, dim3(qz_div_up(n, kThreads * kOutItems)), dim3(kThreads), 0, s, text,
-                       (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
-    QZ_HIP(hipGetLastError());
+    if (cb_) {  // char mode (read back with round 0's counts): a streaming pass
+        hipLaunchKernelGGL(k_bwt_out_chars, dim3(qz_div_up(n, kThreads * kOutItems)), dim3(kThreads), 0, s, text,
+                           (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
+        QZ_HIP(hipGetLastError());
+    } else {
+        // the sort is done with its keys: keys0_ (4 bytes per position)
+        // holds the 4-bit text when the alphabet allows it
+        hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
+                           (const PackParams*)ppd_, keys0_);
+        QZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text,
+                           (const uint32_t*)keys0_, (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
+        QZ_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
     QZ_HIP(hipGetLastError());
     return 0;

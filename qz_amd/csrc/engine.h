@@ -27,6 +27,7 @@ public:
     int rounds_ = 0;
     bool trace_ = false;
     uint32_t k0_ = 0;
+    uint32_t cb_ = 0;            // char mode of the last batch (SA entries carry the preceding symbol)
     uint64_t cap_ = 0;
     uint32_t cap_blk_ = 0;
 

@@ -441,33 +441,11 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
     const uint32_t b = find_block_u32(sbase, nblk, sid);
     const uint32_t ob = off[b], n = off[b + 1] - ob;
     const uint32_t t0 = n + 1u - dist[sid] + k0;
-    if (t0 >= n) return;  // steps past the block's last symbol (the sentinel's)
     const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
-    uint32_t m = l - k0 < 8u ? l - k0 : 8u;
-    m = m < n - t0 ? m : n - t0;
-    // step j goes to T[n-1-t0-j]: the m bytes land at [lo, lo + m) in
-    // reverse order; write them with naturally aligned stores of 8, 4, 2
-    // and 1 bytes (at most four)
-    uint64_t r = 0;
-    for (uint32_t j = 0; j < m; ++j) r |= ((w >> (8 * j)) & 0xffull) << (8 * (m - 1u - j));
-    uint8_t* dst = T + ob + n - t0 - m;
-    uint32_t k = 0;
-    while (k < m) {
-        const uintptr_t a = (uintptr_t)(dst + k);
-        const uint32_t left = m - k;
-        if ((a & 7u) == 0 && left >= 8u) {
-            *reinterpret_cast<uint64_t*>(dst + k) = r >> (8 * k);
-            k += 8;
-        } else if ((a & 3u) == 0 && left >= 4u) {
-            *reinterpret_cast<uint32_t*>(dst + k) = (uint32_t)(r >> (8 * k));
-            k += 4;
-        } else if ((a & 1u) == 0 && left >= 2u) {
-            *reinterpret_cast<uint16_t*>(dst + k) = (uint16_t)(r >> (8 * k));
-            k += 2;
-        } else {
-            dst[k] = (uint8_t)(r >> (8 * k));
-            k += 1;
-        }
+    const uint32_t m = l - k0 < 8u ? l - k0 : 8u;
+    for (uint32_t j = 0; j < m; ++j) {
+        const uint32_t t = t0 + j;
+        if (t < n) T[ob + n - 1u - t] = (uint8_t)(w >> (8 * j));
     }
 }
 
