@@ -166,6 +166,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
 {
     constexpr uint32_t D = 1u << BITS;
     constexpr int W = kSortThreads / QZ_WAVE;
+    constexpr int IT = kSortTile / kSortThreads;
     // R copies of each wave's counters, a digit's copies in adjacent words
     // (different banks), the copy chosen by lane: lanes of one wave that
     // share a digit (frequent with 2-bit symbols packed into digits) then
@@ -174,12 +175,25 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
     __shared__ uint32_t cnt[W][D * R];
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, rep = tid & (R - 1);
+    if (td.len == 0) {  // uniform; build_tiles makes no empty tiles
+        for (uint32_t d = tid; d < D; d += kSortThreads) hist[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] = 0u;
+        return;
+    }
+    // All of the tile's loads go out before the first counter update (lanes
+    // past the tile's end read its last key and count nothing), so a wave
+    // waits on memory once rather than once per key.
+    const uint32_t last = td.start + td.len - 1u;
+    K kk[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) kk[it] = keys[min(td.start + tid + (uint32_t)it * kSortThreads, last)];
     for (uint32_t d = tid; d < D * R; d += kSortThreads)
 #pragma unroll
         for (int w = 0; w < W; ++w) cnt[w][d] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < td.len; i += kSortThreads)
-        atomicAdd(&cnt[wave][((uint32_t)(keys[td.start + i] >> shift) & (D - 1)) * R + rep], 1u);
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+        if (tid + (uint32_t)it * kSortThreads < td.len)
+            atomicAdd(&cnt[wave][((uint32_t)(kk[it] >> shift) & (D - 1)) * R + rep], 1u);
     __syncthreads();
     for (uint32_t d = tid; d < D; d += kSortThreads) {
         uint32_t t = 0;
@@ -192,11 +206,21 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
 }
 
 // One LSD pass over a tile of <= 4096 pairs. Each wave ranks its own 1024
-// consecutive elements (16 iterations of 64) with BITS ballots per element
-// and wave-private LDS counters; the tile is then permuted by digit in LDS
-// and written out so that consecutive lanes store consecutive slots of a
-// digit run. Stable: element order is wave-major, then iteration, then lane.
-template <typename K, int BITS>
+// consecutive elements (16 iterations of 64) with wave-private LDS counters;
+// the tile is then permuted by digit in LDS and written out so that
+// consecutive lanes store consecutive slots of a digit run. Stable: element
+// order is wave-major, then iteration, then lane.
+// Ranking an iteration needs, per lane, the mask of lanes holding the same
+// digit. For digits of <= 9 bits each lane ORs its own bit into a per-wave
+// LDS word of its digit and reads the word back (one 64-bit LDS atomic and
+// one load, in place of one ballot and four 32-bit mask operations per
+// digit bit); the digit's first lane clears the word for the next
+// iteration. A wave's LDS instructions complete in order, so every lane
+// reads the word after all of the iteration's ORs. The words live in the
+// key staging array, which is idle until the permutation. Wider digits use
+// one ballot per bit.
+// IOTA: the values are the elements' positions (vin is not read).
+template <typename K, int BITS, bool IOTA>
 __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -208,44 +232,65 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     constexpr uint32_t R = D / kSortThreads;  // digits per thread in the prefix (1 or 8)
     constexpr int W = kSortThreads / QZ_WAVE;
     constexpr int IT = kSortTile / kSortThreads;  // 16
-    __shared__ K sk[kSortTile];
+    constexpr bool kOrMask = (size_t)W * D * sizeof(uint64_t) <= sizeof(K) * kSortTile;
+    __shared__ __align__(16) K sk[kSortTile];
     __shared__ uint32_t sv[kSortTile];
     __shared__ uint32_t wcnt[W][D];
     __shared__ uint32_t lstart[D];
     __shared__ uint32_t gbase[D];
     __shared__ uint32_t wtot[W];
+    uint64_t* const msk = reinterpret_cast<uint64_t*>(sk);  // [W][D] while ranking
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
+    if (td.len == 0) return;  // uniform; build_tiles makes no empty tiles
+    // All of the tile's loads go out before the ranking starts; lanes past
+    // the tile's end read its last element and rank nothing.
+    const uint32_t e0 = wave * (IT * QZ_WAVE) + lane;
+    const uint32_t last = td.start + td.len - 1u;
+    K kk[IT];
+    uint32_t vv[IT];
+    uint32_t rk[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) kk[it] = kin[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+        vv[it] = IOTA ? td.start + e0 + (uint32_t)it * QZ_WAVE : vin[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
     // seg_delta (optional): per segment, where its output lies relative to
     // the packed order the histograms count in
     const uint32_t delta = seg_delta ? (uint32_t)seg_delta[td.block] : 0u;
     for (uint32_t d = tid; d < D; d += kSortThreads) {
 #pragma unroll
         for (int w = 0; w < W; ++w) wcnt[w][d] = 0;
+        if (kOrMask)
+#pragma unroll
+            for (int w = 0; w < W; ++w) msk[(uint32_t)w * D + d] = 0ull;
         gbase[d] = offs[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] + delta;
     }
     __syncthreads();
-    K kk[IT];
-    uint32_t vv[IT];
-    uint32_t rk[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const uint32_t e = wave * (IT * QZ_WAVE) + it * QZ_WAVE + lane;
-        const bool valid = e < td.len;
-        const K k = valid ? kin[td.start + e] : (K)0;
-        const uint32_t d = (uint32_t)(k >> shift) & (D - 1);
-        uint64_t peer = __ballot(valid);
+        const bool valid = e0 + (uint32_t)it * QZ_WAVE < td.len;
+        const uint32_t d = (uint32_t)(kk[it] >> shift) & (D - 1);
+        uint64_t peer;
+        if (kOrMask) {
+            uint64_t* const word = msk + wave * D + d;
+            if (valid) atomicOr((unsigned long long*)word, 1ull << lane);
+            peer = valid ? *word : 0ull;
+        } else {
+            peer = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < BITS; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peer &= bit ? m : ~m;
+            for (int b = 0; b < BITS; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peer &= bit ? m : ~m;
+            }
         }
         const uint32_t below = popc_below(peer);
         const uint32_t old = wcnt[wave][d];
-        if (valid && below == 0) wcnt[wave][d] = old + (uint32_t)__popcll(peer);
-        kk[it] = k;
-        vv[it] = valid ? (vin ? vin[td.start + e] : td.start + e) : 0u;  // no vin: values are positions
+        if (valid && below == 0) {
+            wcnt[wave][d] = old + (uint32_t)__popcll(peer);
+            if (kOrMask) msk[wave * D + d] = 0ull;
+        }
         rk[it] = valid ? (old + below) : 0xffffffffu;
     }
     __syncthreads();
@@ -322,9 +367,12 @@ static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDe
         QZ_HIP(hipGetLastError());
         QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
         // iota_vals: the first pass takes each key's position as its value
-        const uint32_t* vin = (iota_vals && passes == 0) ? nullptr : va;
-        hipLaunchKernelGGL((k_seg_scatter<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, vin, kb, vb, tiles,
-                           sh, hist, seg_delta);
+        if (iota_vals && passes == 0)
+            hipLaunchKernelGGL((k_seg_scatter<K, BITS, true>), dim3(ntiles), dim3(kSortThreads), 0, s, src,
+                               (const uint32_t*)nullptr, kb, vb, tiles, sh, hist, seg_delta);
+        else
+            hipLaunchKernelGGL((k_seg_scatter<K, BITS, false>), dim3(ntiles), dim3(kSortThreads), 0, s, src, va, kb,
+                               vb, tiles, sh, hist, seg_delta);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
         uint32_t* tv = va; va = vb; vb = tv;
