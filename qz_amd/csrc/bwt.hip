@@ -789,9 +789,26 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t lane = lane_id(), w = threadIdx.x / QZ_WAVE;
     const uint32_t qb = w * Q;
+    const bool last_tile = td.tlocal + 1 == td.ntiles;
+    // All of a lane's keys, neighbouring keys and suffixes first, clamped as
+    // in k_final0f.
+    constexpr uint32_t SQ = Q / QZ_WAVE;
+    const uint32_t lastj = td.start + td.len - 1u;
+    uint32_t kc[SQ], kp[SQ], kn[SQ], vs[SQ];
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t j = min(td.start + qb + st * QZ_WAVE + lane, lastj);
+        kc[st] = keys[j];
+        kp[st] = keys[j ? j - 1u : 0u];
+        kn[st] = keys[last_tile ? min(j + 1u, lastj) : j + 1u];
+        vs[st] = vals[j];
+    }
     uint32_t m = 0;
-    for (uint32_t i = qb + lane; i < qb + Q && i < td.len; i += QZ_WAVE)
-        if (slot_head(keys, td, i)) m = td.start + i;
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t i = qb + st * QZ_WAVE + lane;
+        if (i < td.len && ((i == 0 && td.tlocal == 0) || kc[st] != kp[st])) m = td.start + i;
+    }
     for (int d = 1; d < QZ_WAVE; d <<= 1) {
         const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
         m = o > m ? o : m;
@@ -800,14 +817,16 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
     __syncthreads();
     uint32_t carry = pre[blockIdx.x];
     for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
-    const bool last_tile = td.tlocal + 1 == td.ntiles;
     const uint32_t cb = ppd->cb, bo = bm.off[td.block];
-    for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t b = qb + st * QZ_WAVE;
+        if (b >= td.len) break;  // wave-uniform
         const uint32_t i = b + lane;
         const bool valid = i < td.len;
         const uint32_t j = td.start + i;
-        const uint32_t k = valid ? keys[j] : 0u;
-        const bool head = valid && slot_head(keys, td, i);
+        const uint32_t k = kc[st];
+        const bool head = valid && ((i == 0 && td.tlocal == 0) || k != kp[st]);
         uint32_t g = head ? j : 0u;
         for (int d = 1; d < QZ_WAVE; d <<= 1) {
             const uint32_t o = __shfl_up(g, d, QZ_WAVE);
@@ -816,10 +835,10 @@ __global__ __launch_bounds__(kThreads) void k_final0(const uint32_t* __restrict_
         if (carry > g) g = carry;
         carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
         if (valid) {
-            const uint32_t s = vals[j];
+            const uint32_t s = vs[st];
             sa[j] = s;
             rank[sa_pos(s, cb, bo)] = g;
-            const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
+            const bool end = (last_tile && i + 1 == td.len) || kn[st] != k;
             if (end && j > g) stage_push(S, g, j - g + 1);
         }
     }
@@ -935,9 +954,30 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
     for (uint32_t k = tid; k < kSortTile / 32; k += T) fl[k] = 0u;
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t qb = w * Q;
+    const bool last_tile = td.tlocal + 1 == td.ntiles;
+    // Each lane's keys, neighbouring keys and suffixes for all of its steps,
+    // loaded before any is used (one wait for the wave, not one per step).
+    // Indexes are clamped to the tile's last slot (lanes past it use
+    // nothing); j - 1 is clamped at 0 (the head test does not read it for a
+    // block's first slot) and j + 1 stays inside a block's last tile (its
+    // last slot ends its group without reading it).
+    constexpr uint32_t SQ = Q / QZ_WAVE;
+    const uint32_t lastj = td.start + td.len - 1u;
+    uint32_t kc[SQ], kp[SQ], kn[SQ], vs[SQ];
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t j = min(td.start + qb + st * QZ_WAVE + lane, lastj);
+        kc[st] = keys[j];
+        kp[st] = keys[j ? j - 1u : 0u];
+        kn[st] = keys[last_tile ? min(j + 1u, lastj) : j + 1u];
+        vs[st] = vals[j];
+    }
     uint32_t m = 0;
-    for (uint32_t i = qb + lane; i < qb + Q && i < td.len; i += QZ_WAVE)
-        if (slot_head(keys, td, i)) m = td.start + i;
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t i = qb + st * QZ_WAVE + lane;
+        if (i < td.len && ((i == 0 && td.tlocal == 0) || kc[st] != kp[st])) m = td.start + i;
+    }
     for (int d = 1; d < QZ_WAVE; d <<= 1) {
         const uint32_t o = __shfl_xor(m, d, QZ_WAVE);
         m = o > m ? o : m;
@@ -946,15 +986,17 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
     __syncthreads();
     uint32_t carry = pre[blockIdx.x];
     for (uint32_t v = 0; v < w; ++v) carry = qlast[v] > carry ? qlast[v] : carry;
-    const bool last_tile = td.tlocal + 1 == td.ntiles;
     const uint32_t cb = ppd->cb, bo = bm.off[td.block];
     // phase 1
-    for (uint32_t b = qb; b < qb + Q && b < td.len; b += QZ_WAVE) {  // wave-uniform
+#pragma unroll
+    for (uint32_t st = 0; st < SQ; ++st) {
+        const uint32_t b = qb + st * QZ_WAVE;
+        if (b >= td.len) break;  // wave-uniform
         const uint32_t i = b + lane;
         const bool valid = i < td.len;
         const uint32_t j = td.start + i;
-        const uint32_t k = valid ? keys[j] : 0u;
-        const bool head = valid && slot_head(keys, td, i);
+        const uint32_t k = kc[st];
+        const bool head = valid && ((i == 0 && td.tlocal == 0) || k != kp[st]);
         uint32_t g = head ? j : 0u;
         for (int d = 1; d < QZ_WAVE; d <<= 1) {
             const uint32_t o = __shfl_up(g, d, QZ_WAVE);
@@ -963,8 +1005,8 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
         if (carry > g) g = carry;
         carry = __shfl(g, QZ_WAVE - 1, QZ_WAVE);
         if (valid) {
-            const uint32_t s = vals[j];
-            const bool end = (last_tile && i + 1 == td.len) || keys[j + 1] != k;
+            const uint32_t s = vs[st];
+            const bool end = (last_tile && i + 1 == td.len) || kn[st] != k;
             if (g < td.start || (head && end)) {  // final now
                 sa[j] = s;
                 rank[sa_pos(s, cb, bo)] = g;
@@ -1026,9 +1068,14 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
                 }
             }
         }
+        // every lane loads (those with nothing to fetch share one address),
+        // so all of a thread's gathers are in flight together
+        uint32_t got[PER];
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q) got[q] = kpos[(want[q] && pos[q] < bend) ? pos[q] : td.start];
 #pragma unroll
         for (uint32_t q = 0; q < PER; ++q)
-            if (want[q]) sx[tid + q * T] = pos[q] < bend ? kpos[pos[q]] : 0u;
+            if (want[q]) sx[tid + q * T] = pos[q] < bend ? got[q] : 0u;
     }
     __syncthreads();  // class bases (thread 0) and codes are in place
     for (uint32_t k = tid; k < min(nfe, (uint32_t)kStage); k += T) {

@@ -93,35 +93,45 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
     constexpr int W = kIThreads / QZ_WAVE;
     constexpr int IT = kSortTile / kIThreads;  // 16 steps of 64 per wave
     __shared__ uint32_t cnt[W][256];
+    __shared__ unsigned long long msk[W][256];  // per wave and symbol: the step's lanes holding it
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = lane_id();
     const uint32_t ob = off[td.block];
     const uint32_t p = prim[td.block];
     const uint32_t rb = ob + td.block;  // row base of this block
-#pragma unroll
-    for (int w = 0; w < W; ++w) cnt[w][tid] = 0;
-    __syncthreads();
-    uint8_t cs[IT];
-    uint32_t rk[IT];
     const uint32_t wbase = wave * (IT * QZ_WAVE);
+    // all of a lane's symbols are loaded before the ranking (lanes past the
+    // tile's end read its last symbol and rank nothing)
+    const uint32_t last = td.start + td.len - 1u;
+    uint8_t cs[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) cs[it] = U[min(td.start + wbase + (uint32_t)it * QZ_WAVE + lane, last)];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        cnt[w][tid] = 0;
+        msk[w][tid] = 0ull;
+    }
+    __syncthreads();
+    uint32_t rk[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const uint32_t i = wbase + it * QZ_WAVE + lane;
         const bool valid = i < td.len;
-        const uint32_t c = valid ? U[td.start + i] : 0u;
-        uint64_t peer = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (c >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peer &= bit ? m : ~m;
-        }
+        const uint32_t c = cs[it];
+        // the lanes of this step holding c: each ORs its bit into the
+        // symbol's word and reads the word back; a wave's LDS instructions
+        // complete in order, so the read follows every OR of the step, and
+        // the symbol's first lane clears the word for the next step
+        if (valid) atomicOr(&msk[wave][c], 1ull << lane);
+        const uint64_t peer = valid ? (uint64_t)msk[wave][c] : 0ull;
         const uint32_t below = popc_below(peer);
         // a wave's LDS reads and writes complete in order: the next step
         // reads what this one wrote
         const uint32_t before = cnt[wave][c];
-        if (valid && below == 0) cnt[wave][c] = before + (uint32_t)__popcll(peer);
-        cs[it] = (uint8_t)c;
+        if (valid && below == 0) {
+            cnt[wave][c] = before + (uint32_t)__popcll(peer);
+            msk[wave][c] = 0ull;
+        }
         rk[it] = valid ? before + below : 0xffffffffu;
     }
     __syncthreads();
@@ -435,13 +445,16 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
     const uint64_t gid = (uint64_t)blockIdx.x * kIThreads + threadIdx.x;
     const uint32_t sid = (uint32_t)(gid / kChunks), q = (uint32_t)(gid % kChunks);
     if (sid >= nsplit) return;
+    // the loads that do not depend on the block go out together, before the
+    // block search
     const uint32_t l = len[sid];
+    const uint32_t ds = dist[sid];
+    const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
     const uint32_t k0 = q * 8;
     if (k0 >= l) return;
     const uint32_t b = find_block_u32(sbase, nblk, sid);
     const uint32_t ob = off[b], n = off[b + 1] - ob;
-    const uint32_t t0 = n + 1u - dist[sid] + k0;
-    const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
+    const uint32_t t0 = n + 1u - ds + k0;
     const uint32_t m = l - k0 < 8u ? l - k0 : 8u;
     for (uint32_t j = 0; j < m; ++j) {
         const uint32_t t = t0 + j;
