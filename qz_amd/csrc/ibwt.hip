@@ -100,18 +100,20 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
     const uint32_t p = prim[td.block];
     const uint32_t rb = ob + td.block;  // row base of this block
     const uint32_t wbase = wave * (IT * QZ_WAVE);
-    // all of a lane's symbols are loaded before the ranking (lanes past the
-    // tile's end read its last symbol and rank nothing)
-    const uint32_t last = td.start + td.len - 1u;
-    uint8_t cs[IT];
-#pragma unroll
-    for (int it = 0; it < IT; ++it) cs[it] = U[min(td.start + wbase + (uint32_t)it * QZ_WAVE + lane, last)];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         cnt[w][tid] = 0;
         msk[w][tid] = 0ull;
     }
+    // the counters are cleared before any load is issued (a workgroup
+    // barrier waits for every load in flight); then all of a lane's symbols
+    // go out at once and are ranked as they arrive (lanes past the tile's
+    // end read its last symbol and rank nothing)
     __syncthreads();
+    const uint32_t last = td.start + td.len - 1u;
+    uint8_t cs[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) cs[it] = U[min(td.start + wbase + (uint32_t)it * QZ_WAVE + lane, last)];
     uint32_t rk[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {

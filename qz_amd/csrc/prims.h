@@ -179,17 +179,18 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
         for (uint32_t d = tid; d < D; d += kSortThreads) hist[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] = 0u;
         return;
     }
-    // All of the tile's loads go out before the first counter update (lanes
-    // past the tile's end read its last key and count nothing), so a wave
-    // waits on memory once rather than once per key.
+    for (uint32_t d = tid; d < D * R; d += kSortThreads)
+#pragma unroll
+        for (int w = 0; w < W; ++w) cnt[w][d] = 0;
+    // The counters are cleared before any load is issued (a workgroup
+    // barrier waits for every load in flight); then all of the tile's loads
+    // go out at once and are counted as they arrive. Lanes past the tile's
+    // end read its last key and count nothing.
+    __syncthreads();
     const uint32_t last = td.start + td.len - 1u;
     K kk[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) kk[it] = keys[min(td.start + tid + (uint32_t)it * kSortThreads, last)];
-    for (uint32_t d = tid; d < D * R; d += kSortThreads)
-#pragma unroll
-        for (int w = 0; w < W; ++w) cnt[w][d] = 0;
-    __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it)
         if (tid + (uint32_t)it * kSortThreads < td.len)
@@ -216,9 +217,8 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
 // one load, in place of one ballot and four 32-bit mask operations per
 // digit bit); the digit's first lane clears the word for the next
 // iteration. A wave's LDS instructions complete in order, so every lane
-// reads the word after all of the iteration's ORs. The words live in the
-// key staging array, which is idle until the permutation. Wider digits use
-// one ballot per bit.
+// reads the word after all of the iteration's ORs. Wider digits use one
+// ballot per bit.
 // IOTA: the values are the elements' positions (vin is not read).
 template <typename K, int BITS, bool IOTA>
 __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin,
@@ -239,12 +239,29 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     __shared__ uint32_t lstart[D];
     __shared__ uint32_t gbase[D];
     __shared__ uint32_t wtot[W];
-    uint64_t* const msk = reinterpret_cast<uint64_t*>(sk);  // [W][D] while ranking
+    // [W][D] mask words. For 8-bit digits in an array of their own, which
+    // also keeps the workgroup's LDS above 40 KB, so at most 3 workgroups
+    // share a CU: with 4 a pass over 1.5e9 pairs took 8.4 ms, with 3 6.0 ms.
+    // With 9-bit digits the other arrays already hold 44 KB, and the masks
+    // share the key staging array, which is idle until the permutation.
+    constexpr bool kOwnMask = kOrMask && BITS <= 8;
+    __shared__ unsigned long long mown[kOwnMask ? W * D : 1];
+    uint64_t* const msk = kOwnMask ? reinterpret_cast<uint64_t*>(mown) : reinterpret_cast<uint64_t*>(sk);
     const TileDesc td = tiles[xcd_item(blockIdx.x, gridDim.x)];
     const uint32_t tid = threadIdx.x, wave = tid / QZ_WAVE, lane = tid % QZ_WAVE;
     if (td.len == 0) return;  // uniform; build_tiles makes no empty tiles
-    // All of the tile's loads go out before the ranking starts; lanes past
-    // the tile's end read its last element and rank nothing.
+    for (uint32_t d = tid; d < D; d += kSortThreads) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) wcnt[w][d] = 0;
+        if (kOrMask)
+#pragma unroll
+            for (int w = 0; w < W; ++w) msk[(uint32_t)w * D + d] = 0ull;
+    }
+    // The counters are cleared before any load is issued: a workgroup
+    // barrier waits for every load in flight, so the tile's loads go out
+    // after it and the ranking consumes them as they arrive. Lanes past the
+    // tile's end read its last element and rank nothing.
+    __syncthreads();
     const uint32_t e0 = wave * (IT * QZ_WAVE) + lane;
     const uint32_t last = td.start + td.len - 1u;
     K kk[IT];
@@ -255,18 +272,13 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 #pragma unroll
     for (int it = 0; it < IT; ++it)
         vv[it] = IOTA ? td.start + e0 + (uint32_t)it * QZ_WAVE : vin[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
+    // this tile's output start per digit, stored to LDS after the ranking;
     // seg_delta (optional): per segment, where its output lies relative to
     // the packed order the histograms count in
     const uint32_t delta = seg_delta ? (uint32_t)seg_delta[td.block] : 0u;
-    for (uint32_t d = tid; d < D; d += kSortThreads) {
+    uint32_t gb[R];
 #pragma unroll
-        for (int w = 0; w < W; ++w) wcnt[w][d] = 0;
-        if (kOrMask)
-#pragma unroll
-            for (int w = 0; w < W; ++w) msk[(uint32_t)w * D + d] = 0ull;
-        gbase[d] = offs[(size_t)td.hbase + (size_t)d * td.ntiles + td.tlocal] + delta;
-    }
-    __syncthreads();
+    for (uint32_t r = 0; r < R; ++r) gb[r] = offs[(size_t)td.hbase + (size_t)(tid * R + r) * td.ntiles + td.tlocal] + delta;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const bool valid = e0 + (uint32_t)it * QZ_WAVE < td.len;
@@ -293,6 +305,8 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
         }
         rk[it] = valid ? (old + below) : 0xffffffffu;
     }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) gbase[tid * R + r] = gb[r];
     __syncthreads();
     {
         // digits [tid*R, tid*R+R): wave offsets per digit, then the tile-local
