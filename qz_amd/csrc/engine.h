@@ -111,6 +111,7 @@ private:
     uint32_t* flag_ = nullptr;
     uint8_t* buf_ = nullptr;       // first symbols of every segment
     uint32_t* resume_ = nullptr;   // row after those symbols
+    uint32_t* tails_ = nullptr;    // splitters whose segment is longer than its buffer
     uint32_t* rbase_ = nullptr;    // per block: its first ruler of the list ranking
     uint32_t* rul_ = nullptr;      // rulers' next / rows / distances (five arrays)
     void* pinned_ = nullptr;

@@ -162,8 +162,9 @@ __global__ __launch_bounds__(kIThreads) void k_ib_rank(const uint8_t* __restrict
 
 // Sentinel rows: LF(p) = 0.
 __global__ void k_ib_sentinel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ prim, uint32_t nblk,
-                              uint32_t split, uint32_t* __restrict__ E)
+                              uint32_t split, uint32_t* __restrict__ E, uint32_t* __restrict__ ntail)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ntail = 0u;  // the walks' tail count, before the first walk
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblk) return;
     const uint32_t p = prim[b];
@@ -273,6 +274,7 @@ constexpr uint32_t kMinSplit = 32;  // splitter spacing limits (sizes the per-sp
 constexpr uint32_t kMaxSplit = 256;
 constexpr uint64_t kWideSplitBytes = 128ull << 20;  // batches from here on use spacing 64
 constexpr uint32_t kPoolMax = 512;  // splitters per walking workgroup (swept: 256..2048)
+constexpr uint32_t kPoolLimit = 4096;  // the most a pool may be set to
 // A segment keeps its first 2 * split symbols from the first walk, so the
 // buffer costs 2 bytes per row whatever the spacing.
 __host__ __device__ constexpr uint32_t seg_buf(uint32_t split) { return 2 * split; }
@@ -290,13 +292,19 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint32_t* __restri
                                                          const uint32_t* __restrict__ fsig,
                                                          uint32_t nsplit, uint32_t split, uint32_t pool,
                                                          uint32_t* __restrict__ nxt, uint32_t* __restrict__ len,
-                                                         uint8_t* __restrict__ buf, uint32_t* __restrict__ resume)
+                                                         uint8_t* __restrict__ buf, uint32_t* __restrict__ resume,
+                                                         uint32_t* __restrict__ tails, uint32_t* __restrict__ ntail)
 {
     __shared__ uint32_t next;
+    __shared__ uint32_t lt, lbase;
+    __shared__ uint32_t ltail[kPoolLimit];  // this pool's splitters with a tail
     const uint32_t kBuf = seg_buf(split);
     const uint32_t pbeg = blockIdx.x * pool;
     const uint32_t pend = min(pbeg + pool, nsplit);
-    if (threadIdx.x == 0) next = pbeg;
+    if (threadIdx.x == 0) {
+        next = pbeg;
+        lt = 0;
+    }
     __syncthreads();
     uint32_t sid = atomicAdd(&next, 1u);
     bool active = sid < pend;
@@ -329,6 +337,7 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint32_t* __restri
                 if (l < kBuf && (l & 7)) buf64[(size_t)sid * (kBuf / 8) + (l >> 3)] = word;
                 word = 0;
                 len[sid] = l;
+                if (l > kBuf) ltail[atomicAdd(&lt, 1u)] = sid;
                 nxt[sid] = (x == 0) ? kNil : sb + x / split;
                 sid = atomicAdd(&next, 1u);
                 active = sid < pend;
@@ -345,6 +354,11 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk1(const uint32_t* __restri
             }
         }
     }
+    // the pool's tails go to the batch list, one reservation per workgroup
+    __syncthreads();
+    if (threadIdx.x == 0) lbase = lt ? atomicAdd(ntail, lt) : 0u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < lt; k += kIThreads) tails[lbase + k] = ltail[k];
 }
 
 // Phase 3: pointer jumping; dist[s] ends as the row count from s to the end
@@ -471,51 +485,77 @@ __global__ __launch_bounds__(kIThreads) void k_ib_walk2(const uint32_t* __restri
                                                          const uint32_t* __restrict__ fst,
                                                          const uint8_t* __restrict__ fsym,
                                                          const uint32_t* __restrict__ fsig,
-                                                         uint32_t nsplit, uint32_t split, uint32_t pool,
+                                                         uint32_t split, uint32_t pool,
+                                                         const uint32_t* __restrict__ tails,
+                                                         const uint32_t* __restrict__ ntail,
                                                          const uint32_t* __restrict__ len,
                                                          const uint32_t* __restrict__ dist,
                                                          const uint32_t* __restrict__ resume, uint8_t* __restrict__ T)
 {
+    // the list of tails (walk1 appends every splitter whose segment is
+    // longer than its buffer) in pools of `pool` per workgroup; the grid
+    // covers the longest possible list, and workgroups past its end return
     __shared__ uint32_t next;
     const uint32_t kBuf = seg_buf(split);
     const uint32_t pbeg = blockIdx.x * pool;
-    const uint32_t pend = min(pbeg + pool, nsplit);
+    const uint32_t pend = min(pbeg + pool, *ntail);
+    if (pbeg >= pend) return;  // uniform
     if (threadIdx.x == 0) next = pbeg;
     __syncthreads();
-    uint32_t sid = atomicAdd(&next, 1u);
     uint32_t rb = 0, l = 0, t = 0, last = 0, e = 0;
     FTable ft;
     ft.sig = 0;
     uint8_t* dst = nullptr;
-    // next splitter with a tail, or done
     auto pull = [&]() -> bool {
-        for (;;) {
-            if (sid >= pend) return false;
-            const uint32_t L = len[sid];
-            if (L > kBuf) {
-                const uint32_t b = find_block_u32(sbase, nblk, sid);
-                const uint32_t ob = off[b], n = off[b + 1] - ob;
-                rb = ob + b;
-                l = L - kBuf;
-                t = n + 1u - dist[sid] + kBuf;
-                last = n;
-                dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
-                ft.load(fst, fsym, fsig, b);
-                e = E[rb + resume[sid]];
-                return true;
-            }
-            sid = atomicAdd(&next, 1u);
+        const uint32_t k = atomicAdd(&next, 1u);
+        if (k >= pend) return false;
+        const uint32_t sid = tails[k];
+        const uint32_t b = find_block_u32(sbase, nblk, sid);
+        const uint32_t ob = off[b], n = off[b + 1] - ob;
+        rb = ob + b;
+        l = len[sid] - kBuf;
+        t = n + 1u - dist[sid] + kBuf;
+        last = n;
+        dst = T + ob + n - 1u;  // dst - t addresses T[n-1-t]
+        ft.load(fst, fsym, fsig, b);
+        e = E[rb + resume[sid]];
+        return true;
+    };
+    // A lane writes its tail downwards, one byte per step. It gathers them
+    // per aligned 8-byte window of T and stores a window it fills whole with
+    // one 8-byte store (a wave's byte stores would each be a request of
+    // their own); the partial windows at a tail's two ends, whose other
+    // bytes belong to other segments, go out byte by byte.
+    uint64_t acc = 0;
+    uint32_t have = 0;      // bytes of the window gathered so far (bit k: offset k)
+    uint8_t* win = nullptr;  // the window's first byte
+    auto flush = [&]() {
+        if (have == 0xffu) {
+            *reinterpret_cast<uint64_t*>(win) = acc;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((have >> k) & 1u) win[k] = (uint8_t)(acc >> (8 * k));
         }
+        acc = 0;
+        have = 0;
     };
     bool active = pull();
     while (__ballot(active)) {
         if (active) {
             const uint32_t x = e & kLFMask;
             if (--l != 0) e = E[rb + x];  // issued before the symbol lookup
-            if (t < last) dst[-(int64_t)t] = ft.lookup(x);
+            if (t < last) {
+                uint8_t* const q = dst - (int64_t)t;
+                const uint32_t o = (uint32_t)((uintptr_t)q & 7u);
+                win = q - o;
+                acc |= (uint64_t)ft.lookup(x) << (8 * o);
+                have |= 1u << o;
+                if (o == 0) flush();  // the window's lowest byte is its last step
+            }
             ++t;
             if (l == 0) {
-                sid = atomicAdd(&next, 1u);
+                if (have) flush();
                 active = pull();
             }
         }
@@ -528,7 +568,7 @@ uint64_t IbwtEngine::footprint(uint64_t n, uint32_t nblk)
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     const uint64_t rows = c + cb, tiles = c / kSortTile + cb + 1, nsplit = rows / kMinSplit + cb + 1;
     return 2 * (c + 64) + rows * 4 + (uint64_t)cb * (256 * 5 + 4) + tiles * 256 * 4 + (scan_tmp_words(tiles * 256) + 64) * 4 +
-           tiles * sizeof(TileDesc) + 4 * (cb + 1) * 4 + 6 * nsplit * 4 + 5 * rule_cap(nsplit, cb) * 4 +
+           tiles * sizeof(TileDesc) + 4 * (cb + 1) * 4 + 7 * nsplit * 4 + 5 * rule_cap(nsplit, cb) * 4 +
            seg_buf_bytes(rows, cb) + 64;
 }
 
@@ -566,6 +606,7 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&flag_, 64));
     QZ_HIP(hipMalloc(&buf_, seg_buf_bytes(rows, cb)));
     QZ_HIP(hipMalloc(&resume_, nsplit * 4));
+    QZ_HIP(hipMalloc(&tails_, nsplit * 4));
     QZ_HIP(hipHostMalloc(&pinned_, 64));
     cap_ = c;
     cap_blk_ = cb;
@@ -580,7 +621,8 @@ int IbwtEngine::reserve(uint64_t n, uint32_t nblk)
 static size_t carve(uint8_t* base, uint64_t n, uint32_t nblk, uint32_t** E, uint32_t** fst, uint8_t** fsym,
                     uint32_t** fsig, uint32_t** hist, uint32_t** tmp, void** tiles, uint32_t** off, uint32_t** sbase,
                     uint32_t** prim, uint32_t** nxt0, uint32_t** nxt1, uint32_t** len, uint32_t** d0, uint32_t** d1,
-                    uint32_t** flag, uint8_t** buf, uint32_t** resume, uint32_t** rbase, uint32_t** rul)
+                    uint32_t** flag, uint8_t** buf, uint32_t** resume, uint32_t** rbase, uint32_t** rul,
+                    uint32_t** tails)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
@@ -610,15 +652,16 @@ static size_t carve(uint8_t* base, uint64_t n, uint32_t nblk, uint32_t** E, uint
     take(resume, nsplit * 4);
     take(rbase, (uint64_t)(cb + 1) * 4);
     take(rul, 5 * rule_cap(nsplit, cb) * 4);
+    take(tails, nsplit * 4);
     return at;
 }
 
 size_t IbwtEngine::bound_bytes(uint64_t n, uint32_t nblk)
 {
-    uint32_t *a, *b, *c, *d, *e, *f, *g, *h, *i, *j, *k, *l, *m, *o, *q, *r, *u;
+    uint32_t *a, *b, *c, *d, *e, *f, *g, *h, *i, *j, *k, *l, *m, *o, *q, *r, *u, *v;
     uint8_t *y, *z;
     void* t;
-    return carve(nullptr, n, nblk, &a, &b, &y, &c, &d, &e, &t, &f, &g, &h, &i, &j, &k, &l, &m, &o, &z, &q, &r, &u);
+    return carve(nullptr, n, nblk, &a, &b, &y, &c, &d, &e, &t, &f, &g, &h, &i, &j, &k, &l, &m, &o, &z, &q, &r, &u, &v);
 }
 
 int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
@@ -626,7 +669,7 @@ int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
     if (!base || bound_bytes(n, nblk) > bytes) return QZ_NOT_ENOUGH_MEMORY;
     release();
     carve(base, n, nblk, &E_, &fst_, &fsym_, &fsig_, &hist_, &tmp_, &tiles_, &off_, &sbase_, &prim_, &nxt0_, &nxt1_,
-          &len_, &d0_, &d1_, &flag_, &buf_, &resume_, &rbase_, &rul_);
+          &len_, &d0_, &d1_, &flag_, &buf_, &resume_, &rbase_, &rul_, &tails_);
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
     cap_ = c;
@@ -643,7 +686,7 @@ int IbwtEngine::bind(uint8_t* base, size_t bytes, uint64_t n, uint32_t nblk)
 void IbwtEngine::release()
 {
     void* ps[] = {U_,   T_,   E_,    hist_, tmp_, tiles_,  off_, sbase_, prim_, nxt0_, nxt1_,
-                  len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_, rbase_, rul_};
+                  len_, d0_, d1_, flag_, buf_, resume_, fst_, fsym_, fsig_, rbase_, rul_, tails_};
     if (!borrowed_)
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -654,7 +697,7 @@ void IbwtEngine::release()
     fst_ = fsig_ = nullptr;
     fsym_ = nullptr;
     hist_ = tmp_ = off_ = sbase_ = prim_ = nxt0_ = nxt1_ = len_ = d0_ = d1_ = flag_ = resume_ = nullptr;
-    rbase_ = rul_ = nullptr;
+    rbase_ = rul_ = tails_ = nullptr;
     buf_ = nullptr;
     tiles_ = nullptr;
     pinned_ = nullptr;
@@ -677,7 +720,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
     uint32_t pool = kPoolMax;  // splitters per walking workgroup
     if (const char* e = getenv("QZBSC_POOL")) {
         const long v = strtol(e, nullptr, 10);
-        if (v >= kIThreads && v <= 4096) pool = (uint32_t)v;
+        if (v >= kIThreads && v <= (long)kPoolLimit) pool = (uint32_t)v;
     }
     // block-aligned tiles and per-block splitter bases (sizes only; the
     // descriptors are built when the graph is not replayed)
@@ -723,7 +766,9 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         hipLaunchKernelGGL(k_ib_rank, dim3(ntiles), dim3(kIThreads), 0, st, U, (const TileDesc*)tiles_, hist_, off_,
                            prim_, split, E_);
         QZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, st, off_, prim_, nblk, split, E_);
+        uint32_t* const ntail = flag_ + 1;  // flag_[0]: the jumps' flag
+        hipLaunchKernelGGL(k_ib_sentinel, dim3(qz_div_up(nblk, 64)), dim3(64), 0, st, off_, prim_, nblk, split, E_,
+                           ntail);
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_ftable, dim3(ntiles), dim3(kIThreads), 0, st, (const TileDesc*)tiles_, hist_, off_,
                            fst_, fsym_, fsig_);
@@ -731,7 +776,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
         const uint32_t gs = qz_div_up(nsplit, kIThreads);
         const uint32_t gp = qz_div_up(nsplit, pool);
         hipLaunchKernelGGL(k_ib_walk1, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
-                           nsplit, split, pool, nxt0_, len_, buf_, resume_);
+                           nsplit, split, pool, nxt0_, len_, buf_, resume_, tails_, ntail);
         QZ_HIP(hipGetLastError());
         // list ranking by a ruling set: rulers walk to the next ruler
         // (owner / pre in nxt1_ / d1_), pointer jumping ranks the rulers,
@@ -762,7 +807,7 @@ int IbwtEngine::decode_device(const uint8_t* U, uint8_t* T, uint32_t n, uint32_t
                            dim3(kIThreads), 0, st, off_, sbase_, nblk, nsplit, split, len_, d0_, buf_, T);
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_ib_walk2, dim3(gp), dim3(kIThreads), 0, st, E_, off_, sbase_, nblk, fst_, fsym_, fsig_,
-                           nsplit, split, pool, len_, d0_, resume_, T);
+                           split, pool, (const uint32_t*)tails_, (const uint32_t*)ntail, len_, d0_, resume_, T);
         QZ_HIP(hipGetLastError());
         return 0;
     };
