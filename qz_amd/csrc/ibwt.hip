@@ -457,18 +457,22 @@ __global__ __launch_bounds__(kIThreads) void k_ib_copy(const uint32_t* __restric
                                                         const uint32_t* __restrict__ dist,
                                                         const uint8_t* __restrict__ buf, uint8_t* __restrict__ T)
 {
+    __shared__ uint32_t b0;
     const uint32_t kChunks = seg_buf(split) / 8;
     const uint64_t gid = (uint64_t)blockIdx.x * kIThreads + threadIdx.x;
     const uint32_t sid = (uint32_t)(gid / kChunks), q = (uint32_t)(gid % kChunks);
+    // the block of the workgroup's first splitter, searched once; a thread
+    // steps on from it to its own (a workgroup spans few splitters)
+    if (threadIdx.x == 0) b0 = find_block_u32(sbase, nblk, (uint32_t)(((uint64_t)blockIdx.x * kIThreads) / kChunks));
+    __syncthreads();
     if (sid >= nsplit) return;
-    // the loads that do not depend on the block go out together, before the
-    // block search
     const uint32_t l = len[sid];
     const uint32_t ds = dist[sid];
     const uint64_t w = reinterpret_cast<const uint64_t*>(buf)[(size_t)sid * kChunks + q];
     const uint32_t k0 = q * 8;
     if (k0 >= l) return;
-    const uint32_t b = find_block_u32(sbase, nblk, sid);
+    uint32_t b = b0;
+    while (b + 1 < nblk && sbase[b + 1] <= sid) ++b;
     const uint32_t ob = off[b], n = off[b + 1] - ob;
     const uint32_t t0 = n + 1u - ds + k0;
     const uint32_t m = l - k0 < 8u ? l - k0 : 8u;
