@@ -769,6 +769,15 @@ __device__ __forceinline__ bool fuse_round1(const PackParams& pp, uint32_t fuse)
     return fuse != 0u && pp.rle == 0u;
 }
 
+// Whether the fused round reads its keys from the 4-bit text (k_pack_nib):
+// codes of at most 4 bits, packed keys of at most kNibKeyMax symbols (their
+// codes then lie in two aligned 64-bit words).
+constexpr uint32_t kNibKeyMax = 17;
+__device__ __forceinline__ bool nib_keys(const PackParams& pp)
+{
+    return pp.radix <= 16u && pp.rle == 0u && pp.k0 <= kNibKeyMax;
+}
+
 // Round-0 finish, one tile per workgroup: SA, ranks and the unsorted groups.
 // pre[t] = the last head slot before tile t (exclusive max-scan of lh). Wave
 // w takes the tile's quarter [1024w, 1024w+1024) in 16 coalesced steps and
@@ -931,7 +940,8 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
                                                const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ pre,
                                                uint32_t* __restrict__ sa, uint32_t* __restrict__ rank,
                                                GroupLists glA, GroupLists glB, const PackParams* __restrict__ ppd,
-                                               BlockMap bm, const uint32_t* __restrict__ kpos, uint32_t fuse)
+                                               BlockMap bm, const uint32_t* __restrict__ kpos,
+                                               const uint32_t* __restrict__ nib, uint32_t fuse)
 {
     if (!fuse_round1(*ppd, fuse)) return;  // uniform: k_final0 finishes this batch
     constexpr uint32_t W = T / QZ_WAVE;
@@ -1070,12 +1080,41 @@ __global__ __launch_bounds__(T) void k_final0f(const uint32_t* __restrict__ keys
         }
         // every lane loads (those with nothing to fetch share one address),
         // so all of a thread's gathers are in flight together
-        uint32_t got[PER];
+        if (nib_keys(*ppd)) {
+            // the key from the 4-bit text (an eighth of the positional keys'
+            // bytes, so more of it stays in the L2s): the two aligned 64-bit
+            // words holding its codes, then the packing of k_round0_keys
+            const uint64_t* const nw = reinterpret_cast<const uint64_t*>(nib);
+            const uint32_t radix = ppd->radix;
+            uint64_t lo[PER], hi[PER];
 #pragma unroll
-        for (uint32_t q = 0; q < PER; ++q) got[q] = kpos[(want[q] && pos[q] < bend) ? pos[q] : td.start];
+            for (uint32_t q = 0; q < PER; ++q) {
+                const uint32_t p = (want[q] && pos[q] < bend) ? pos[q] : td.start;
+                lo[q] = nw[p >> 4];
+                hi[q] = nw[(p >> 4) + 1u];
+            }
 #pragma unroll
-        for (uint32_t q = 0; q < PER; ++q)
-            if (want[q]) sx[tid + q * T] = pos[q] < bend ? got[q] : 0u;
+            for (uint32_t q = 0; q < PER; ++q) {
+                if (!want[q]) continue;
+                uint32_t key = 0;
+                if (pos[q] < bend) {
+                    const uint32_t o = pos[q] & 15u, lim = bend - pos[q];
+                    for (uint32_t j = 0; j < k0; ++j) {
+                        const uint32_t at = o + j;  // < 16 + kNibKeyMax
+                        const uint32_t c = (uint32_t)((at < 16u ? lo[q] >> (4u * at) : hi[q] >> (4u * (at - 16u))) & 15u);
+                        key = key * radix + (j < lim ? c : 0u);
+                    }
+                }
+                sx[tid + q * T] = key;
+            }
+        } else {
+            uint32_t got[PER];
+#pragma unroll
+            for (uint32_t q = 0; q < PER; ++q) got[q] = kpos[(want[q] && pos[q] < bend) ? pos[q] : td.start];
+#pragma unroll
+            for (uint32_t q = 0; q < PER; ++q)
+                if (want[q]) sx[tid + q * T] = pos[q] < bend ? got[q] : 0u;
+        }
     }
     __syncthreads();  // class bases (thread 0) and codes are in place
     for (uint32_t k = tid; k < min(nfe, (uint32_t)kStage); k += T) {
@@ -1578,10 +1617,17 @@ __global__ __launch_bounds__(kThreads) void k_final_large(const TileDesc* __rest
 // 15 distinct bytes (sigma + 1 <= 16): k_bwt_out's random gathers then read
 // half the bytes, so twice as much of a block's text stays in L2. A larger
 // alphabet leaves the kernel idle and k_bwt_out reads the bytes.
+// The 4-bit text: nib[i / 8] holds the codes of positions [i, i + 8), the
+// first in the low nibble. for_fuse: built for the fused first doubling
+// round's keys (nib_keys), else for the BWT output gathers (not needed in
+// char mode).
+
 __global__ __launch_bounds__(kThreads) void k_pack_nib(const uint8_t* __restrict__ text, uint32_t n,
-                                                       const PackParams* __restrict__ ppd, uint32_t* __restrict__ nib)
+                                                       const PackParams* __restrict__ ppd, uint32_t* __restrict__ nib,
+                                                       uint32_t for_fuse)
 {
-    if (ppd->radix > 16u || ppd->cb) return;  // uniform over the grid (char mode needs no text)
+    // uniform over the grid
+    if (for_fuse ? !nib_keys(*ppd) : (ppd->radix > 16u || ppd->cb)) return;
     __shared__ uint8_t code[256];
     code[threadIdx.x] = (uint8_t)ppd->code[threadIdx.x];
     __syncthreads();
@@ -2030,9 +2076,16 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
                        (const TileDesc*)tiles0_, bufA_, sa_, rank_, glA, (const PackParams*)ppd_, bmap_, fuse);
     QZ_HIP(hipGetLastError());
     if (fuse) {
+        // the 4-bit text for the fused round's keys, in bufB (the radix
+        // histograms are done with); the kernel returns at once when the
+        // keys cannot use it
+        hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
+                           (const PackParams*)ppd_, bufB_, 1u);
+        QZ_HIP(hipGetLastError());
 #define QZ_FINAL0F(T, SVL)                                                                                      \
     hipLaunchKernelGGL((k_final0f<T, SVL>), dim3(ntiles), dim3(T), 0, s, keys, vals, (const TileDesc*)tiles0_, \
-                       bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)kpos, fuse)
+                       bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)kpos,      \
+                       (const uint32_t*)bufB_, fuse)
         switch (fshape) {
             case 1: QZ_FINAL0F(256, false); break;
             case 2: QZ_FINAL0F(256, true); break;
@@ -2330,7 +2383,7 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
         // the sort is done with its keys: keys0_ (4 bytes per position)
         // holds the 4-bit text when the alphabet allows it
         hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
-                           (const PackParams*)ppd_, keys0_);
+                           (const PackParams*)ppd_, keys0_, 0u);
         QZ_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bwt_out, dim3(qz_div_up(n, kThreads)), dim3(kThreads), occ_lds(kOccBwtOut), s, text,
                            (const uint32_t*)keys0_, (const PackParams*)ppd_, sa_, rank_, bmap_, n, out);
