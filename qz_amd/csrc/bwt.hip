@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
             }
         }
         store_keys(keys, i0, n, out);
-        store_keys(vals, i0, n, vo);
+        if (vals) store_keys(vals, i0, n, vo);
         return;
     }
     // packed keys: the workgroup stages its text window in LDS (16-byte
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
     // 16-byte vectors
     const uint32_t cnt = min(n - base, (uint32_t)(kThreads * kKeyItems));
     const uint4* src = reinterpret_cast<const uint4*>(kl);
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < (vals ? 2 : 1); ++pass) {  // vals == nullptr: keys only
         uint32_t* o = pass ? vals : keys;
         if (pass) {
             __syncthreads();  // the keys are out of LDS
@@ -1850,19 +1850,55 @@ int tune_state(int which, int* choice, double* ns_per_elem)
     return t.trials;
 }
 
-template <typename K>
+// The SA entries of round 0's first radix pass, made from the text instead
+// of read (sa_entry of each position; tiles never span two blocks). The
+// code table and the char-mode flag are decided on the device, so they are
+// read from the packing parameters into LDS.
+struct SaEntrySource {
+    const uint8_t* text;
+    const PackParams* pp;
+    const uint32_t* off;  // block offsets
+    struct Lds {
+        uint16_t code[256];
+        uint32_t cb;
+    };
+    __device__ __forceinline__ void stage(Lds& l) const
+    {
+        for (uint32_t d = threadIdx.x; d < 256u; d += blockDim.x) l.code[d] = pp->code[d];
+        if (threadIdx.x == 0) l.cb = pp->cb;
+    }
+    template <int IT>
+    __device__ __forceinline__ void load(const Lds& l, const TileDesc& td, uint32_t e0, uint32_t last,
+                                         uint32_t (&vv)[IT]) const
+    {
+        const uint32_t bo = off[td.block];
+        uint8_t prev[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const uint32_t i = min(td.start + e0 + (uint32_t)it * QZ_WAVE, last);
+            prev[it] = text[i > bo ? i - 1u : i];
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const uint32_t i = min(td.start + e0 + (uint32_t)it * QZ_WAVE, last);
+            vv[it] = sa_entry(i, bo, l.cb, l.code, prev[it]);
+        }
+    }
+};
+
+template <typename K, class VS0>
 static int seg_sort_bits(int bits, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t nt,
                          uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s, bool* alt,
-                         bool iota_vals = false, const K* kin = nullptr, const int32_t* seg_delta = nullptr)
+                         VS0 first, const K* kin = nullptr, const int32_t* seg_delta = nullptr)
 {
     if (bits == 11)
-        return seg_radix_sort<K, 11>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
-                                     kin, seg_delta);
+        return seg_radix_sort<K, 11, VS0>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, first,
+                                          kin, seg_delta);
     if (bits == 9)
-        return seg_radix_sort<K, 9>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals,
-                                    kin, seg_delta);
-    return seg_radix_sort<K, 8>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, iota_vals, kin,
-                                seg_delta);
+        return seg_radix_sort<K, 9, VS0>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, first,
+                                         kin, seg_delta);
+    return seg_radix_sort<K, 8, VS0>(k0, v0, k1, v1, tiles, nt, hist_words, key_bits, hist, stmp, s, alt, first, kin,
+                                     seg_delta);
 }
 
 // Radix passes of a key_bits-bit sort with bits-bit digits.
@@ -2045,20 +2081,23 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     uint32_t* kpos = (uint32_t*)groups_[1];
     // the radix ping-pong: (keys0, vals0) and (group pool, sa), both free
     // until the finish writes sa, ranks and groups; the last pass lands in
-    // (keys0, vals0), so with an odd pass count the SA entries start in sa
+    // (keys0, vals0), so with an odd pass count the first pass writes to it
     const bool odd = (radix_passes(32, bits) & 1) != 0;
     uint32_t* pk = (uint32_t*)groups_[0];
     hipLaunchKernelGGL(k_round0_keys, dim3(qz_div_up(n, kThreads * kKeyItems)), dim3(kThreads), 0, s, text, n, bmap_,
-                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos, odd ? sa_ : vals0_);
+                        (const PackParams*)ppd_, (const uint32_t*)brun_, kpos, (uint32_t*)nullptr);
     QZ_HIP(hipGetLastError());
     bool alt = false;
     if (timed) QZ_HIP(hipEventRecord(tev_[0], s));
+    // the first pass makes the SA entries from the text (k_round0_keys
+    // writes only the keys)
+    const SaEntrySource ents{text, (const PackParams*)ppd_, off_};
     if (odd)
         QZ_TRY(seg_sort_bits<uint32_t>(bits, pk, sa_, keys0_, vals0_, (const TileDesc*)tiles0_, ntiles, hist_words, 32,
-                                       bufB_, tmp_, s, &alt, false, kpos));
+                                       bufB_, tmp_, s, &alt, ents, kpos));
     else
         QZ_TRY(seg_sort_bits<uint32_t>(bits, keys0_, vals0_, pk, sa_, (const TileDesc*)tiles0_, ntiles, hist_words, 32,
-                                       bufB_, tmp_, s, &alt, false, kpos));
+                                       bufB_, tmp_, s, &alt, ents, kpos));
     if (timed) QZ_HIP(hipEventRecord(tev_[1], s));
     if (alt != odd) return QZ_GPU_ERROR;  // cannot happen: the last pass writes (keys0, vals0)
     uint32_t* keys = keys0_;
@@ -2347,7 +2386,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             // keys at the groups' slots in bufB (beside the other classes'
             // keys), entries in sa; the other side is (keys0, vals0)
             QZ_TRY(seg_sort_bits<uint32_t>(lbits, bufB_, sa_, keys0_, vals0_, ltiles, lt, lhist, lkey_bits, lhist_, tmp_,
-                                           s, &alt, false, nullptr, ldelta_));
+                                           s, &alt, VFromArray{sa_}, nullptr, ldelta_));
             QZ_HIP(hipEventRecord(tev_[3], s));
             large_timed = true;
             large_bits = lbits;

@@ -219,10 +219,27 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const K* __restrict__
 // iteration. A wave's LDS instructions complete in order, so every lane
 // reads the word after all of the iteration's ORs. Wider digits use one
 // ballot per bit.
-// IOTA: the values are the elements' positions (vin is not read).
-template <typename K, int BITS, bool IOTA>
-__global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin,
-                                                              const uint32_t* __restrict__ vin,
+// VS: where a pass takes its values from. VFromArray reads them; a source
+// may instead compute them from the element's position (the first pass of a
+// sort whose values are a function of position). A source provides its LDS
+// (Lds), fills it before the pass's first barrier (stage) and yields the
+// values of a lane's IT elements (load; lanes past the tile's end get the
+// last element's value).
+struct VFromArray {
+    const uint32_t* v;
+    struct Lds {};
+    __device__ __forceinline__ void stage(Lds&) const {}
+    template <int IT>
+    __device__ __forceinline__ void load(const Lds&, const TileDesc& td, uint32_t e0, uint32_t last,
+                                         uint32_t (&vv)[IT]) const
+    {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) vv[it] = v[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
+    }
+};
+
+template <typename K, int BITS, class VS>
+__global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restrict__ kin, VS vs,
                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                               const TileDesc* __restrict__ tiles, int shift,
                                                               const uint32_t* __restrict__ offs,
@@ -239,6 +256,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     __shared__ uint32_t lstart[D];
     __shared__ uint32_t gbase[D];
     __shared__ uint32_t wtot[W];
+    __shared__ typename VS::Lds vl;
     // [W][D] mask words. For 8-bit digits in an array of their own, which
     // also keeps the workgroup's LDS above 40 KB, so at most 3 workgroups
     // share a CU: with 4 a pass over 1.5e9 pairs took 8.4 ms, with 3 6.0 ms.
@@ -257,6 +275,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 #pragma unroll
             for (int w = 0; w < W; ++w) msk[(uint32_t)w * D + d] = 0ull;
     }
+    vs.stage(vl);
     // The counters are cleared before any load is issued: a workgroup
     // barrier waits for every load in flight, so the tile's loads go out
     // after it and the ranking consumes them as they arrive. Lanes past the
@@ -269,9 +288,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
     uint32_t rk[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) kk[it] = kin[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
-#pragma unroll
-    for (int it = 0; it < IT; ++it)
-        vv[it] = IOTA ? td.start + e0 + (uint32_t)it * QZ_WAVE : vin[min(td.start + e0 + (uint32_t)it * QZ_WAVE, last)];
+    vs.load(vl, td, e0, last, vv);
     // this tile's output start per digit, stored to LDS after the ranking;
     // seg_delta (optional): per segment, where its output lies relative to
     // the packed order the histograms count in
@@ -360,16 +377,17 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(const K* __restric
 // Sorts by key bits [0, key_bits) with BITS-bit digits. `tiles` must have
 // been built with radix 1 << BITS (hist_words from build_tiles). hist holds
 // hist_words, stmp the scan scratch (scan_tmp_words(hist_words)). Result in (k0,v0) for even pass counts.
+// first: the first pass's values (VFromArray{v0} to read them from v0; v0
+// then serves only as a buffer of the later passes).
 // kin (optional): the first pass reads its keys from kin instead of k0,
 // which then serves only as a buffer, so kin is left as it was.
 // seg_delta (optional): segment b's elements are at tile positions (tiles'
 // start) and are sorted into positions shifted by seg_delta[b] from the
 // packed order of the histograms (segments laid out with gaps).
-template <typename K, int BITS>
+template <typename K, int BITS, class VS0>
 static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDesc* tiles, uint32_t ntiles,
                           uint32_t hist_words, int key_bits, uint32_t* hist, uint32_t* stmp, hipStream_t s,
-                          bool* in_alt, bool iota_vals = false, const K* kin = nullptr,
-                          const int32_t* seg_delta = nullptr)
+                          bool* in_alt, VS0 first, const K* kin = nullptr, const int32_t* seg_delta = nullptr)
 {
     *in_alt = false;
     K *ka = k0, *kb = k1;
@@ -380,13 +398,12 @@ static int seg_radix_sort(K* k0, uint32_t* v0, K* k1, uint32_t* v1, const TileDe
         hipLaunchKernelGGL((k_seg_hist<K, BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, tiles, sh, hist);
         QZ_HIP(hipGetLastError());
         QZ_TRY((device_scan<OpAdd, false>(hist, hist, hist_words, stmp, s)));
-        // iota_vals: the first pass takes each key's position as its value
-        if (iota_vals && passes == 0)
-            hipLaunchKernelGGL((k_seg_scatter<K, BITS, true>), dim3(ntiles), dim3(kSortThreads), 0, s, src,
-                               (const uint32_t*)nullptr, kb, vb, tiles, sh, hist, seg_delta);
-        else
-            hipLaunchKernelGGL((k_seg_scatter<K, BITS, false>), dim3(ntiles), dim3(kSortThreads), 0, s, src, va, kb,
+        if (passes == 0)
+            hipLaunchKernelGGL((k_seg_scatter<K, BITS, VS0>), dim3(ntiles), dim3(kSortThreads), 0, s, src, first, kb,
                                vb, tiles, sh, hist, seg_delta);
+        else
+            hipLaunchKernelGGL((k_seg_scatter<K, BITS, VFromArray>), dim3(ntiles), dim3(kSortThreads), 0, s, src,
+                               VFromArray{va}, kb, vb, tiles, sh, hist, seg_delta);
         QZ_HIP(hipGetLastError());
         K* t = ka; ka = kb; kb = t;
         uint32_t* tv = va; va = vb; vb = tv;
