@@ -1285,13 +1285,29 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
     for (int t = 0; t < kSmallTasks; ++t) {
         const uint32_t gwave = wave_g * kSmallTasks + t;
         if (gwave >= plan.wave_base[kSmallClasses]) break;  // wave-uniform
-        int c;
-        uint32_t start = 0, size = 0, e, seg;
-        const bool valid = small_lane(cl.list, cl.cnt, plan, gwave, &c, &start, &size, &e, &seg);
-        uint32_t key = kPadKey, val = 0;
-        if (valid) {
-            key = key2[start + e];
-            val = sa[start + e];
+        // the task's group (clamped to one that exists: a class with waves
+        // has groups), then its keys, suffixes and, in char mode, the block
+        // table entries of the rank stores, all issued before any is used
+        int c = 0;
+        while (c < kSmallClasses - 1 && gwave >= plan.wave_base[c + 1]) ++c;
+        const uint32_t seg = 2u << c;
+        const uint32_t gi = (gwave - plan.wave_base[c]) * (QZ_WAVE / seg) + lane / seg;
+        const uint32_t e = lane & (seg - 1u);
+        const uint32_t ncls = cl.cnt[c];
+        const Grp grp = cl.list[c][min(gi, ncls - 1u)];
+        const uint32_t start = grp.start, size = grp.size;
+        const bool valid = gi < ncls && e < size;
+        const uint32_t slot = start + (valid ? e : 0u);
+        uint32_t key = key2[slot], val = sa[slot];
+        uint32_t blo = 0, bhi = 0;
+        if (cb) {  // uniform
+            const uint32_t k = start >> kTabShift;
+            blo = bm.tab[k];
+            bhi = bm.tab[k + 1] + 1u;
+        }
+        if (!valid) {
+            key = kPadKey;
+            val = 0;
         }
         // bitonic sort inside aligned segments of `seg` lanes, ascending
         for (uint32_t k = 2; k <= seg; k <<= 1) {
@@ -1318,7 +1334,17 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
         }
         if (valid) {
             sa[start + e] = val;
-            if (hp) rank[sa_pos(val, cb, cb ? bm.off[bm.find(start)] : 0u)] = start + hp;  // hp == 0: still the group's start
+            if (hp) {  // hp == 0: still the group's start
+                uint32_t bo = 0;
+                if (cb) {
+                    while (bhi - blo > 1) {  // rare: blocks smaller than a bucket
+                        const uint32_t mid = (blo + bhi) >> 1;
+                        if (bm.off[mid] <= start) blo = mid; else bhi = mid;
+                    }
+                    bo = bm.off[blo];
+                }
+                rank[sa_pos(val, cb, bo)] = start + hp;
+            }
             const bool end = (e + 1 == size) || knext != key;
             if (end && e > hp) {
                 stage_push(S, start + hp, e - hp + 1);
