@@ -46,13 +46,10 @@ int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::v
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
     hipStream_t s = sl.stream;
     QZ_TRY(sl.fwd.reserve(n, nblk));
-    sl.fwd.offs_host_ = offs;
-    QZ_HIP(hipMemcpyAsync(sl.fwd.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
     QZ_HIP(hipEventRecord(sl.ev0, s));
-    QZ_TRY(sl.fwd.encode_device(d_in, d_out, n, nblk, s));
+    std::vector<int32_t> idx;
+    QZ_TRY(sl.fwd.encode_batch(d_in, d_out, offs, s, idx));
     QZ_HIP(hipEventRecord(sl.ev1, s));
-    std::vector<int32_t> idx((size_t)nblk * 256);
-    QZ_HIP(hipMemcpyAsync(idx.data(), sl.fwd.idx_, idx.size() * 4, hipMemcpyDeviceToHost, s));
     QZ_HIP(hipStreamSynchronize(s));
     if (gpu_ms) QZ_HIP(hipEventElapsedTime(gpu_ms, sl.ev0, sl.ev1));
     for (uint32_t b = 0; b < nblk; ++b) {

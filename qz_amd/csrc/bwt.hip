@@ -1831,6 +1831,92 @@ __global__ void k_bwt_index(const uint32_t* __restrict__ rank, BlockMap bm,
         idx[(size_t)b * 256 + k] = (int32_t)(rank[o + k * r] - o + 1u);
 }
 
+// ---- alphabet classes ------------------------------------------------------
+// Round 0 packs one alphabet for the whole batch, so a batch that mixes
+// reads (5 symbols) with names (about 40) packs the reads' keys in radix
+// 41: 5 symbols per key instead of 12, and no 4-bit text or char-mode
+// entries. Keys are only ever compared within a block, so blocks of
+// different classes can be sorted as separate sub-batches with their own
+// packing. k_block_class samples each block (16 bytes every kClassSample)
+// for the symbols present and the run boundaries that decide the key mode;
+// the host groups blocks by class from that. The sample decides speed only:
+// each sub-batch still counts its exact alphabet.
+constexpr uint32_t kClassSample = 512;
+constexpr uint32_t kClassWords = 10;  // per block: 8 presence words, run boundaries, positions compared
+__global__ __launch_bounds__(kThreads) void k_block_class(const uint8_t* __restrict__ text, uint32_t n, BlockMap bm,
+                                                          uint32_t* __restrict__ cls)
+{
+    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kClassSample;
+    const bool live = i0 < n;
+    const uint32_t b = live ? bm.find(i0) : 0xffffffffu;
+    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t cnt = 0, seen = 0;
+    if (live) {
+        const uint32_t end = bm.off[b + 1];
+        uint32_t prev = text[i0];
+        for (uint32_t q = 0; q < 16; ++q) {
+            const uint32_t i = i0 + q;
+            if (i >= end) break;  // the sample stays inside its block
+            const uint32_t c = text[i];
+#pragma unroll
+            for (uint32_t w = 0; w < 8; ++w) m[w] |= (c >> 5) == w ? 1u << (c & 31u) : 0u;
+            if (q) {
+                cnt += c != prev ? 1u : 0u;
+                ++seen;
+            }
+            prev = c;
+        }
+    }
+    const uint32_t b0 = __shfl(b, 0, QZ_WAVE);
+    if (!__ballot(b != b0)) {  // the whole wave samples one block
+        for (int d = 1; d < QZ_WAVE; d <<= 1) {
+#pragma unroll
+            for (uint32_t w = 0; w < 8; ++w) m[w] |= __shfl_xor(m[w], d, QZ_WAVE);
+            cnt += __shfl_xor(cnt, d, QZ_WAVE);
+            seen += __shfl_xor(seen, d, QZ_WAVE);
+        }
+        if (lane_id() != 0 || b0 == 0xffffffffu) return;
+    } else if (!live) {
+        return;
+    }
+    uint32_t* c = cls + (size_t)b * kClassWords;
+#pragma unroll
+    for (uint32_t w = 0; w < 8; ++w)
+        if (m[w]) atomicOr(&c[w], m[w]);
+    if (cnt) atomicAdd(&c[8], cnt);
+    if (seen) atomicAdd(&c[9], seen);
+}
+
+// Copies byte runs between the batch and a class-ordered staging buffer:
+// one workgroup per run of at most kCopyChunk bytes, any alignment. Stores
+// are aligned 32-bit words; each word's source bytes come from the two
+// aligned words that hold them (the second one always holds a byte of the
+// run, so no load leaves the source's pages).
+struct CopyRun {
+    uint32_t dst, src, len;
+};
+constexpr uint32_t kCopyChunk = 65536;
+__global__ __launch_bounds__(kThreads) void k_copy_runs(const CopyRun* __restrict__ runs,
+                                                        const uint8_t* __restrict__ src, uint8_t* __restrict__ dst)
+{
+    const CopyRun r = runs[blockIdx.x];
+    const uint8_t* s = src + r.src;
+    uint8_t* d = dst + r.dst;
+    const uint32_t head = min((uint32_t)((4u - ((uintptr_t)d & 3u)) & 3u), r.len);
+    if (threadIdx.x < head) d[threadIdx.x] = s[threadIdx.x];
+    const uint32_t nw = (r.len - head) / 4u;
+    const uint8_t* s1 = s + head;
+    uint32_t* d1 = reinterpret_cast<uint32_t*>(d + head);
+    const uint32_t sh = (uint32_t)((uintptr_t)s1 & 3u) * 8u;
+    const uint32_t* sw = reinterpret_cast<const uint32_t*>((uintptr_t)s1 & ~(uintptr_t)3);
+    if (sh == 0) {
+        for (uint32_t w = threadIdx.x; w < nw; w += kThreads) d1[w] = sw[w];
+    } else {
+        for (uint32_t w = threadIdx.x; w < nw; w += kThreads) d1[w] = (sw[w] >> sh) | (sw[w + 1] << (32u - sh));
+    }
+    for (uint32_t i = head + nw * 4u + threadIdx.x; i < r.len; i += kThreads) d[i] = s[i];
+}
+
 // ---- digit-width autotuner -------------------------------------------------
 // The two radix sorts (round 0 and large groups) choose their digit width:
 // 11 bits saves a pass on 32-bit keys, 8 bits keeps digit runs long enough
@@ -1993,6 +2079,8 @@ static int bitlen(uint64_t v)
 constexpr int kCntB = 32, kTopOff = kClasses, kOverflow = 2 * kClasses, kFill = 48;
 
 static uint64_t pool_cap(uint64_t n) { return n / 2 + 1; }
+// Copy runs of one direction of a class split: blocks cut at kCopyChunk.
+static uint64_t copy_runs_cap(uint64_t n, uint32_t nblk) { return n / kCopyChunk + nblk + 1; }
 
 uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 {
@@ -2009,7 +2097,8 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     return 2 * (c + 64) + 4 * 4 * c + 2 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
            3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 + 2 * groups * sizeof(Grp) +
            nlg * (sizeof(Grp) + 4) + tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc) +
-           sizeof(PackParams) + ((c >> kTabShift) + 2) * 4;
+           sizeof(PackParams) + ((c >> kTabShift) + 2) * 4 + (uint64_t)cb * kClassWords * 4 +
+           2 * copy_runs_cap(c, cb) * sizeof(CopyRun);
 }
 
 // Depth arrays, allocated for the engine's first batch with run-length keys
@@ -2056,6 +2145,8 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
     QZ_HIP(hipMalloc(&btab_, ((c >> kTabShift) + 2) * 4));
+    QZ_HIP(hipMalloc(&cls_, (size_t)cb * kClassWords * 4));
+    QZ_HIP(hipMalloc(&runs_, 2 * copy_runs_cap(c, cb) * sizeof(CopyRun)));
     tiles_host_.resize(tile_cap);
     QZ_HIP(hipHostMalloc(&pinned_, 4096));
     for (auto& ev : tev_)
@@ -2069,7 +2160,7 @@ void BwtEngine::release()
 {
     void* ps[] = {text_,      out_,       sa_,    rank_,  bufA_,   bufB_, keys0_, vals0_, dep_[0], dep_[1], tmp_,
                   off_,       idx_,       hist_,  brun_,  cnt_,    groups_[0], groups_[1], lgrp_,   ldelta_,
-                  lhist_,     tiles_,     tiles0_, ppd_,  btab_};
+                  lhist_,     tiles_,     tiles0_, ppd_,  btab_,  cls_,    runs_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -2083,6 +2174,8 @@ void BwtEngine::release()
     groups_[0] = groups_[1] = nullptr;
     tiles_ = tiles0_ = ppd_ = nullptr;
     btab_ = nullptr;
+    cls_ = nullptr;
+    runs_ = nullptr;
     idx_ = nullptr;
     pinned_ = nullptr;
     cap_ = 0;
@@ -2506,6 +2599,158 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
         QZ_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_bwt_index, dim3(nblk), dim3(64), 0, s, rank_, bmap_, idx_);
+    QZ_HIP(hipGetLastError());
+    return 0;
+}
+
+// QZBSC_CLASSES=0 sorts every batch as one, whatever its blocks' alphabets.
+static bool classes_enabled()
+{
+    const char* e = getenv("QZBSC_CLASSES");
+    return !(e && e[0] == '0');
+}
+constexpr uint32_t kSplitMinBytes = 8u << 20;   // smaller batches are never split
+constexpr uint64_t kClassMinBytes = 4u << 20;   // smaller classes are sorted together
+
+// Symbols per packed 32-bit key for sigma symbols (as k_make_pack).
+static uint32_t packed_k0(uint32_t sigma)
+{
+    const uint64_t radix = (uint64_t)sigma + 1;
+    uint64_t v = 1;
+    uint32_t k = 0;
+    while (k < 32 && v * radix <= (1ull << 32)) {
+        v *= radix;
+        ++k;
+    }
+    return k ? k : 1;
+}
+
+// Groups the blocks of the batch at offs by the round-0 packing they would
+// get alone: run-length keys (by code width), or packed keys with at most
+// 15 symbols (4-bit text, char-mode entries) or more, each by symbols per
+// key. Classes under kClassMinBytes, and blocks too small to be sampled,
+// are put together. classes is left empty when the batch forms one class.
+int BwtEngine::classify(const uint8_t* text, const std::vector<uint32_t>& offs, hipStream_t s,
+                        std::vector<std::vector<uint32_t>>& classes)
+{
+    classes.clear();
+    const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
+    build_block_tab(offs, n, btab_host_);
+    QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipMemcpyAsync(off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+    QZ_HIP(hipMemsetAsync(cls_, 0, (size_t)nblk * kClassWords * 4, s));
+    hipLaunchKernelGGL(k_block_class, dim3(qz_div_up(n, (uint64_t)kThreads * kClassSample)), dim3(kThreads), 0, s,
+                       text, n, BlockMap{off_, btab_, nblk}, cls_);
+    QZ_HIP(hipGetLastError());
+    std::vector<uint32_t> h((size_t)nblk * kClassWords);
+    QZ_HIP(hipMemcpyAsync(h.data(), cls_, h.size() * 4, hipMemcpyDeviceToHost, s));
+    QZ_HIP(hipStreamSynchronize(s));
+    const int force = rle_force();
+    std::vector<std::pair<uint32_t, uint32_t>> keyed(nblk);  // (class key, block)
+    constexpr uint32_t kMisc = 0xffffffffu;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const uint32_t* c = &h[(size_t)b * kClassWords];
+        uint32_t sigma = 0;
+        for (int w = 0; w < 8; ++w) sigma += (uint32_t)__builtin_popcount(c[w]);
+        const uint32_t runs = c[8], seen = c[9];
+        uint32_t key = kMisc;
+        if (sigma) {
+            const bool rle = force >= 0 ? force != 0 : (seen >= 64u && (uint64_t)seen >= 3ull * ((uint64_t)runs + 1));
+            if (rle)
+                key = 0x100u | (sigma <= 31 ? 0u : 1u);  // 3 or 2 run segments per key
+            else
+                key = (sigma <= 15 ? 0x200u : 0x300u) | packed_k0(sigma);
+        }
+        keyed[b] = {key, b};
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                         return a.first < b.first;
+                     });
+    std::vector<uint32_t> misc;
+    for (size_t i = 0; i < keyed.size();) {
+        size_t j = i;
+        uint64_t bytes = 0;
+        for (; j < keyed.size() && keyed[j].first == keyed[i].first; ++j)
+            bytes += offs[keyed[j].second + 1] - offs[keyed[j].second];
+        std::vector<uint32_t> blocks;
+        for (size_t k = i; k < j; ++k) blocks.push_back(keyed[k].second);
+        if (keyed[i].first == kMisc || bytes < kClassMinBytes)
+            misc.insert(misc.end(), blocks.begin(), blocks.end());
+        else
+            classes.push_back(std::move(blocks));
+        i = j;
+    }
+    if (!misc.empty()) {
+        std::sort(misc.begin(), misc.end());
+        classes.push_back(std::move(misc));
+    }
+    if (classes.size() <= 1) classes.clear();
+    return 0;
+}
+
+int BwtEngine::encode_batch(const uint8_t* text, uint8_t* out, const std::vector<uint32_t>& offs, hipStream_t s,
+                            std::vector<int32_t>& idx)
+{
+    const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
+    if (n > cap_ || nblk > cap_blk_) return QZ_BAD_PARAMETER;
+    idx.assign((size_t)nblk * 256, 0);
+    // staging of a split: T holds the blocks in class order, O their BWTs;
+    // T is written while text is read, O is read while out is written
+    uint8_t* T = (text == text_) ? out_ : text_;
+    uint8_t* O = (T == text_) ? out_ : text_;
+    std::vector<std::vector<uint32_t>> classes;
+    if (classes_enabled() && nblk >= 2 && n >= kSplitMinBytes && T != text && O != out)
+        QZ_TRY(classify(text, offs, s, classes));
+    if (classes.empty()) {
+        offs_host_ = offs;
+        QZ_HIP(hipMemcpyAsync(off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+        QZ_TRY(encode_device(text, out, n, nblk, s));
+        QZ_HIP(hipMemcpyAsync(idx.data(), idx_, idx.size() * 4, hipMemcpyDeviceToHost, s));
+        QZ_HIP(hipStreamSynchronize(s));
+        return 0;
+    }
+    // each block's place in T, and the copy runs into T and back out
+    std::vector<uint32_t> at(nblk);
+    std::vector<CopyRun> runs;
+    {
+        uint32_t p = 0;
+        for (const auto& c : classes)
+            for (uint32_t b : c) {
+                at[b] = p;
+                p += offs[b + 1] - offs[b];
+            }
+        if (p != n) return QZ_BAD_PARAMETER;  // every block in exactly one class
+        for (int dir = 0; dir < 2; ++dir)
+            for (uint32_t b = 0; b < nblk; ++b)
+                for (uint32_t o = 0, len = offs[b + 1] - offs[b]; o < len; o += kCopyChunk) {
+                    const uint32_t l = std::min(kCopyChunk, len - o);
+                    runs.push_back(dir == 0 ? CopyRun{at[b] + o, offs[b] + o, l} : CopyRun{offs[b] + o, at[b] + o, l});
+                }
+    }
+    const uint32_t nrun = (uint32_t)runs.size() / 2;
+    if (nrun > copy_runs_cap(cap_, cap_blk_)) return QZ_GPU_ERROR;  // cannot happen: blocks within the caps
+    QZ_HIP(hipMemcpyAsync(runs_, runs.data(), runs.size() * sizeof(CopyRun), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_copy_runs, dim3(nrun), dim3(kThreads), 0, s, (const CopyRun*)runs_, text, T);
+    QZ_HIP(hipGetLastError());
+    std::vector<int32_t> part;
+    for (const auto& c : classes) {
+        std::vector<uint32_t> lo(c.size() + 1, 0);
+        for (size_t i = 0; i < c.size(); ++i) lo[i + 1] = lo[i] + (offs[c[i] + 1] - offs[c[i]]);
+        offs_host_ = lo;
+        QZ_HIP(hipMemcpyAsync(off_, offs_host_.data(), offs_host_.size() * 4, hipMemcpyHostToDevice, s));
+        const uint32_t base = at[c[0]];
+        QZ_TRY(encode_device(T + base, O + base, lo.back(), (uint32_t)c.size(), s));
+        part.resize(c.size() * 256);
+        QZ_HIP(hipMemcpyAsync(part.data(), idx_, part.size() * 4, hipMemcpyDeviceToHost, s));
+        QZ_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < c.size(); ++i)
+            std::copy(part.begin() + i * 256, part.begin() + (i + 1) * 256, idx.begin() + (size_t)c[i] * 256);
+        if (getenv("QZBSC_TRACE"))
+            fprintf(stderr, "qzbsc: class of %zu blocks, %u bytes, k0 %u, char mode %u\n", c.size(), lo.back(), k0_,
+                    cb_);
+    }
+    hipLaunchKernelGGL(k_copy_runs, dim3(nrun), dim3(kThreads), 0, s, (const CopyRun*)runs_ + nrun, O, out);
     QZ_HIP(hipGetLastError());
     return 0;
 }

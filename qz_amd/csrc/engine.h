@@ -15,6 +15,14 @@ public:
     // off_[0..nblk] must already hold the batch offsets; text/out are device
     // buffers of n bytes (text_/out_ serve as staging for host callers).
     int encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint32_t nblk, hipStream_t s);
+    // Forward BWT of the batch at offs (reserve() done): idx receives every
+    // block's primary and sampled indexes (nblk x 256, the layout of idx_).
+    // Blocks of different alphabet classes run as separate sub-batches,
+    // staged class by class in text_ / out_ (whichever text and out are
+    // not; when text is text_, its contents are overwritten). Returns with
+    // the stream's work enqueued up to the last copy into out.
+    int encode_batch(const uint8_t* text, uint8_t* out, const std::vector<uint32_t>& offs, hipStream_t s,
+                     std::vector<int32_t>& idx);
     // Device bytes reserve(n, nblk) allocates (admission control).
     static uint64_t footprint(uint64_t n, uint32_t nblk);
 
@@ -33,6 +41,8 @@ public:
 
 private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);
+    int classify(const uint8_t* text, const std::vector<uint32_t>& offs, hipStream_t s,
+                 std::vector<std::vector<uint32_t>>& classes);
     int enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, int bits, uint32_t ntiles,
                        uint32_t hist_words, bool timed, uint32_t fuse, int fshape, hipStream_t s);
     void* tiles0_ = nullptr;       // round-0 radix tiles (device; read by the round-0 graph)
@@ -51,6 +61,8 @@ private:
     int32_t* ldelta_ = nullptr;    // per large group: its slot start minus its packed offset
     int ensure_depths();
     uint32_t* tmp_ = nullptr;
+    uint32_t* cls_ = nullptr;      // per block: sampled symbols and runs (k_block_class)
+    void* runs_ = nullptr;         // copy runs of a class split: into the staging, then back
     uint32_t* hist_ = nullptr;
     uint32_t* brun_ = nullptr;     // run boundaries per block (round-0 key mode)
     uint32_t* cnt_ = nullptr;      // group-list counters (two sets) + overflow flag
