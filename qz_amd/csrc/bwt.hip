@@ -1609,12 +1609,16 @@ __global__ __launch_bounds__(WAVES * QZ_WAVE) void k_sort_lds(const Grp* __restr
 // entries in sa, the other side of the ping-pong slot-indexed as well (tiles
 // never span two groups; td.block is the group's index in lg, td.start a
 // slot).
+// mm[2g], mm[2g + 1]: the smallest and largest key of group g (cleared to
+// all ones and zero before the launch). A group whose keys are all equal
+// does not split this round: its members' successors all lie in one group.
 __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restrict__ tiles,
                                                          const Grp* __restrict__ lg, const uint32_t* __restrict__ sa,
                                                          const uint32_t* __restrict__ rank, BlockMap bm, uint32_t cb,
                                                          uint32_t h, const uint32_t* __restrict__ dep,
-                                                         uint32_t* __restrict__ keys)
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ mm)
 {
+    __shared__ uint32_t wmin[kThreads / QZ_WAVE], wmax[kThreads / QZ_WAVE];
     const TileDesc td = tiles[blockIdx.x];
     const uint32_t g0 = lg[td.block].start;
     // a group lies in one block, whose text positions and SA slots are the
@@ -1622,11 +1626,65 @@ __global__ __launch_bounds__(kThreads) void k_fill_large(const TileDesc* __restr
     const uint32_t blk = bm.find(g0);
     const uint32_t bstart = bm.off[blk], bend = bm.off[blk + 1];
     const uint32_t d = dep ? dep[g0] : h;
+    uint32_t kmin = 0xffffffffu, kmax = 0u;
     for (uint32_t i = threadIdx.x; i < td.len; i += kThreads) {
         const uint32_t slot = td.start + i;
         const uint64_t p = (uint64_t)sa_pos(sa[slot], cb, bstart) + d;
-        keys[slot] = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
+        const uint32_t k = (p < bend) ? rank[(uint32_t)p] - bstart + 1u : 0u;
+        keys[slot] = k;
+        kmin = min(kmin, k);
+        kmax = max(kmax, k);
     }
+    for (int o = 1; o < QZ_WAVE; o <<= 1) {
+        kmin = min(kmin, (uint32_t)__shfl_xor(kmin, o, QZ_WAVE));
+        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, o, QZ_WAVE));
+    }
+    const uint32_t w = threadIdx.x / QZ_WAVE;
+    if (lane_id() == 0) {
+        wmin[w] = kmin;
+        wmax[w] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t v = 1; v < kThreads / QZ_WAVE; ++v) {
+            kmin = min(kmin, wmin[v]);
+            kmax = max(kmax, wmax[v]);
+        }
+        atomicMin(&mm[2 * td.block], kmin);
+        atomicMax(&mm[2 * td.block + 1], kmax);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_minmax_init(uint32_t* __restrict__ mm, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        mm[2 * i] = 0xffffffffu;
+        mm[2 * i + 1] = 0u;
+    }
+}
+
+// The large groups that do not split this round (mm from k_fill_large) go
+// to the next round as they are: same members, same SA order and ranks, a
+// depth grown by that of the group their keys point into.
+__global__ __launch_bounds__(kThreads) void k_keep_large(const Grp* __restrict__ lg, uint32_t nlarge,
+                                                         const uint32_t* __restrict__ mm, GroupLists gl, BlockMap bm,
+                                                         const uint32_t* __restrict__ dep_in,
+                                                         uint32_t* __restrict__ dep_out)
+{
+    __shared__ StageT<kThreads> S;
+    stage_init(S);
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < nlarge && mm[2 * i] == mm[2 * i + 1]) {
+        const Grp g = lg[i];
+        stage_push_t(S, g.start, g.size, kThreads);
+        if (dep_out) {
+            const uint32_t k = mm[2 * i], bstart = bm.off[bm.find(g.start)];
+            dep_out[g.start] = sub_depth(dep_in, g.start, k ? k - 1u + bstart + 1u : 0u);
+        }
+    }
+    stage_flush_t(S, gl, kThreads);
 }
 
 // Large-group finish, one radix tile per workgroup, over the sorted keys and
@@ -2096,7 +2154,7 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
     // histograms (9-bit digits at most), their list, offsets and shifts
     return 2 * (c + 64) + 4 * 4 * c + 2 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
            3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 + 2 * groups * sizeof(Grp) +
-           nlg * (sizeof(Grp) + 4) + tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc) +
+           nlg * (sizeof(Grp) + 4 + 8) + tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc) +
            sizeof(PackParams) + ((c >> kTabShift) + 2) * 4 + (uint64_t)cb * kClassWords * 4 +
            2 * copy_runs_cap(c, cb) * sizeof(CopyRun);
 }
@@ -2141,6 +2199,9 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
     QZ_HIP(hipMalloc(&lgrp_, (c / (kMedMax + 1) + 2) * sizeof(Grp)));
     QZ_HIP(hipMalloc(&ldelta_, (c / (kMedMax + 1) + 2) * 4));
+    QZ_HIP(hipMalloc(&lmm_, (c / (kMedMax + 1) + 2) * 8));
+    QZ_HIP(hipHostMalloc((void**)&lmm_host_, (c / (kMedMax + 1) + 2) * 8));
+    if (!lev_) QZ_HIP(hipEventCreateWithFlags(&lev_, hipEventDisableTiming));
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&ppd_, sizeof(PackParams)));
@@ -2160,11 +2221,13 @@ void BwtEngine::release()
 {
     void* ps[] = {text_,      out_,       sa_,    rank_,  bufA_,   bufB_, keys0_, vals0_, dep_[0], dep_[1], tmp_,
                   off_,       idx_,       hist_,  brun_,  cnt_,    groups_[0], groups_[1], lgrp_,   ldelta_,
-                  lhist_,     tiles_,     tiles0_, ppd_,  btab_,  cls_,    runs_};
+                  lhist_,     tiles_,     tiles0_, ppd_,  btab_,  cls_,    runs_,  lmm_};
     g0_.reset();
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (pinned_) (void)hipHostFree(pinned_);
+    if (lmm_host_) (void)hipHostFree(lmm_host_);
+    lmm_ = lmm_host_ = nullptr;
     text_ = out_ = nullptr;
     sa_ = rank_ = bufA_ = bufB_ = vals0_ = tmp_ = off_ = hist_ = brun_ = cnt_ = lhist_ = nullptr;
     keys0_ = nullptr;
@@ -2364,6 +2427,11 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     int large_bits = 8;
     uint64_t large_n = 0;
     uint32_t* hc = (uint32_t*)pinned_;
+    // large-group lists of a round; their uploads are read before the next
+    // round's synchronisation, so they live outside the loop
+    std::vector<Grp> lg, lg_split;
+    std::vector<uint32_t> lb;
+    std::vector<int32_t> delta;
     for (;;) {
         // one copy brings back the current set's class counts and the
         // overflow word, which lies between the two sets in cnt_
@@ -2469,8 +2537,6 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         // tiles at the groups' own slots
         uint32_t M = 0, lt = 0, lhist = 0;
         int lbits = 8;
-        std::vector<Grp> lg;
-        std::vector<uint32_t> lb;
         if (nlarge) {
             lg.resize(nlarge);
             QZ_HIP(hipMemcpyAsync(lg.data(), cl.list[kClsLarge], nlarge * sizeof(Grp), hipMemcpyDeviceToHost, s));
@@ -2498,13 +2564,12 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt, 1u << lbits);
             // tiles at the groups' slots; the sort writes each group shifted
             // from its packed offset to its slot start
-            std::vector<int32_t> delta(nlarge);
+            delta.resize(nlarge);
             for (uint32_t i = 0; i < nlarge; ++i) delta[i] = (int32_t)(lg[i].start - lb[i]);
             for (uint32_t t = 0; t < lt; ++t) tiles_host_[t].start += (uint32_t)delta[tiles_host_[t].block];
             QZ_HIP(hipMemcpyAsync(lgrp_, lg.data(), nlarge * sizeof(Grp), hipMemcpyHostToDevice, s));
             QZ_HIP(hipMemcpyAsync(ldelta_, delta.data(), nlarge * 4, hipMemcpyHostToDevice, s));
             QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
-            QZ_HIP(hipStreamSynchronize(s));  // delta and lg leave scope before the copies would be read otherwise
         }
         const TileDesc* ltiles = (const TileDesc*)tiles_;
         // group depths: read from this round's buffer (written when the
@@ -2530,9 +2595,14 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (M) {
+            // key range per group, copied back while the small classes sort
+            hipLaunchKernelGGL(k_fill_minmax_init, dim3(qz_div_up(nlarge, kThreads)), dim3(kThreads), 0, s, lmm_,
+                               nlarge);
             hipLaunchKernelGGL(k_fill_large, dim3(lt), dim3(kThreads), 0, s, ltiles, (const Grp*)lgrp_, sa_, rank_,
-                               bmap_, cb, h, dep_in, bufB_);
+                               bmap_, cb, h, dep_in, bufB_, lmm_);
             QZ_HIP(hipGetLastError());
+            QZ_HIP(hipMemcpyAsync(lmm_host_, lmm_, (size_t)nlarge * 8, hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipEventRecord(lev_, s));
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
         if (small_waves) {
@@ -2549,6 +2619,45 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, bufB_,
                                sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
+        }
+        if (M) {
+            // groups whose keys are all equal go on unsplit; the others are
+            // sorted at their slots (tiles, offsets and list rebuilt for them)
+            QZ_HIP(hipEventSynchronize(lev_));
+            lg_split.clear();
+            uint64_t kept_elems = 0;
+            for (uint32_t i = 0; i < nlarge; ++i) {
+                if (lmm_host_[2 * i] == lmm_host_[2 * i + 1])
+                    kept_elems += lg[i].size;
+                else
+                    lg_split.push_back(lg[i]);
+            }
+            if (trace_)
+                fprintf(stderr, "qzbsc:   large groups unsplit %u of %u (%llu of %u suffixes)\n",
+                        nlarge - (uint32_t)lg_split.size(), nlarge, (unsigned long long)kept_elems, M);
+            if (lg_split.size() < nlarge) {
+                hipLaunchKernelGGL(k_keep_large, dim3(qz_div_up(nlarge, kThreads)), dim3(kThreads), 0, s,
+                                   (const Grp*)lgrp_, nlarge, lmm_, N, bmap_, dep_in, dep_out);
+                QZ_HIP(hipGetLastError());
+                const uint32_t ns = (uint32_t)lg_split.size();
+                M = 0;
+                lb.resize(ns + 1);
+                for (uint32_t i = 0; i < ns; ++i) {
+                    lb[i] = M;
+                    M += lg_split[i].size;
+                }
+                lb[ns] = M;
+                if (ns) {
+                    lhist = build_tiles(lb.data(), ns, tiles_host_.data(), &lt, 1u << lbits);
+                    delta.resize(ns);
+                    for (uint32_t i = 0; i < ns; ++i) delta[i] = (int32_t)(lg_split[i].start - lb[i]);
+                    for (uint32_t t = 0; t < lt; ++t) tiles_host_[t].start += (uint32_t)delta[tiles_host_[t].block];
+                    // stream order: k_keep_large has read the full list first
+                    QZ_HIP(hipMemcpyAsync(lgrp_, lg_split.data(), ns * sizeof(Grp), hipMemcpyHostToDevice, s));
+                    QZ_HIP(hipMemcpyAsync(ldelta_, delta.data(), ns * 4, hipMemcpyHostToDevice, s));
+                    QZ_HIP(hipMemcpyAsync(tiles_, tiles_host_.data(), lt * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+                }
+            }
         }
         if (M) {
             QZ_HIP(hipEventRecord(tev_[2], s));

@@ -59,6 +59,9 @@ private:
     uint32_t* lhist_ = nullptr;    // histograms of the large-group sort
     void* lgrp_ = nullptr;         // large groups of a round in slot order (device)
     int32_t* ldelta_ = nullptr;    // per large group: its slot start minus its packed offset
+    uint32_t* lmm_ = nullptr;      // per large group: smallest and largest key of the round
+    uint32_t* lmm_host_ = nullptr; // pinned copy of lmm_
+    hipEvent_t lev_ = nullptr;     // large-group keys gathered (lmm_host_ filled)
     int ensure_depths();
     uint32_t* tmp_ = nullptr;
     uint32_t* cls_ = nullptr;      // per block: sampled symbols and runs (k_block_class)
