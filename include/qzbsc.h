@@ -231,6 +231,20 @@ QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
  * into ns_per_elem[3]; returns the number of trials. */
 QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns_per_elem);
 
+/* Launch-shape tuner, every knob (no counterpart in the reference): knob 0
+ * round-0 digit width (8, 11), 1 large-group digit width (8, 9), 2 tile
+ * shape of the fused round-0 finish (QZBSC_FUSE_SHAPE codes: 0 = 512
+ * threads per 4096-slot tile, 5 = 1024, both with the tile's suffixes in
+ * LDS), 3 LDS budget of the small-group sort in bytes per workgroup (16384,
+ * 0; it caps the workgroups sharing a CU). Each knob rotates its candidates
+ * call by call on calls with enough work, timed with HIP events around its
+ * own kernels, and keeps the fastest per unit of work (elements; for knob 3
+ * small-group slots). Writes the choice (-1 while undecided), the number of
+ * candidates, the candidates (cand[4]) and their mean ns per unit
+ * (ns_per_unit[4]); returns the number of timed trials, or -1 for an
+ * unknown knob. */
+QZBSC_API int qzbsc_tune_knob(int knob, int* choice, int* ncand, int* cand, double* ns_per_unit);
+
 /* HIP graph replay counts of this process (no reference counterpart).
  * Fixed launch sequences (the inverse BWT, round 0 of the forward sort) are
  * captured the second time in a row that an engine sees a batch shape and

@@ -57,6 +57,8 @@ _SIGS = {
     "qzbsc_server_peak_bytes": (C.c_longlong, [C.c_void_p]),
     "qzbsc_footprint": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
     "qzbsc_tuning": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "qzbsc_tune_knob": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]),
     "qzbsc_graph_stats": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "qzbsc_trim": (C.c_longlong, []),
     "qzbsc_lzp_compress": (C.c_int, [u8p, u8p, C.c_int, C.c_int, C.c_int]),

@@ -120,14 +120,23 @@ def dev_decode_batch(d_in_ptr: int, d_out_ptr: int, sizes, primaries):
     return ms.value
 
 
+_KNOBS = ((0, "round0_digit_bits", "element"), (1, "large_group_digit_bits", "element"),
+          (2, "fused_finish_shape", "element"), (3, "sort_small_lds_bytes", "small-group slot"))
+_SHAPES = {0: "512 threads, suffixes in LDS", 5: "1024 threads, suffixes in LDS"}
+
+
 def tuning():
-    """Radix digit-width tuner state for the round-0 and large-group sorts."""
+    """Launch-shape tuner state: per knob the chosen value (None while
+    undecided), the timed trials and the mean ns per unit of each candidate."""
     res = {}
-    for which, name in ((0, "round0"), (1, "large_groups")):
-        ch, ns = C.c_int(0), (C.c_double * 3)()
-        trials = lib().qzbsc_tuning(which, C.byref(ch), ns)
-        res[name] = {"digit_bits": ch.value or None, "trials": trials,
-                     "ns_per_elem": {str(w): round(ns[k], 4) for k, w in enumerate((8, 9, 11)) if ns[k] > 0}}
+    for knob, name, unit in _KNOBS:
+        ch, nc = C.c_int(0), C.c_int(0)
+        cand, ns = (C.c_int * 4)(), (C.c_double * 4)()
+        trials = lib().qzbsc_tune_knob(knob, C.byref(ch), C.byref(nc), cand, ns)
+        label = (lambda v: _SHAPES.get(v, str(v))) if knob == 2 else str
+        res[name] = {"choice": None if ch.value < 0 else (_SHAPES.get(ch.value, ch.value) if knob == 2 else ch.value),
+                     "trials": trials, "unit": unit,
+                     "ns_per_unit": {label(cand[k]): round(ns[k], 5) for k in range(nc.value) if ns[k] > 0}}
     return res
 
 

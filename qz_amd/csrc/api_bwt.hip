@@ -184,6 +184,13 @@ QZBSC_API int qzbsc_tuning(int which, int* choice, double* ns_per_elem)
     return tune_state(which, choice, ns_per_elem);
 }
 
+QZBSC_API int qzbsc_tune_knob(int knob, int* choice, int* ncand, int* cand, double* ns_per_unit)
+{
+    if (knob < 0 || knob > 3) return -1;
+    if (!choice || !ncand || !cand || !ns_per_unit) return QZ_BAD_PARAMETER;
+    return tune_knob(knob, choice, ncand, cand, ns_per_unit);
+}
+
 QZBSC_API int qzbsc_graph_stats(unsigned long long* captures, unsigned long long* replays)
 {
     if (!captures || !replays) return QZ_BAD_PARAMETER;

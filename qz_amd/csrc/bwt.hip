@@ -1975,99 +1975,149 @@ __global__ __launch_bounds__(kThreads) void k_copy_runs(const CopyRun* __restric
     for (uint32_t i = head + nw * 4u + threadIdx.x; i < r.len; i += kThreads) d[i] = s[i];
 }
 
-// ---- digit-width autotuner -------------------------------------------------
-// The two radix sorts (round 0 and large groups) choose their digit width:
-// 11 bits saves a pass on 32-bit keys, 8 bits keeps digit runs long enough
-// for coalesced stores, and 9 bits serves the large-group sort, whose
-// block-local keys need at most 27 bits for blocks of up to 128 MiB (3
-// passes where 8-bit digits take 4; round 0's 32-bit keys gain nothing from
-// it). Every width gives the same order, so the tuner rotates the candidates
-// ({8, 11} for round 0, {8, 9} for large groups, whose histograms have a
-// buffer sized for 9-bit digits) on the first sorts of at least kTuneMin
-// elements, times them with HIP events, and keeps the fastest per element for
-// the process. QZBSC_RADIX_BITS=8|9|11 forces one width for both sorts (at
-// most 9 for large groups).
-constexpr int kWidths[3] = {8, 9, 11};
+// ---- launch-shape autotuner ---------------------------------------------
+// Knobs that change speed and never a result, each chosen per process by
+// timing its candidates on the first calls with enough work:
+//   0 round-0 digit width {8, 11}: 11 bits saves a pass on 32-bit keys,
+//     8 bits keeps digit runs long enough for coalesced stores;
+//   1 large-group digit width {8, 9}: block-local keys need at most 27 bits
+//     for blocks of up to 128 MiB (3 passes where 8-bit digits take 4; the
+//     histograms have a buffer sized for 9-bit digits);
+//   2 tile shape of the fused round-0 finish {512, 1024 threads per 4096-slot
+//     tile}, both with the tile's suffixes held in LDS (the shapes without
+//     them, and 256 threads, measured no better);
+//   3 LDS budget of k_sort_small {16 KB, none}: extra LDS per workgroup only
+//     caps how many of its random-access waves share a CU.
+// A knob rotates its candidates call by call, timed with HIP events on the
+// engine's stream around its own kernels only, and keeps the fastest per
+// unit of work once every candidate has run. QZBSC_RADIX_BITS=8|9|11 (both
+// widths; at most 9 for large groups), QZBSC_FUSE_SHAPE and
+// QZBSC_LDS_SORT_SMALL force a choice.
+static int fuse_shape();
+enum TuneKnob { kKnobRound0Bits, kKnobLargeBits, kKnobFuseShape, kKnobSortSmallLds, kKnobCount };
 constexpr int kLargeRadixBits = 9;  // the large-group sort's widest digit (its histograms have their own buffer)
-struct RadixTuner {
+constexpr int kKnobMaxCand = 4;
+struct Knob {
     std::mutex mu;
-    int choice = 0;
-    double ns[3] = {0, 0, 0};
-    double elems[3] = {0, 0, 0};
+    int choice = -1;  // candidate index, -1 while undecided
+    double ns[kKnobMaxCand] = {0, 0, 0, 0};
+    double units[kKnobMaxCand] = {0, 0, 0, 0};
     int trials = 0;
-    int ncand = 2;
-    int cand[3] = {8, 11, 0};
+    int ncand = 0;
+    int cand[kKnobMaxCand] = {0, 0, 0, 0};
+    uint64_t min_units = 0;  // calls with less work are neither timed nor rotated
 };
-RadixTuner g_tune[2];  // [0] round 0, [1] large groups
-constexpr uint64_t kTuneMin = 1u << 22;
+Knob g_knob[kKnobCount];
 static std::once_flag g_tune_once;
 
 static void tune_init()
 {
     std::call_once(g_tune_once, [] {
-        g_tune[1].ncand = 2;
-        for (int k = 0; k < 2; ++k) g_tune[1].cand[k] = kWidths[k];
+        auto set = [](int k, std::initializer_list<int> c, uint64_t mn) {
+            g_knob[k].ncand = 0;
+            for (int v : c) g_knob[k].cand[g_knob[k].ncand++] = v;
+            g_knob[k].min_units = mn;
+        };
+        set(kKnobRound0Bits, {8, 11}, 1u << 22);      // elements
+        set(kKnobLargeBits, {8, 9}, 1u << 22);        // elements
+        set(kKnobFuseShape, {0, 5}, 1u << 22);        // fuse_shape() codes; elements
+        set(kKnobSortSmallLds, {16384, 0}, 1u << 16);  // waves of small-class work per call
     });
 }
 
-static int tune_pick(int which, uint64_t n)
+// The forced value of a knob (from its environment variable), or -1.
+static int tune_forced(int which)
+{
+    const char* e = nullptr;
+    switch (which) {
+        case kKnobRound0Bits:
+        case kKnobLargeBits:
+            if ((e = getenv("QZBSC_RADIX_BITS"))) {
+                const int v = atoi(e);
+                if (v == 8 || v == 9 || v == 11) return v;
+            }
+            return -1;
+        case kKnobFuseShape: return getenv("QZBSC_FUSE_SHAPE") ? fuse_shape() : -1;
+        case kKnobSortSmallLds: return getenv(kOccEnv[kOccSortSmall]) ? (int)occ_lds(kOccSortSmall) : -1;
+        default: return -1;
+    }
+}
+
+// The value to use for a call with `units` of work.
+static int tune_pick(int which, uint64_t units)
 {
     tune_init();
-    if (const char* e = getenv("QZBSC_RADIX_BITS")) {
-        const int v = atoi(e);
-        if (v == 8 || v == 9 || v == 11) return v;
-    }
-    RadixTuner& t = g_tune[which];
+    const int f = tune_forced(which);
+    if (f >= 0) return f;
+    Knob& t = g_knob[which];
     std::lock_guard<std::mutex> lk(t.mu);
-    if (t.choice) return t.choice;
-    if (n < kTuneMin) return 8;
+    if (t.choice >= 0) return t.cand[t.choice];
+    if (units < t.min_units) return t.cand[0];
     return t.cand[t.trials % t.ncand];
 }
 
-// True while the tuner still times sorts of n elements (its picks rotate).
-static bool tune_wants(int which, uint64_t n)
+// True while the knob still times calls with `units` of work.
+static bool tune_wants(int which, uint64_t units)
 {
     tune_init();
-    if (getenv("QZBSC_RADIX_BITS")) return false;
-    RadixTuner& t = g_tune[which];
+    if (tune_forced(which) >= 0) return false;
+    Knob& t = g_knob[which];
     std::lock_guard<std::mutex> lk(t.mu);
-    return !t.choice && n >= kTuneMin;
+    return t.choice < 0 && units >= t.min_units;
 }
 
-static int width_index(int bits) { return bits == 9 ? 1 : bits == 11 ? 2 : 0; }
-
-// One timed sort of n elements with `bits`-bit digits; after one trial of
-// every candidate the fastest per element is kept for the process.
-static void tune_report(int which, int bits, uint64_t n, float ms)
+// One timed call with candidate `value` over `units` of work; after a trial
+// of every candidate the fastest per unit is kept for the process.
+static void tune_report(int which, int value, uint64_t units, float ms)
 {
-    if (n < kTuneMin || ms <= 0) return;
-    RadixTuner& t = g_tune[which];
+    Knob& t = g_knob[which];
     std::lock_guard<std::mutex> lk(t.mu);
-    if (t.choice) return;
-    const int k = width_index(bits);
+    if (t.choice >= 0 || units < t.min_units || ms <= 0) return;
+    int k = -1;
+    for (int c = 0; c < t.ncand; ++c)
+        if (t.cand[c] == value) k = c;
+    if (k < 0) return;
     t.ns[k] += ms * 1e6;
-    t.elems[k] += (double)n;
+    t.units[k] += (double)units;
     ++t.trials;
-    int best = 0;
+    int best = -1;
     double best_ns = 0;
     for (int c = 0; c < t.ncand; ++c) {
-        const int w = width_index(t.cand[c]);
-        if (t.elems[w] <= 0) return;  // a candidate is still untried
-        const double v = t.ns[w] / t.elems[w];
-        if (!best || v < best_ns) {
-            best = t.cand[c];
+        if (t.units[c] <= 0) return;  // a candidate is still untried
+        const double v = t.ns[c] / t.units[c];
+        if (best < 0 || v < best_ns) {
+            best = c;
             best_ns = v;
         }
     }
     t.choice = best;
 }
 
+int tune_knob(int which, int* choice, int* ncand, int* cand, double* ns_per_unit)
+{
+    tune_init();
+    Knob& t = g_knob[which];
+    std::lock_guard<std::mutex> lk(t.mu);
+    *choice = t.choice >= 0 ? t.cand[t.choice] : -1;
+    *ncand = t.ncand;
+    for (int c = 0; c < t.ncand; ++c) {
+        cand[c] = t.cand[c];
+        ns_per_unit[c] = t.units[c] > 0 ? t.ns[c] / t.units[c] : 0;
+    }
+    return t.trials;
+}
+
 int tune_state(int which, int* choice, double* ns_per_elem)
 {
-    RadixTuner& t = g_tune[which];
+    tune_init();
+    Knob& t = g_knob[which];
     std::lock_guard<std::mutex> lk(t.mu);
-    *choice = t.choice;
-    for (int k = 0; k < 3; ++k) ns_per_elem[k] = t.elems[k] > 0 ? t.ns[k] / t.elems[k] : 0;
+    *choice = t.choice >= 0 ? t.cand[t.choice] : 0;
+    for (int k = 0; k < 3; ++k) ns_per_elem[k] = 0;
+    for (int c = 0; c < t.ncand; ++c) {
+        const int w = t.cand[c] == 9 ? 1 : t.cand[c] == 11 ? 2 : 0;
+        ns_per_elem[w] = t.units[c] > 0 ? t.ns[c] / t.units[c] : 0;
+    }
     return t.trials;
 }
 
@@ -2354,6 +2404,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
         hipLaunchKernelGGL(k_pack_nib, dim3(qz_div_up(n, kThreads * 8u)), dim3(kThreads), 0, s, text, n,
                            (const PackParams*)ppd_, bufB_, 1u);
         QZ_HIP(hipGetLastError());
+        if (timed) QZ_HIP(hipEventRecord(tev_[4], s));
 #define QZ_FINAL0F(T, SVL)                                                                                      \
     hipLaunchKernelGGL((k_final0f<T, SVL>), dim3(ntiles), dim3(T), 0, s, keys, vals, (const TileDesc*)tiles0_, \
                        bufA_, sa_, rank_, glA, glB, (const PackParams*)ppd_, bmap_, (const uint32_t*)kpos,      \
@@ -2368,6 +2419,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
         }
 #undef QZ_FINAL0F
         QZ_HIP(hipGetLastError());
+        if (timed) QZ_HIP(hipEventRecord(tev_[5], s));
     }
     return 0;
 }
@@ -2387,12 +2439,13 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     Grp* const list = (Grp*)groups_[1];
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
-    const int bits0 = tune_pick(0, n);
-    const bool timed0 = tune_wants(0, n);  // the tuner is still timing: run eagerly with events
+    const int bits0 = tune_pick(kKnobRound0Bits, n);
+    const int fshape = tune_pick(kKnobFuseShape, n);
+    // a knob of round 0 is still timing: run eagerly with events
+    const bool timed0 = tune_wants(kKnobRound0Bits, n) || tune_wants(kKnobFuseShape, n);
     uint32_t ntiles = 0;
     const uint32_t hist_words = count_tiles(offs_host_.data(), nblk, &ntiles, 1u << bits0);
     // the fused first doubling round needs room for the 4-bit text in bufB
-    const int fshape = fuse_shape();
     const uint32_t fuse = fuse_enabled() ? 1u : 0u;
     auto prepare0 = [&](hipStream_t st) -> int {
         uint32_t nt = 0;
@@ -2426,6 +2479,12 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     bool large_timed = false;
     int large_bits = 8;
     uint64_t large_n = 0;
+    // k_sort_small's LDS budget: chosen at the call's first round with
+    // small-class work, timed over all of the call's rounds
+    int ss_lds = -1;
+    bool ss_timed = false, ss_round = false;
+    uint64_t ss_slots = 0;
+    double ss_ms = 0;
     uint32_t* hc = (uint32_t*)pinned_;
     // large-group lists of a round; their uploads are read before the next
     // round's synchronisation, so they live outside the loop
@@ -2461,12 +2520,21 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         if (timed0 && rounds_ == 1) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, tev_[0], tev_[1]) == hipSuccess) tune_report(0, bits0, n, ms);
+            if (hipEventElapsedTime(&ms, tev_[0], tev_[1]) == hipSuccess) tune_report(kKnobRound0Bits, bits0, n, ms);
+            // the fused finish works only on packed keys (else it returns at once)
+            if (fuse && !rle && hipEventElapsedTime(&ms, tev_[4], tev_[5]) == hipSuccess)
+                tune_report(kKnobFuseShape, fshape, n, ms);
         }
         if (large_timed) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, tev_[2], tev_[3]) == hipSuccess) tune_report(1, large_bits, large_n, ms);
+            if (hipEventElapsedTime(&ms, tev_[2], tev_[3]) == hipSuccess)
+                tune_report(kKnobLargeBits, large_bits, large_n, ms);
             large_timed = false;
+        }
+        if (ss_round) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, tev_[6], tev_[7]) == hipSuccess) ss_ms += ms;
+            ss_round = false;
         }
         uint64_t fused = 0;  // groups round 0 already took through the first doubling round (set B)
         if (rounds_ == 1)
@@ -2481,7 +2549,10 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             fprintf(stderr, "\n");
         }
         if (total == 0) {
-            if (!fused) break;
+            if (!fused) {
+                if (ss_timed) tune_report(kKnobSortSmallLds, ss_lds, ss_slots, (float)ss_ms);
+                break;
+            }
             // every group round 0 left was fused: the first doubling round
             // has nothing to sort, and set B holds the second round's groups
             ++rounds_;
@@ -2560,7 +2631,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                         mx, (unsigned long long)sz[0], (unsigned long long)sz[1], (unsigned long long)sz[2],
                         (unsigned long long)sz[3], (unsigned long long)sz[4]);
             }
-            lbits = std::min(tune_pick(1, M), kLargeRadixBits);
+            lbits = std::min(tune_pick(kKnobLargeBits, M), kLargeRadixBits);
             lhist = build_tiles(lb.data(), nlarge, tiles_host_.data(), &lt, 1u << lbits);
             // tiles at the groups' slots; the sort writes each group shifted
             // from its packed offset to its slot start
@@ -2606,9 +2677,22 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         // phase B: sort groups, update SA / ranks, emit subgroups
         if (small_waves) {
-            hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), occ_lds(kOccSortSmall), s, cl, plan,
+            const uint64_t slots = (uint64_t)small_waves * QZ_WAVE;
+            if (ss_lds < 0) {
+                ss_lds = tune_pick(kKnobSortSmallLds, slots);
+                ss_timed = tune_wants(kKnobSortSmallLds, slots);
+            }
+            if (ss_timed) {
+                QZ_HIP(hipEventRecord(tev_[6], s));
+                ss_slots += slots;
+            }
+            hipLaunchKernelGGL(k_sort_small, dim3(small_sort_blocks), dim3(kThreads), (uint32_t)ss_lds, s, cl, plan,
                                bufB_, sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
+            if (ss_timed) {
+                QZ_HIP(hipEventRecord(tev_[7], s));
+                ss_round = true;
+            }
         }
         if (nwave) {
             hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave, bufB_,

@@ -76,12 +76,17 @@ private:
     std::vector<uint32_t> offs_host_;  // block offsets of the current batch (host copy)
   private:
     void* pinned_ = nullptr;
-    hipEvent_t tev_[4] = {nullptr, nullptr, nullptr, nullptr};  // radix-sort timing for the tuner
+    hipEvent_t tev_[8] = {};       // tuner timing: round-0 sort, large-group sort, fused finish, k_sort_small
 };
 
 // Digit-width tuner state: which = 0 round 0, 1 large groups. Returns the
 // number of timed trials; choice is 0 while undecided.
 int tune_state(int which, int* choice, double* ns_per_elem /* [3]: 8, 9, 11 bits */);
+// Any knob of the launch-shape tuner (0 round-0 digit width, 1 large-group
+// digit width, 2 fused-finish tile shape, 3 k_sort_small LDS budget): the
+// chosen value (-1 while undecided), the candidates and the mean ns per unit
+// of work of each. Returns the number of timed trials.
+int tune_knob(int which, int* choice, int* ncand, int* cand /* [4] */, double* ns_per_unit /* [4] */);
 
 class IbwtEngine {
 public:

@@ -236,3 +236,30 @@ def test_forward_workspace_per_input_byte(gpu):
             u, p, _ = O.bwt_aux(blk[:4 << 20])
             o2, p2, _ = gpu.bwt.encode_batch([blk[:4 << 20]])
             assert np.array_equal(o2[0], u) and p2[0] == p
+
+
+def test_tuner_knobs_settle_and_never_change_results(gpu, monkeypatch):
+    """The launch-shape tuner: after a few calls of >= 4M suffixes the
+    round-0 digit width and the fused finish's tile shape are chosen among
+    their candidates, every knob reports only its candidates, and each
+    forced candidate (digit width, tile shape, small-sort LDS budget) gives
+    the bytes the oracle gives."""
+    from qz_amd import bwt
+    blocks = [synth.sequence_stream(3_000_000, 40_000, 150, 31 + k, 0.002, 0.0001)[:5_000_000] for k in range(2)]
+    for _ in range(4):
+        out, prim, aux = bwt.encode_batch(blocks)
+    t = bwt.tuning()
+    print(t, flush=True)
+    assert t["round0_digit_bits"]["choice"] in (8, 11)
+    assert t["fused_finish_shape"]["choice"] in ("512 threads, suffixes in LDS", "1024 threads, suffixes in LDS")
+    assert set(t["sort_small_lds_bytes"]["ns_per_unit"]) <= {"16384", "0"}
+    want = [O.bwt_aux(b) for b in blocks]
+    for i, (u, p, a) in enumerate(want):
+        assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a)
+    for env, val in (("QZBSC_FUSE_SHAPE", "1024s"), ("QZBSC_FUSE_SHAPE", "512s"), ("QZBSC_LDS_SORT_SMALL", "0"),
+                     ("QZBSC_LDS_SORT_SMALL", "16384"), ("QZBSC_RADIX_BITS", "11")):
+        monkeypatch.setenv(env, val)
+        o2, p2, a2 = bwt.encode_batch(blocks)
+        monkeypatch.delenv(env)
+        for i, (u, p, a) in enumerate(want):
+            assert np.array_equal(o2[i], u) and p2[i] == p and np.array_equal(a2[i], a), f"{env}={val} block {i}"
