@@ -488,10 +488,10 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for counter scaling)")
     ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
-    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-lane-mib", type=int, default=128, help="small-job lane: jobs of at most this many MiB")
-    ap.add_argument("--serve-workers", type=int, default=3)
-    ap.add_argument("--serve-unit-mib", type=int, default=64, help="serving work units (MiB of whole blocks)")
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-unit-mib", type=int, default=128, help="serving work units (MiB of whole blocks)")
     ap.add_argument("--serve-policy", type=int, default=1, help="0 oldest job first, 1 fewest bytes left first")
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
