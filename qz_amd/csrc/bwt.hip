@@ -48,6 +48,10 @@ constexpr int kClsLarge = 8;      // > 4096: segmented radix sort
 constexpr int kWaveMax = 1024;
 constexpr int kMedMax = 4096;
 constexpr uint32_t kPadKey = 0xffffffffu;
+// Device counters (group-list class counts, pool tops, bucket fills) each
+// own a 256-byte line: atomics on words of one line serialise, and every
+// workgroup that emits groups adds to several of them.
+constexpr int kCtrStride = 64;
 
 struct Grp {
     uint32_t start, size;
@@ -180,7 +184,8 @@ __device__ __forceinline__ void stage_flush_e(StageHdr& H, const uint2* E, const
 {
     __syncthreads();
     const uint32_t n = H.n < cap ? H.n : cap;
-    if (threadIdx.x < kClasses && H.cnt[threadIdx.x]) atomicAdd(&gl.cnt[threadIdx.x], H.cnt[threadIdx.x]);
+    if (threadIdx.x < kClasses && H.cnt[threadIdx.x])
+        atomicAdd(&gl.cnt[threadIdx.x * kCtrStride], H.cnt[threadIdx.x]);
     if (threadIdx.x == 0) {
         if (H.n > cap) *gl.overflow = 1u;
         H.base = n ? atomicAdd(gl.top, n) : 0u;
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, ui
         uint32_t acc = 0;
         for (int c = 0; c < kClasses; ++c) {
             co[c] = acc;
-            acc += cnt[c];
+            acc += cnt[c * kCtrStride];
         }
     }
     uint32_t run[kClasses];
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(256) void k_bucket(const Grp* __restrict__ pool, ui
         const uint32_t c = threadIdx.x;
         uint32_t tot = 0;
         for (uint32_t v = 0; v < W; ++v) tot += wb[v][c];
-        uint32_t base = tot ? atomicAdd(&fill[c], tot) + co[c] : 0u;
+        uint32_t base = tot ? atomicAdd(&fill[c * kCtrStride], tot) + co[c] : 0u;
         for (uint32_t v = 0; v < W; ++v) {
             const uint32_t x = wb[v][c];
             wb[v][c] = base;
@@ -915,8 +920,8 @@ __device__ __forceinline__ void stage2_flush(Stage2& S, const GroupLists& glA, c
     __syncthreads();
     const uint32_t t = threadIdx.x;
     if (t < (uint32_t)kClasses) {
-        if (S.cnt[0][t]) atomicAdd(&glA.cnt[t], S.cnt[0][t]);
-        if (S.cnt[1][t]) atomicAdd(&glB.cnt[t], S.cnt[1][t]);
+        if (S.cnt[0][t]) atomicAdd(&glA.cnt[t * kCtrStride], S.cnt[0][t]);
+        if (S.cnt[1][t]) atomicAdd(&glB.cnt[t * kCtrStride], S.cnt[1][t]);
     }
     if (t == 0) {
         if (S.n[0] + S.n[1] > (uint32_t)kStage) *glA.overflow = 1u;
@@ -1189,7 +1194,7 @@ __device__ __forceinline__ bool small_lane(const Grp* const* cur, const uint32_t
     *cls = c;
     *segw = seg;
     *e = lane & (seg - 1);
-    if (gi >= cnt[c]) return false;
+    if (gi >= cnt[c * kCtrStride]) return false;
     const Grp g = cur[c][gi];
     *start = g.start;
     *size = g.size;
@@ -1231,7 +1236,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_small(CurLists cl, SmallPla
         cls[t] = c;
         gi[t] = (gw - plan.wave_base[c]) * (QZ_WAVE / seg) + lane / seg;
         e[t] = lane & (seg - 1u);
-        cnt[t] = cl.cnt[c];
+        cnt[t] = cl.cnt[c * kCtrStride];
     }
     Grp g[TASKS];
 #pragma unroll
@@ -1293,7 +1298,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_small(CurLists cl, SmallPlan 
         const uint32_t seg = 2u << c;
         const uint32_t gi = (gwave - plan.wave_base[c]) * (QZ_WAVE / seg) + lane / seg;
         const uint32_t e = lane & (seg - 1u);
-        const uint32_t ncls = cl.cnt[c];
+        const uint32_t ncls = cl.cnt[c * kCtrStride];
         const Grp grp = cl.list[c][min(gi, ncls - 1u)];
         const uint32_t start = grp.start, size = grp.size;
         const bool valid = gi < ncls && e < size;
@@ -2182,9 +2187,13 @@ static int bitlen(uint64_t v)
     return b;
 }
 
-// Counter words in cnt_: set A classes [0, 9) and pool top 9, set B classes
-// [32, 41) and top 41, overflow flag 18, bucket fill counters [48, 57).
+// Counter slots in cnt_ (slot i at word i * kCtrStride): set A classes
+// [0, 9) and pool top 9, set B classes [32, 41) and top 41, overflow flag 18,
+// bucket fill counters [48, 57).
 constexpr int kCntB = 32, kTopOff = kClasses, kOverflow = 2 * kClasses, kFill = 48;
+constexpr int kCtrSlots = kFill + kClasses;
+constexpr size_t kCtrWords = (size_t)kCtrSlots * kCtrStride;
+constexpr uint32_t kPinnedPack = 4096;  // pinned_ word of the PackParams read back after round 0
 
 static uint64_t pool_cap(uint64_t n) { return n / 2 + 1; }
 // Copy runs of one direction of a class split: blocks cut at kCopyChunk.
@@ -2245,7 +2254,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&idx_, (size_t)cb * 256 * 4));
     QZ_HIP(hipMalloc(&hist_, 256 * 4));
     QZ_HIP(hipMalloc(&brun_, 2 * ((size_t)cb + 1) * 4));
-    QZ_HIP(hipMalloc(&cnt_, 2 * 32 * 4));
+    QZ_HIP(hipMalloc(&cnt_, kCtrWords * 4));
     for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
     QZ_HIP(hipMalloc(&lgrp_, (c / (kMedMax + 1) + 2) * sizeof(Grp)));
     QZ_HIP(hipMalloc(&ldelta_, (c / (kMedMax + 1) + 2) * 4));
@@ -2259,7 +2268,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&cls_, (size_t)cb * kClassWords * 4));
     QZ_HIP(hipMalloc(&runs_, 2 * copy_runs_cap(c, cb) * sizeof(CopyRun)));
     tiles_host_.resize(tile_cap);
-    QZ_HIP(hipHostMalloc(&pinned_, 4096));
+    QZ_HIP(hipHostMalloc(&pinned_, (kPinnedPack + 4) * 4));  // counter copy, then the packing words
     for (auto& ev : tev_)
         if (!ev) QZ_HIP(hipEventCreate(&ev));
     cap_ = c;
@@ -2300,7 +2309,7 @@ static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint
     GroupLists gl;
     gl.pool = (Grp*)pool;
     gl.cnt = cnt;
-    gl.top = cnt + kTopOff;
+    gl.top = cnt + kTopOff * kCtrStride;
     gl.cap = (uint32_t)pool_cap(cap);
     gl.overflow = overflow;
     gl.down = down ? 1u : 0u;
@@ -2344,7 +2353,7 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     const BlockMap bmap_{off_, btab_, nblk};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, hist_, 256u);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, brun_, 2 * nblk);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, 64u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kThreads), 0, s, cnt_, (uint32_t)kCtrWords);
     QZ_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_byte_hist, dim3(std::min<uint32_t>(qz_div_up(n, kThreads), 2048)), dim3(kThreads), 0, s,
                        text, n, hist_);
@@ -2390,8 +2399,8 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
-    const GroupLists glA = make_lists(groups_[0], cnt_, cnt_ + kOverflow, cap_, false);
-    const GroupLists glB = make_lists(groups_[0], cnt_ + kCntB, cnt_ + kOverflow, cap_, true);
+    const GroupLists glA = make_lists(groups_[0], cnt_, cnt_ + kOverflow * kCtrStride, cap_, false);
+    const GroupLists glB = make_lists(groups_[0], cnt_ + kCntB * kCtrStride, cnt_ + kOverflow * kCtrStride, cap_, true);
     // one of the two finishes the batch (the other returns at once): the
     // choice depends on the key mode and alphabet, decided on the device
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
@@ -2432,10 +2441,10 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     trace_ = getenv("QZBSC_TRACE") != nullptr;
 
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
-    uint32_t* overflow = cnt_ + kOverflow;
+    uint32_t* overflow = cnt_ + kOverflow * kCtrStride;
     // emission targets: the pool with counter set A or B, alternating
     const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_, false),
-                               make_lists(groups_[0], cnt_ + kCntB, overflow, cap_, true)};
+                               make_lists(groups_[0], cnt_ + kCntB * kCtrStride, overflow, cap_, true)};
     Grp* const list = (Grp*)groups_[1];
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
@@ -2496,25 +2505,28 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         // overflow word, which lies between the two sets in cnt_
         // (after round 0 both sets: the fused first round may have filled B)
         const int lo = cur ? kOverflow : 0, hi = (cur || rounds_ == 1) ? kCntB + kClasses : kOverflow + 1;
-        QZ_HIP(hipMemcpyAsync(hc, cnt_ + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToHost, s));
-        if (rounds_ == 1)
-            QZ_HIP(hipMemcpyAsync(hc + 64, &((const PackParams*)ppd_)->k0, 12, hipMemcpyDeviceToHost, s));  // k0, rle, cb
+        QZ_HIP(hipMemcpyAsync(hc, cnt_ + (size_t)lo * kCtrStride, ((size_t)(hi - 1 - lo) * kCtrStride + 1) * 4,
+                              hipMemcpyDeviceToHost, s));
+        if (rounds_ == 1)  // k0, rle, cb
+            QZ_HIP(hipMemcpyAsync(hc + kPinnedPack, &((const PackParams*)ppd_)->k0, 12, hipMemcpyDeviceToHost, s));
         QZ_HIP(hipStreamSynchronize(s));
+        // counter slot i of the copy
+        auto slot = [&](int i) { return ((volatile uint32_t*)hc)[(size_t)(i - lo) * kCtrStride]; };
         if (rounds_ == 1) {
-            h = ((volatile uint32_t*)hc)[64];
-            rle = ((volatile uint32_t*)hc)[65] != 0;
-            cb = ((volatile uint32_t*)hc)[66];
+            h = ((volatile uint32_t*)hc)[kPinnedPack];
+            rle = ((volatile uint32_t*)hc)[kPinnedPack + 1] != 0;
+            cb = ((volatile uint32_t*)hc)[kPinnedPack + 2];
             k0_ = h;
             cb_ = cb;
         }
-        const int cofs = (cur ? kCntB : 0) - lo;
+        const int cbase = cur ? kCntB : 0;
         uint32_t counts[kClasses];
         uint64_t total = 0;
         for (int k = 0; k < kClasses; ++k) {
-            counts[k] = ((volatile uint32_t*)hc)[cofs + k];
+            counts[k] = slot(cbase + k);
             total += counts[k];
         }
-        if (((volatile uint32_t*)hc)[kOverflow - lo]) {
+        if (slot(kOverflow)) {
             fprintf(stderr, "qzbsc: group list overflow\n");
             return QZ_GPU_ERROR;
         }
@@ -2538,7 +2550,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         }
         uint64_t fused = 0;  // groups round 0 already took through the first doubling round (set B)
         if (rounds_ == 1)
-            for (int k = 0; k < kClasses; ++k) fused += ((volatile uint32_t*)hc)[kCntB + k];
+            for (int k = 0; k < kClasses; ++k) fused += slot(kCntB + k);
         if (trace_) {
             if (rle)
                 fprintf(stderr, "qzbsc: round %d (per-group depths) groups", rounds_);
@@ -2583,14 +2595,14 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                 o += counts[k];
             }
             cl.cnt = C.cnt;
-            QZ_HIP(hipMemsetAsync(cnt_ + kFill, 0, kClasses * 4, s));
+            QZ_HIP(hipMemsetAsync(cnt_ + kFill * kCtrStride, 0, ((kClasses - 1) * kCtrStride + 1) * 4, s));
             hipLaunchKernelGGL(k_bucket, dim3(qz_div_up(o, kBucketRange)), dim3(256), 0, s, C.pool, o, C.down, C.cap,
-                               C.cnt, list, cnt_ + kFill);
+                               C.cnt, list, cnt_ + kFill * kCtrStride);
             QZ_HIP(hipGetLastError());
         }
         // this round emits into the pool again, with the other counter set
         // (after round 0 that set may already hold fused groups: append)
-        if (rounds_ != 1) QZ_HIP(hipMemsetAsync(N.cnt, 0, (kClasses + 1) * 4, s));
+        if (rounds_ != 1) QZ_HIP(hipMemsetAsync(N.cnt, 0, (kClasses * kCtrStride + 1) * 4, s));  // classes and top
 
         SmallPlan plan;
         plan.wave_base[0] = 0;
@@ -2656,13 +2668,13 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             QZ_HIP(hipGetLastError());
         }
         if (nwave) {
-            hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave,
-                               sa_, rank_, bmap_, cb, h, dep_in, bufB_);
+            hipLaunchKernelGGL(k_gather_wave, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave],
+                               C.cnt + kClsWave * kCtrStride, sa_, rank_, bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
-            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, sa_,
-                               rank_, bmap_, cb, h, dep_in, bufB_);
+            hipLaunchKernelGGL(k_gather_med, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed],
+                               C.cnt + kClsMed * kCtrStride, sa_, rank_, bmap_, cb, h, dep_in, bufB_);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
@@ -2695,13 +2707,13 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
             }
         }
         if (nwave) {
-            hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave], C.cnt + kClsWave, bufB_,
-                               sa_, rank_, N, dep_in, dep_out, bmap_, cb);
+            hipLaunchKernelGGL(k_sort_lds<1>, dim3(nwave), dim3(QZ_WAVE), 0, s, cl.list[kClsWave],
+                               C.cnt + kClsWave * kCtrStride, bufB_, sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
         }
         if (nmed) {
-            hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed], C.cnt + kClsMed, bufB_,
-                               sa_, rank_, N, dep_in, dep_out, bmap_, cb);
+            hipLaunchKernelGGL(k_sort_lds<4>, dim3(nmed), dim3(kThreads), 0, s, cl.list[kClsMed],
+                               C.cnt + kClsMed * kCtrStride, bufB_, sa_, rank_, N, dep_in, dep_out, bmap_, cb);
             QZ_HIP(hipGetLastError());
         }
         if (M) {
