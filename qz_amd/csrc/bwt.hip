@@ -1915,19 +1915,29 @@ __global__ __launch_bounds__(kThreads) void k_block_class(const uint8_t* __restr
     uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t cnt = 0, seen = 0;
     if (live) {
-        const uint32_t end = bm.off[b + 1];
-        uint32_t prev = text[i0];
+        // one 16-byte load: the aligned window that holds position i0 (it
+        // holds a byte of the text, so it lies in the text's pages), the
+        // positions of the window inside i0's block and the batch
+        const uint32_t bs = bm.off[b], be = bm.off[b + 1];
+        const uint32_t mis = (uint32_t)((uintptr_t)(text + i0) & 15u);
+        const uint4 v = *reinterpret_cast<const uint4*>(text + i0 - mis);
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+        const int64_t p0 = (int64_t)i0 - mis;
+        uint32_t prev = 0;
+        bool have = false;
+#pragma unroll
         for (uint32_t q = 0; q < 16; ++q) {
-            const uint32_t i = i0 + q;
-            if (i >= end) break;  // the sample stays inside its block
-            const uint32_t c = text[i];
+            const int64_t p = p0 + q;
+            if (p < (int64_t)bs || p >= (int64_t)be) continue;
+            const uint32_t c = (w4[q >> 2] >> (8u * (q & 3u))) & 255u;
 #pragma unroll
             for (uint32_t w = 0; w < 8; ++w) m[w] |= (c >> 5) == w ? 1u << (c & 31u) : 0u;
-            if (q) {
+            if (have) {
                 cnt += c != prev ? 1u : 0u;
                 ++seen;
             }
             prev = c;
+            have = true;
         }
     }
     const uint32_t b0 = __shfl(b, 0, QZ_WAVE);
