@@ -11,6 +11,9 @@ class BwtEngine {
 public:
     ~BwtEngine() { release(); }
     int reserve(uint64_t n, uint32_t nblk);
+    // The depth arrays of run-length batches, allocated now rather than on
+    // the first such batch (for callers that must not allocate mid-run).
+    int reserve_depths() { return ensure_depths(); }
     void release();
     // off_[0..nblk] must already hold the batch offsets; text/out are device
     // buffers of n bytes (text_/out_ serve as staging for host callers).

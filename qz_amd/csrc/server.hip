@@ -189,6 +189,7 @@ private:
         {
             int rc = g.ok() ? 0 : QZ_GPU_ERROR;
             if (rc == 0) rc = g.slot().fwd.reserve(cap, cap_blk);
+            if (rc == 0) rc = g.slot().fwd.reserve_depths();  // no allocation once serving
             if (rc == 0) rc = g.slot().inv.reserve(cap, cap_blk);
             std::lock_guard<std::mutex> lk(mu_);
             if (rc) init_rc_ = rc;
