@@ -454,7 +454,13 @@ def serving_section(bwt, serve, synth, rates, jobs_per_rate: int, seed: int, wor
                                  "p99_vs_median_single_job": round(float(np.percentile(lat, 99)) / res["single_job_ms"], 2),
                                  "slowdown_p50": round(float(np.percentile(slow, 50)), 2),
                                  "slowdown_p99": round(float(np.percentile(slow, 99)), 2),
-                                 "mean_jobs_per_batch": round(float(np.mean(nbat)), 2)})
+                                 "mean_jobs_per_batch": round(float(np.mean(nbat)), 2),
+                                 # the three slowest jobs: what sets the tail
+                                 "slowest": [{"mb": round(int(templates[picks[i]]["sizes"].sum()) / 1e6, 1),
+                                              "kind": "compress" if kinds[i] == 0 else "decompress",
+                                              "ms": round(lat[i], 1), "alone_ms": round(alone[i], 1),
+                                              "queue_ms": round(sub[i].queue_ms, 1)}
+                                             for i in np.argsort(lat)[::-1][:3]]})
             log(f"serving rate {rate}: {res['rates'][-1]}")
         res["peak_inflight_bytes"] = srv.peak_bytes()
     for t in templates:
@@ -492,7 +498,7 @@ def main():
     ap.add_argument("--serve-lane-mib", type=int, default=128, help="small-job lane: jobs of at most this many MiB")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-unit-mib", type=int, default=128, help="serving work units (MiB of whole blocks)")
-    ap.add_argument("--serve-policy", type=int, default=1, help="0 oldest job first, 1 fewest bytes left first")
+    ap.add_argument("--serve-policy", type=int, default=0, help="0 oldest job first, 1 fewest bytes left first")
     ap.add_argument("--no-archive", action="store_true", help="skip the .qz archive section")
     ap.add_argument("--no-streams", action="store_true", help="skip the quality / header stream section")
     ap.add_argument("--archive-reads", type=int, default=1_000_000)
