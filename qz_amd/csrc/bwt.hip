@@ -2738,9 +2738,21 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
                 else
                     lg_split.push_back(lg[i]);
             }
-            if (trace_)
+            if (trace_) {
                 fprintf(stderr, "qzbsc:   large groups unsplit %u of %u (%llu of %u suffixes)\n",
                         nlarge - (uint32_t)lg_split.size(), nlarge, (unsigned long long)kept_elems, M);
+                // suffixes of split groups by the bits their key range spans
+                uint64_t byb[4] = {0, 0, 0, 0};  // <= 9, <= 18, <= 27, more
+                for (uint32_t i = 0; i < nlarge; ++i) {
+                    const uint32_t span = lmm_host_[2 * i + 1] - lmm_host_[2 * i];
+                    if (!span) continue;
+                    const int b = bitlen(span);
+                    byb[b <= 9 ? 0 : b <= 18 ? 1 : b <= 27 ? 2 : 3] += lg[i].size;
+                }
+                fprintf(stderr, "qzbsc:   split by key span <=9 bits %llu <=18 %llu <=27 %llu more %llu\n",
+                        (unsigned long long)byb[0], (unsigned long long)byb[1], (unsigned long long)byb[2],
+                        (unsigned long long)byb[3]);
+            }
             if (lg_split.size() < nlarge) {
                 hipLaunchKernelGGL(k_keep_large, dim3(qz_div_up(nlarge, kThreads)), dim3(kThreads), 0, s,
                                    (const Grp*)lgrp_, nlarge, lmm_, N, bmap_, dep_in, dep_out);
