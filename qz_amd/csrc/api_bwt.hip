@@ -45,7 +45,7 @@ int fwd_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::v
 {
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
     hipStream_t s = sl.stream;
-    QZ_TRY(sl.fwd.reserve(n, nblk));
+    QZ_TRY(sl.fwd.reserve(n, nblk, BwtEngine::max_block(offs)));
     QZ_HIP(hipEventRecord(sl.ev0, s));
     std::vector<int32_t> idx;
     QZ_TRY(sl.fwd.encode_batch(d_in, d_out, offs, s, idx));
@@ -98,7 +98,7 @@ QZBSC_API int qzbsc_bwt_encode_batch(const unsigned char* in, unsigned char* out
     if (!g.ok()) return QZ_GPU_ERROR;
     Slot& sl = g.slot();
     const uint32_t n = offs.back();
-    QZ_TRY(sl.fwd.reserve(n, (uint32_t)nblocks));
+    QZ_TRY(sl.fwd.reserve(n, (uint32_t)nblocks, BwtEngine::max_block(offs)));
     QZ_HIP(hipMemcpyAsync(sl.fwd.text_, in, n, hipMemcpyHostToDevice, sl.stream));
     QZ_TRY(fwd_batch_device(sl, sl.fwd.text_, sl.fwd.out_, offs, primary, aux, nullptr));
     QZ_HIP(hipMemcpyAsync(out, sl.fwd.out_, n, hipMemcpyDeviceToHost, sl.stream));

@@ -770,7 +770,7 @@ QZBSC_API int qzbsc_compress_batch(const unsigned char* input, const int* sizes,
         const int k = (int)list.size();
         std::vector<uint32_t> offs(k + 1, 0);
         for (int i = 0; i < k; ++i) offs[i + 1] = offs[i] + (uint32_t)dn[list[i]];
-        QZ_TRY(sl.fwd.reserve(offs[k], (uint32_t)k));
+        QZ_TRY(sl.fwd.reserve(offs[k], (uint32_t)k, BwtEngine::max_block(offs)));
         const auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < k; ++i) {
             const int b = list[i];

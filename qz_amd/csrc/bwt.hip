@@ -2209,23 +2209,47 @@ static uint64_t pool_cap(uint64_t n) { return n / 2 + 1; }
 // Copy runs of one direction of a class split: blocks cut at kCopyChunk.
 static uint64_t copy_runs_cap(uint64_t n, uint32_t nblk) { return n / kCopyChunk + nblk + 1; }
 
-uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
+// The sort workspace covers at most kChunkDefault bytes of whole blocks (or
+// the largest block, if larger); a bigger batch is sorted chunk by chunk in
+// it, and only the text and output staging scale with the batch. Chunks of
+// 512 MiB cost 0-3% of forward time against one batch (1.5 GB of reads,
+// binned qualities, a mixed batch), chunks of 256 MiB 2-8%.
+// QZBSC_CHUNK_MIB sets the chunk (0: none).
+constexpr uint64_t kChunkDefault = 512ull << 20;
+static uint64_t chunk_bytes()
+{
+    const char* e = getenv("QZBSC_CHUNK_MIB");
+    return e ? (uint64_t)std::max(0L, atol(e)) << 20 : kChunkDefault;
+}
+uint64_t BwtEngine::chunk() { return chunk_bytes(); }
+
+// Workspace elements for a batch of n bytes whose largest block has maxblk.
+static uint64_t work_cap(uint64_t n, uint64_t maxblk)
+{
+    const uint64_t c = chunk_bytes();
+    return c == 0 ? n : std::min<uint64_t>(n, std::max<uint64_t>(c, maxblk));
+}
+
+uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk, uint64_t maxblk)
 {
     const uint64_t c = std::max<uint64_t>(n, 1024);
     const uint32_t cb = std::max<uint32_t>(nblk, 1);
-    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
+    const uint64_t w = std::max<uint64_t>(work_cap(c, maxblk), 1024);
+    const size_t seg_words = ((size_t)max_tiles(w, cb) + w / kSortTile + 2) * (1u << kMaxRadixBits);
     const size_t tmpw = scan_tmp_words(seg_words) + 1024;
-    const uint64_t groups = pool_cap(c);
-    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;
-    const uint64_t nlg = c / (kMedMax + 1) + 2;  // large groups of a round, at most
-    // text, out; sa, rank, keys0, vals0 (4 B each); dep0/1 (4 B each, allocated for run-length batches
-    // only, counted always); bufB (doubling keys or round-0 histograms); bufA per tile; large-group
-    // histograms (9-bit digits at most), their list, offsets and shifts
-    return 2 * (c + 64) + 4 * 4 * c + 2 * 4 * c + std::max<uint64_t>(c, seg_words) * 4 + tile_cap * 4 + tmpw * 4 +
-           3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + 1024 + 256 + 2 * groups * sizeof(Grp) +
-           nlg * (sizeof(Grp) + 4 + 8) + tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc) +
-           sizeof(PackParams) + ((c >> kTabShift) + 2) * 4 + (uint64_t)cb * kClassWords * 4 +
-           2 * copy_runs_cap(c, cb) * sizeof(CopyRun);
+    const uint64_t groups = pool_cap(w);
+    const size_t tile_cap = (size_t)max_tiles(w, cb) + w / kSortTile + 2;
+    const uint64_t nlg = w / (kMedMax + 1) + 2;  // large groups of a round, at most
+    // staging by the batch: text, out, block table, copy runs, per-block words;
+    // workspace by the chunk: sa, rank, keys0, vals0 (4 B each); dep0/1 (4 B
+    // each, allocated for run-length batches only, counted always); bufB
+    // (doubling keys or round-0 histograms); bufA per tile; large-group
+    // histograms (9-bit digits at most), their list, offsets, shifts, key ranges
+    return 2 * (c + 64) + ((c >> kTabShift) + 2) * 4 + 2 * copy_runs_cap(c, cb) * sizeof(CopyRun) +
+           3 * (cb + 1) * 4 + (uint64_t)cb * 1024 + (uint64_t)cb * kClassWords * 4 + 1024 + 256 +
+           sizeof(PackParams) + 4 * 4 * w + 2 * 4 * w + std::max<uint64_t>(w, seg_words) * 4 + tile_cap * 4 +
+           tmpw * 4 + 2 * groups * sizeof(Grp) + nlg * (sizeof(Grp) + 4 + 8) +
+           tile_cap * (1u << kLargeRadixBits) * 4 + 2 * tile_cap * sizeof(TileDesc);
 }
 
 // Depth arrays, allocated for the engine's first batch with run-length keys
@@ -2233,31 +2257,33 @@ uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk)
 int BwtEngine::ensure_depths()
 {
     if (dep_[0]) return 0;
-    QZ_HIP(hipMalloc(&dep_[0], cap_ * 4));
-    QZ_HIP(hipMalloc(&dep_[1], cap_ * 4));
+    QZ_HIP(hipMalloc(&dep_[0], wcap_ * 4));
+    QZ_HIP(hipMalloc(&dep_[1], wcap_ * 4));
     return 0;
 }
 
-int BwtEngine::reserve(uint64_t n, uint32_t nblk)
+int BwtEngine::reserve(uint64_t n, uint32_t nblk, uint64_t maxblk)
 {
-    if (n <= cap_ && nblk <= cap_blk_) return 0;
+    const uint64_t wn = work_cap(std::max<uint64_t>(n, 1024), std::min<uint64_t>(maxblk, n));
+    if (n <= cap_ && nblk <= cap_blk_ && wn <= wcap_) return 0;
     const MemTrace mt("forward", n, nblk);
     // grow geometrically so a stream of slowly growing batches reallocates
     // only a few times
     const uint64_t c = std::max<uint64_t>(std::max<uint64_t>(n, 1024), cap_ ? cap_ + cap_ / 2 : 0);
     const uint32_t cb = std::max<uint32_t>(std::max<uint32_t>(nblk, 1), cap_blk_ ? cap_blk_ + cap_blk_ / 2 : 0);
+    const uint64_t w = std::min<uint64_t>(c, std::max<uint64_t>(wn, wcap_ ? wcap_ + wcap_ / 2 : 0));
     release();
-    const size_t seg_words = ((size_t)max_tiles(c, cb) + c / kSortTile + 2) * (1u << kMaxRadixBits);
+    const size_t seg_words = ((size_t)max_tiles(w, cb) + w / kSortTile + 2) * (1u << kMaxRadixBits);
     const size_t tmpw = scan_tmp_words(seg_words) + 1024;  // scan scratch; the histograms live in bufB
-    const size_t tile_cap = (size_t)max_tiles(c, cb) + c / kSortTile + 2;  // round 0 or large groups
+    const size_t tile_cap = (size_t)max_tiles(w, cb) + w / kSortTile + 2;  // round 0 or large groups
     QZ_HIP(hipMalloc(&text_, c + 64));
     QZ_HIP(hipMalloc(&out_, c + 64));
-    QZ_HIP(hipMalloc(&sa_, c * 4));
-    QZ_HIP(hipMalloc(&rank_, c * 4));
+    QZ_HIP(hipMalloc(&sa_, w * 4));
+    QZ_HIP(hipMalloc(&rank_, w * 4));
     QZ_HIP(hipMalloc(&bufA_, tile_cap * 4));  // last head per radix tile
-    QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(c, seg_words) * 4));
-    QZ_HIP(hipMalloc(&keys0_, c * 4));
-    QZ_HIP(hipMalloc(&vals0_, c * 4));
+    QZ_HIP(hipMalloc(&bufB_, std::max<size_t>(w, seg_words) * 4));
+    QZ_HIP(hipMalloc(&keys0_, w * 4));
+    QZ_HIP(hipMalloc(&vals0_, w * 4));
     QZ_HIP(hipMalloc(&lhist_, tile_cap * (1u << kLargeRadixBits) * 4));
     QZ_HIP(hipMalloc(&tmp_, tmpw * 4));
     QZ_HIP(hipMalloc(&off_, (cb + 1) * 4));
@@ -2265,11 +2291,11 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
     QZ_HIP(hipMalloc(&hist_, 256 * 4));
     QZ_HIP(hipMalloc(&brun_, 2 * ((size_t)cb + 1) * 4));
     QZ_HIP(hipMalloc(&cnt_, kCtrWords * 4));
-    for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(c) * sizeof(Grp)));  // pool, list
-    QZ_HIP(hipMalloc(&lgrp_, (c / (kMedMax + 1) + 2) * sizeof(Grp)));
-    QZ_HIP(hipMalloc(&ldelta_, (c / (kMedMax + 1) + 2) * 4));
-    QZ_HIP(hipMalloc(&lmm_, (c / (kMedMax + 1) + 2) * 8));
-    QZ_HIP(hipHostMalloc((void**)&lmm_host_, (c / (kMedMax + 1) + 2) * 8));
+    for (int set = 0; set < 2; ++set) QZ_HIP(hipMalloc(&groups_[set], pool_cap(w) * sizeof(Grp)));  // pool, list
+    QZ_HIP(hipMalloc(&lgrp_, (w / (kMedMax + 1) + 2) * sizeof(Grp)));
+    QZ_HIP(hipMalloc(&ldelta_, (w / (kMedMax + 1) + 2) * 4));
+    QZ_HIP(hipMalloc(&lmm_, (w / (kMedMax + 1) + 2) * 8));
+    QZ_HIP(hipHostMalloc((void**)&lmm_host_, (w / (kMedMax + 1) + 2) * 8));
     if (!lev_) QZ_HIP(hipEventCreateWithFlags(&lev_, hipEventDisableTiming));
     QZ_HIP(hipMalloc(&tiles_, tile_cap * sizeof(TileDesc)));
     QZ_HIP(hipMalloc(&tiles0_, tile_cap * sizeof(TileDesc)));
@@ -2283,6 +2309,7 @@ int BwtEngine::reserve(uint64_t n, uint32_t nblk)
         if (!ev) QZ_HIP(hipEventCreate(&ev));
     cap_ = c;
     cap_blk_ = cb;
+    wcap_ = w;
     return 0;
 }
 
@@ -2312,6 +2339,7 @@ void BwtEngine::release()
     pinned_ = nullptr;
     cap_ = 0;
     cap_blk_ = 0;
+    wcap_ = 0;
 }
 
 static GroupLists make_lists(void* pool, uint32_t* cnt, uint32_t* overflow, uint64_t cap, bool down)
@@ -2409,8 +2437,8 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     hipLaunchKernelGGL(k_tile_lasthead, dim3(ntiles), dim3(kThreads), 0, s, keys, (const TileDesc*)tiles0_, bufA_);
     QZ_HIP(hipGetLastError());
     QZ_TRY((device_scan<OpMax, false>(bufA_, bufA_, ntiles, tmp_, s)));
-    const GroupLists glA = make_lists(groups_[0], cnt_, cnt_ + kOverflow * kCtrStride, cap_, false);
-    const GroupLists glB = make_lists(groups_[0], cnt_ + kCntB * kCtrStride, cnt_ + kOverflow * kCtrStride, cap_, true);
+    const GroupLists glA = make_lists(groups_[0], cnt_, cnt_ + kOverflow * kCtrStride, wcap_, false);
+    const GroupLists glB = make_lists(groups_[0], cnt_ + kCntB * kCtrStride, cnt_ + kOverflow * kCtrStride, wcap_, true);
     // one of the two finishes the batch (the other returns at once): the
     // choice depends on the key mode and alphabet, decided on the device
     hipLaunchKernelGGL(k_final0, dim3(ntiles), dim3(kThreads), occ_lds(kOccFinal0), s, keys, vals,
@@ -2453,8 +2481,8 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
     if (offs_host_.size() != (size_t)nblk + 1) return QZ_BAD_PARAMETER;
     uint32_t* overflow = cnt_ + kOverflow * kCtrStride;
     // emission targets: the pool with counter set A or B, alternating
-    const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, cap_, false),
-                               make_lists(groups_[0], cnt_ + kCntB * kCtrStride, overflow, cap_, true)};
+    const GroupLists gls[2] = {make_lists(groups_[0], cnt_, overflow, wcap_, false),
+                               make_lists(groups_[0], cnt_ + kCntB * kCtrStride, overflow, wcap_, true)};
     Grp* const list = (Grp*)groups_[1];
 
     // ---- round 0: block-segmented radix sort on the packed prefix --------
@@ -2480,7 +2508,7 @@ int BwtEngine::sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hip
         QZ_TRY(prepare0(s));
         QZ_TRY(launch0(s));
     } else {
-        std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_,
+        std::vector<uint64_t> key = {n, nblk, (uint64_t)(uintptr_t)text, (uint64_t)bits0, cap_, cap_blk_, wcap_,
                                      (uint64_t)(rle_force() + 1), (uint64_t)fuse, (uint64_t)fshape};
         key.insert(key.end(), offs_host_.begin(), offs_host_.end());
         QZ_TRY(g0_.run(key, s, prepare0, launch0));
@@ -2808,6 +2836,7 @@ int BwtEngine::encode_device(const uint8_t* text, uint8_t* out, uint32_t n, uint
 {
     const BlockMap bmap_{off_, btab_, nblk};
     if (offs_host_.size() != (size_t)nblk + 1 || offs_host_.back() != n) return QZ_BAD_PARAMETER;
+    if (n > wcap_ || nblk > cap_blk_) return QZ_BAD_PARAMETER;  // the sort workspace holds one chunk
     build_block_tab(offs_host_, n, btab_host_);
     QZ_HIP(hipMemcpyAsync(btab_, btab_host_.data(), btab_host_.size() * 4, hipMemcpyHostToDevice, s));
     QZ_TRY(sort_suffixes(text, n, nblk, s));
@@ -2921,21 +2950,64 @@ int BwtEngine::encode_batch(const uint8_t* text, uint8_t* out, const std::vector
 {
     const uint32_t nblk = (uint32_t)offs.size() - 1, n = offs.back();
     if (n > cap_ || nblk > cap_blk_) return QZ_BAD_PARAMETER;
+    for (uint32_t b = 0; b < nblk; ++b)
+        if (offs[b + 1] - offs[b] > wcap_) return QZ_BAD_PARAMETER;  // reserve() sizes the workspace for it
     idx.assign((size_t)nblk * 256, 0);
-    // staging of a split: T holds the blocks in class order, O their BWTs;
-    // T is written while text is read, O is read while out is written
+    const bool trace = getenv("QZBSC_TRACE") != nullptr;
+    // One sub-batch: blocks (batch indexes) laid end to end from position
+    // base of src, BWTs to the same positions of dst.
+    std::vector<int32_t> part;
+    auto run_part = [&](const uint8_t* src, uint8_t* dst, const uint32_t* blocks, size_t nb, uint32_t base) -> int {
+        std::vector<uint32_t>& lo = offs_host_;
+        lo.assign(nb + 1, 0);
+        for (size_t i = 0; i < nb; ++i) lo[i + 1] = lo[i] + (offs[blocks[i] + 1] - offs[blocks[i]]);
+        QZ_HIP(hipMemcpyAsync(off_, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, s));
+        QZ_TRY(encode_device(src + base, dst + base, lo.back(), (uint32_t)nb, s));
+        part.resize(nb * 256);
+        QZ_HIP(hipMemcpyAsync(part.data(), idx_, part.size() * 4, hipMemcpyDeviceToHost, s));
+        QZ_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < nb; ++i)
+            std::copy(part.begin() + i * 256, part.begin() + (i + 1) * 256, idx.begin() + (size_t)blocks[i] * 256);
+        if (trace)
+            fprintf(stderr, "qzbsc: sub-batch of %zu blocks, %u bytes, k0 %u, char mode %u\n", nb, lo.back(), k0_,
+                    cb_);
+        return 0;
+    };
+    // Runs of consecutive entries of `blocks` holding at most wcap_ bytes,
+    // laid end to end from position base.
+    auto run_chunks = [&](const uint8_t* src, uint8_t* dst, const std::vector<uint32_t>& blocks, uint32_t base) -> int {
+        for (size_t i = 0; i < blocks.size();) {
+            size_t j = i;
+            uint64_t bytes = 0;
+            while (j < blocks.size() && (j == i || bytes + (offs[blocks[j] + 1] - offs[blocks[j]]) <= wcap_)) {
+                bytes += offs[blocks[j] + 1] - offs[blocks[j]];
+                ++j;
+            }
+            QZ_TRY(run_part(src, dst, blocks.data() + i, j - i, base));
+            base += (uint32_t)bytes;
+            i = j;
+        }
+        return 0;
+    };
+    // staging of a class split: T holds the blocks in class order, O their
+    // BWTs; T is written while text is read, O is read while out is written
     uint8_t* T = (text == text_) ? out_ : text_;
     uint8_t* O = (T == text_) ? out_ : text_;
     std::vector<std::vector<uint32_t>> classes;
     if (classes_enabled() && nblk >= 2 && n >= kSplitMinBytes && T != text && O != out)
         QZ_TRY(classify(text, offs, s, classes));
     if (classes.empty()) {
-        offs_host_ = offs;
-        QZ_HIP(hipMemcpyAsync(off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
-        QZ_TRY(encode_device(text, out, n, nblk, s));
-        QZ_HIP(hipMemcpyAsync(idx.data(), idx_, idx.size() * 4, hipMemcpyDeviceToHost, s));
-        QZ_HIP(hipStreamSynchronize(s));
-        return 0;
+        if (n <= wcap_) {  // one sub-batch, in place
+            offs_host_ = offs;
+            QZ_HIP(hipMemcpyAsync(off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+            QZ_TRY(encode_device(text, out, n, nblk, s));
+            QZ_HIP(hipMemcpyAsync(idx.data(), idx_, idx.size() * 4, hipMemcpyDeviceToHost, s));
+            QZ_HIP(hipStreamSynchronize(s));
+            return 0;
+        }
+        std::vector<uint32_t> all(nblk);
+        for (uint32_t b = 0; b < nblk; ++b) all[b] = b;
+        return run_chunks(text, out, all, 0);
     }
     // each block's place in T, and the copy runs into T and back out
     std::vector<uint32_t> at(nblk);
@@ -2960,23 +3032,7 @@ int BwtEngine::encode_batch(const uint8_t* text, uint8_t* out, const std::vector
     QZ_HIP(hipMemcpyAsync(runs_, runs.data(), runs.size() * sizeof(CopyRun), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_copy_runs, dim3(nrun), dim3(kThreads), 0, s, (const CopyRun*)runs_, text, T);
     QZ_HIP(hipGetLastError());
-    std::vector<int32_t> part;
-    for (const auto& c : classes) {
-        std::vector<uint32_t> lo(c.size() + 1, 0);
-        for (size_t i = 0; i < c.size(); ++i) lo[i + 1] = lo[i] + (offs[c[i] + 1] - offs[c[i]]);
-        offs_host_ = lo;
-        QZ_HIP(hipMemcpyAsync(off_, offs_host_.data(), offs_host_.size() * 4, hipMemcpyHostToDevice, s));
-        const uint32_t base = at[c[0]];
-        QZ_TRY(encode_device(T + base, O + base, lo.back(), (uint32_t)c.size(), s));
-        part.resize(c.size() * 256);
-        QZ_HIP(hipMemcpyAsync(part.data(), idx_, part.size() * 4, hipMemcpyDeviceToHost, s));
-        QZ_HIP(hipStreamSynchronize(s));
-        for (size_t i = 0; i < c.size(); ++i)
-            std::copy(part.begin() + i * 256, part.begin() + (i + 1) * 256, idx.begin() + (size_t)c[i] * 256);
-        if (getenv("QZBSC_TRACE"))
-            fprintf(stderr, "qzbsc: class of %zu blocks, %u bytes, k0 %u, char mode %u\n", c.size(), lo.back(), k0_,
-                    cb_);
-    }
+    for (const auto& c : classes) QZ_TRY(run_chunks(T, O, c, at[c[0]]));
     hipLaunchKernelGGL(k_copy_runs, dim3(nrun), dim3(kThreads), 0, s, (const CopyRun*)runs_ + nrun, O, out);
     QZ_HIP(hipGetLastError());
     return 0;

@@ -10,7 +10,19 @@ namespace qz {
 class BwtEngine {
 public:
     ~BwtEngine() { release(); }
-    int reserve(uint64_t n, uint32_t nblk);
+    // Staging for batches of n bytes in nblk blocks, and a sort workspace
+    // for one chunk of them (whole blocks, at most 512 MiB or the largest
+    // block, maxblk: by default as large as the batch).
+    int reserve(uint64_t n, uint32_t nblk, uint64_t maxblk = ~0ull);
+    // The chunk the sort workspace is sized for (bytes; 0: the whole batch).
+    static uint64_t chunk();
+    // Largest block of a batch.
+    static uint64_t max_block(const std::vector<uint32_t>& offs)
+    {
+        uint64_t m = 0;
+        for (size_t b = 0; b + 1 < offs.size(); ++b) m = m > offs[b + 1] - offs[b] ? m : offs[b + 1] - offs[b];
+        return m;
+    }
     // The depth arrays of run-length batches, allocated now rather than on
     // the first such batch (for callers that must not allocate mid-run).
     int reserve_depths() { return ensure_depths(); }
@@ -26,8 +38,9 @@ public:
     // the stream's work enqueued up to the last copy into out.
     int encode_batch(const uint8_t* text, uint8_t* out, const std::vector<uint32_t>& offs, hipStream_t s,
                      std::vector<int32_t>& idx);
-    // Device bytes reserve(n, nblk) allocates (admission control).
-    static uint64_t footprint(uint64_t n, uint32_t nblk);
+    // Device bytes reserve(n, nblk, maxblk) allocates (admission control);
+    // maxblk 0: blocks no larger than a chunk.
+    static uint64_t footprint(uint64_t n, uint32_t nblk, uint64_t maxblk = 0);
 
     uint8_t* text_ = nullptr;    // input symbols
     uint8_t* out_ = nullptr;     // BWT output
@@ -39,8 +52,9 @@ public:
     bool trace_ = false;
     uint32_t k0_ = 0;
     uint32_t cb_ = 0;            // char mode of the last batch (SA entries carry the preceding symbol)
-    uint64_t cap_ = 0;
+    uint64_t cap_ = 0;       // staging bytes (text_, out_) and blocks
     uint32_t cap_blk_ = 0;
+    uint64_t wcap_ = 0;      // sort workspace: suffixes of one chunk
 
 private:
     int sort_suffixes(const uint8_t* text, uint32_t n, uint32_t nblk, hipStream_t s);

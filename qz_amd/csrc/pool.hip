@@ -130,7 +130,7 @@ uint64_t pool_trim()
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto& s : g_slots) {
         if (s->busy) continue;
-        freed += s->fwd.cap_ ? BwtEngine::footprint(s->fwd.cap_, s->fwd.cap_blk_) : 0;
+        freed += s->fwd.cap_ ? BwtEngine::footprint(s->fwd.cap_, s->fwd.cap_blk_, s->fwd.wcap_) : 0;
         freed += s->inv.cap_ ? IbwtEngine::footprint(s->inv.cap_, s->inv.cap_blk_) : 0;
         s->fwd.release();
         s->inv.release();

@@ -97,6 +97,10 @@ public:
     Server(uint64_t budget, uint64_t max_batch, int workers, uint64_t lane_max, uint64_t unit_max, int policy)
         : budget_(budget), max_batch_(max_batch), unit_max_(unit_max), policy_(policy)
     {
+        // the forward workspace holds one chunk of whole blocks: blocks up
+        // to min(batch cap, chunk) are served
+        const uint64_t ch = BwtEngine::chunk();
+        max_block_ = ch ? std::min(max_batch_, ch) : max_batch_;
         trace_ = getenv("QZBSC_TRACE_SERVE") != nullptr;
         for (int w = 0; w < workers; ++w) threads_.emplace_back([this] { work(0); });
         if (lane_max) threads_.emplace_back([this, lane_max] { work(lane_max); });
@@ -119,6 +123,8 @@ public:
 
     int submit(std::unique_ptr<Job> j)
     {
+        for (int sz : j->sizes)
+            if ((uint64_t)sz > max_block_) return QZ_NOT_ENOUGH_MEMORY;  // a block beyond the workspace chunk
         // cut into units of whole blocks
         std::vector<Unit> units;
         uint64_t off = 0;
@@ -188,7 +194,7 @@ private:
         SlotGuard g;
         {
             int rc = g.ok() ? 0 : QZ_GPU_ERROR;
-            if (rc == 0) rc = g.slot().fwd.reserve(cap, cap_blk);
+            if (rc == 0) rc = g.slot().fwd.reserve(cap, cap_blk, std::min(cap, max_block_));
             if (rc == 0) rc = g.slot().fwd.reserve_depths();  // no allocation once serving
             if (rc == 0) rc = g.slot().inv.reserve(cap, cap_blk);
             std::lock_guard<std::mutex> lk(mu_);
@@ -302,7 +308,8 @@ private:
             t = x;
             return 0;
         };
-        QZ_TRY(kind == 0 ? sl.fwd.reserve(n, nblk) : sl.inv.reserve(n, nblk));  // no growth: n, nblk within the cap
+        // no growth: n, nblk within the cap, blocks within the chunk
+        QZ_TRY(kind == 0 ? sl.fwd.reserve(n, nblk, BwtEngine::max_block(offs)) : sl.inv.reserve(n, nblk));
         uint8_t* d_in = kind == 0 ? sl.fwd.text_ : sl.inv.U_;
         uint8_t* d_out = kind == 0 ? sl.fwd.out_ : sl.inv.T_;
         uint64_t o = 0;
@@ -341,6 +348,7 @@ private:
     std::unordered_map<int, std::unique_ptr<Job>> jobs_;
     std::vector<std::thread> threads_;
     uint64_t budget_, max_batch_, unit_max_;
+    uint64_t max_block_ = 0;
     int policy_;
     bool trace_ = false;
     int ready_ = 0, init_rc_ = 0;

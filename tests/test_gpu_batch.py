@@ -53,7 +53,7 @@ def test_alphabet_class_split_matches_oracle(gpu, capfd, monkeypatch):
     out, prim, aux = bwt.encode_batch(blocks)
     err = capfd.readouterr().err
     monkeypatch.delenv("QZBSC_TRACE")
-    assert err.count("qzbsc: class of") >= 3, err[-2000:]
+    assert err.count("qzbsc: sub-batch of") >= 3, err[-2000:]
     for i, blk in enumerate(blocks):
         u, p, a = O.bwt_aux(blk)
         assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a), f"block {i}"

@@ -263,3 +263,39 @@ def test_tuner_knobs_settle_and_never_change_results(gpu, monkeypatch):
         monkeypatch.delenv(env)
         for i, (u, p, a) in enumerate(want):
             assert np.array_equal(o2[i], u) and p2[i] == p and np.array_equal(a2[i], a), f"{env}={val} block {i}"
+
+
+def test_chunked_workspace_matches_oracle(gpu, monkeypatch):
+    """The sort workspace holds one chunk of whole blocks; a larger batch is
+    sorted chunk by chunk. With a 4 MiB chunk forced on a fresh engine: a
+    24 MB batch of reads (contiguous chunks) and a mixed batch (alphabet
+    classes, each cut into chunks) equal the oracle block by block, the
+    device holds about a third of the unchunked workspace for the reads,
+    and the inverse gives the blocks back."""
+    from qz_amd import bwt, serve
+    lib = gpu.lib()
+    reads = synth.reads_50mbp_stream(200_000, seed=41)
+    seq = [np.ascontiguousarray(reads[i * 2_000_000:(i + 1) * 2_000_000 - 7 * i]) for i in range(12)]
+    mixed, _ = synth.mixed_blocks(24, seed=42, lo=200_000, hi=1_800_000)
+    monkeypatch.setenv("QZBSC_CHUNK_MIB", "4")
+    try:
+        for name, blocks in (("reads", seq), ("mixed", mixed)):
+            n = sum(b.size for b in blocks)
+            lib.qzbsc_trim()
+            free0, _ = serve.device_memory()
+            monkeypatch.setenv("QZBSC_TRACE", "1")
+            out, prim, aux = bwt.encode_batch(blocks)
+            monkeypatch.delenv("QZBSC_TRACE")
+            free1, _ = serve.device_memory()
+            per = (free0 - free1) / n
+            print(f"{name}: {len(blocks)} blocks, {n} B, {per:.2f} B per input byte resident", flush=True)
+            if name == "reads":
+                assert per < 16.0, per  # 31 without chunks
+            for i, blk in enumerate(blocks):
+                u, p, a = O.bwt_aux(blk)
+                assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a), f"{name} block {i}"
+            back = bwt.decode_batch(out, prim)
+            assert all(np.array_equal(back[i], b) for i, b in enumerate(blocks))
+    finally:
+        monkeypatch.delenv("QZBSC_CHUNK_MIB")
+        lib.qzbsc_trim()
