@@ -513,7 +513,7 @@ def main():
 
     import torch
     import qz_amd
-    from qz_amd import synth, bwt
+    from qz_amd import synth, bwt, serve
 
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
@@ -702,6 +702,13 @@ def main():
         "streams": streams,
         "archive": archive,
         "tuning": bwt.tuning(),
+        # device bytes per input byte that one call over the headline batch
+        # holds (qzbsc_footprint): staging scales with the batch, the forward
+        # sort workspace with one 512 MiB chunk of whole blocks
+        "footprint_b_per_input_b": {
+            "forward": round(serve.footprint(0, total, len(blocks)) / total, 2),
+            "forward_one_25MiB_block": round(serve.footprint(0, synth.BLOCK, 1) / synth.BLOCK, 2),
+            "inverse": round(serve.footprint(1, total, len(blocks)) / total, 2)},
     }
     if per_block:
         out["batched_vs_per_block"] = round(value / per_block["value"], 3)

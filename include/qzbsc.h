@@ -218,7 +218,10 @@ QZBSC_API void qzbsc_host_free(void* p);
  * tests of resident memory. No counterpart in the reference. */
 QZBSC_API int qzbsc_device_memory(long long* free_bytes, long long* total_bytes);
 /* Device bytes one batch of `bytes` input in `blocks` blocks needs:
- * kind 0 forward BWT, kind 1 inverse BWT. */
+ * kind 0 forward BWT, kind 1 inverse BWT. The forward's sort workspace
+ * covers one chunk of whole blocks (512 MiB, QZBSC_CHUNK_MIB) and the
+ * figure assumes no block is larger; only its staging grows with the
+ * batch. */
 QZBSC_API long long qzbsc_footprint(int kind, long long bytes, int blocks);
 
 /* Radix digit-width tuner (no counterpart in the reference): which = 0 for
