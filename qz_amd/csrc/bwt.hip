@@ -429,7 +429,7 @@ __device__ __forceinline__ void store_keys(uint32_t* __restrict__ keys, uint32_t
 // before i (unused at a block start).
 __device__ __forceinline__ uint32_t sa_entry(uint32_t i, uint32_t bo, uint32_t cb, const uint16_t* code, uint8_t prev)
 {
-    return cb ? ((i - bo) << 4) | (i > bo ? (uint32_t)code[prev] : 0u) : i;
+    return cb ? ((i - bo) << cb) | (i > bo ? (uint32_t)code[prev] : 0u) : i;
 }
 
 __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restrict__ text, uint32_t n,
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(kThreads) void k_block_runs(const uint8_t* __restri
 // run-length keys, 1 always).
 __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restrict__ hist, PackParams* __restrict__ pp,
                                                         int force, BlockMap bm, const uint32_t* __restrict__ brun,
-                                                        uint32_t small_blocks)
+                                                        uint32_t maxb_bits)
 {
     __shared__ uint32_t any_rle;
     if (threadIdx.x == 0) any_rle = 0;
@@ -704,8 +704,10 @@ __global__ __launch_bounds__(kThreads) void k_make_pack(const uint32_t* __restri
         }
         pp->radix = (uint32_t)radix;
         pp->k0 = k ? k : 1;
-        // char mode: codes 1..15 and block-local positions below 2^28
-        pp->cb = (radix <= 16 && small_blocks) ? 4u : 0u;
+        // char mode: codes 1..15 in 4 bits, or 1..63 in 6, beside
+        // block-local positions that fit the other bits of the entry
+        const uint32_t cbits = radix <= 16 ? 4u : radix <= 64 ? 6u : 0u;
+        pp->cb = (cbits && maxb_bits + cbits <= 32u) ? cbits : 0u;
         const bool rle = force >= 0 ? force != 0 : any_rle != 0;
         uint32_t sb = 0;
         while ((1ull << sb) < radix) ++sb;  // codes 0..sigma
@@ -1826,10 +1828,11 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out_chars(const uint8_t* __res
                                                             uint8_t* __restrict__ out)
 {
     if (!ppd->cb) return;  // uniform: k_bwt_out serves the batch
-    __shared__ uint8_t dec[16];
+    __shared__ uint8_t dec[64];
+    const uint32_t cmask = (1u << ppd->cb) - 1u;
     {
         const uint32_t cd = ppd->code[threadIdx.x];
-        if (cd != 0u && cd < 16u) dec[cd] = (uint8_t)threadIdx.x;
+        if (cd != 0u && cd <= cmask) dec[cd] = (uint8_t)threadIdx.x;
     }
     __syncthreads();
     const uint32_t j0 = (blockIdx.x * kThreads + threadIdx.x) * kOutItems;
@@ -1866,7 +1869,7 @@ __global__ __launch_bounds__(kThreads) void k_bwt_out_chars(const uint8_t* __res
                 p = rank[o] - o + 1u;
             }
             const uint32_t local = j - o;
-            c = local == 0 ? text[e - 1u] : dec[(local < p ? prev : ent[q]) & 15u];
+            c = local == 0 ? text[e - 1u] : dec[(local < p ? prev : ent[q]) & cmask];
             prev = ent[q];
         }
         w[q / 4] |= c << (8 * (q % 4));
@@ -2401,9 +2404,9 @@ int BwtEngine::enqueue_round0(const uint8_t* text, uint32_t n, uint32_t nblk, in
     QZ_HIP(hipGetLastError());
     uint32_t maxb = 0;
     for (uint32_t b = 0; b < nblk; ++b) maxb = std::max(maxb, offs_host_[b + 1] - offs_host_[b]);
-    const uint32_t small_blocks = maxb < (1u << 28) ? 1u : 0u;  // block-local positions fit 28 bits
+    const uint32_t maxb_bits = (uint32_t)bitlen(maxb);  // bits of the largest block-local position + 1
     hipLaunchKernelGGL(k_make_pack, dim3(1), dim3(kThreads), 0, s, hist_, (PackParams*)ppd_, rle_force(), bmap_,
-                       (const uint32_t*)brun_, small_blocks);
+                       (const uint32_t*)brun_, maxb_bits);
     QZ_HIP(hipGetLastError());
     // positional keys in the group list buffer (free until the first
     // doubling round): the radix sort reads them in its first pass and

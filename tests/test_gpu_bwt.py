@@ -299,3 +299,27 @@ def test_chunked_workspace_matches_oracle(gpu, monkeypatch):
     finally:
         monkeypatch.delenv("QZBSC_CHUNK_MIB")
         lib.qzbsc_trim()
+
+
+def test_six_bit_char_mode_names(gpu, capfd, monkeypatch):
+    """Names (about 40 symbols) carry their preceding symbol in 6-bit codes
+    in the SA entries (block-local positions below 2^26), so their BWT
+    output is a streaming pass: the sub-batches report char mode 6 and every
+    block equals the oracle."""
+    from qz_amd import bwt
+    names = synth.header_stream(160_000, 43)
+    blocks = [np.ascontiguousarray(names[i * 2_500_001:(i + 1) * 2_500_001]) for i in range(4)]
+    monkeypatch.setenv("QZBSC_CHUNK_MIB", "6")
+    monkeypatch.setenv("QZBSC_TRACE", "1")
+    gpu.lib().qzbsc_trim()
+    try:
+        out, prim, aux = bwt.encode_batch(blocks)
+    finally:
+        monkeypatch.delenv("QZBSC_TRACE")
+        monkeypatch.delenv("QZBSC_CHUNK_MIB")
+        gpu.lib().qzbsc_trim()
+    err = capfd.readouterr().err
+    assert "char mode 6" in err, err[-1500:]
+    for i, blk in enumerate(blocks):
+        u, p, a = O.bwt_aux(blk)
+        assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a), f"block {i}"
