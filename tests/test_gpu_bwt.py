@@ -323,3 +323,27 @@ def test_six_bit_char_mode_names(gpu, capfd, monkeypatch):
     for i, blk in enumerate(blocks):
         u, p, a = O.bwt_aux(blk)
         assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a), f"block {i}"
+
+
+def test_char_mode_limits(gpu):
+    """Edges of the char-mode SA entries: 63 distinct symbols take 6-bit
+    codes, 64 fall back to global positions; a name block of 2^26 - 1 bytes
+    (the largest whose block-local positions leave 6 bits free) and one of
+    2^26 bytes (which cannot) both equal the oracle."""
+    from qz_amd import bwt
+    rng = np.random.default_rng(44)
+    for sigma in (63, 64):
+        alpha = rng.choice(256, size=sigma, replace=False).astype(np.uint8)
+        blk = alpha[np.minimum(rng.geometric(0.05, size=1_000_003) - 1, sigma - 1)]
+        assert np.unique(blk).size == sigma
+        out, prim, aux = bwt.encode_batch([blk, blk[:300_001].copy()])
+        for i, b in enumerate((blk, blk[:300_001])):
+            u, p, a = O.bwt_aux(b)
+            assert np.array_equal(out[i], u) and prim[i] == p and np.array_equal(aux[i], a), f"sigma {sigma} block {i}"
+    names = synth.header_stream(1_100_000, 45)
+    for n in ((1 << 26) - 1, 1 << 26):
+        blk = np.ascontiguousarray(names[:n])
+        print("name block", n, flush=True)
+        out, prim, aux = bwt.encode_batch([blk])
+        u, p, a = O.bwt_aux(blk)
+        assert np.array_equal(out[0], u) and prim[0] == p and np.array_equal(aux[0], a), f"name block {n}"
