@@ -186,7 +186,8 @@ QZBSC_API int qzbsc_segment_header(const unsigned char* seg, int size, int* n, u
  * units of whole blocks that workers batch with other jobs' units. Buffers
  * are host memory (pinned buffers from qzbsc_host_alloc copy fastest) and
  * must stay valid until wait returns. Returns a job id (> 0) or < 0
- * (LIBBSC_NOT_ENOUGH_MEMORY for a block larger than the batch cap).
+ * (LIBBSC_NOT_ENOUGH_MEMORY for a block larger than the batch cap or than
+ * the 512 MiB chunk the forward sort workspace is sized for).
  * wait: blocks until every unit of the job is done; returns its status and,
  * optionally, its latency, its queueing time (until its first unit started)
  * and the most jobs any of its batches held. A job can be waited on once. */
