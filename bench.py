@@ -488,7 +488,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-block batch section")
     ap.add_argument("--uneven-blocks", type=int, default=512)
-    ap.add_argument("--batch-cap", type=int, default=2_100_000_000)
+    ap.add_argument("--batch-cap", type=int, default=4_000_000_000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop section")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager-vs-graph section")
     ap.add_argument("--calib-copy", type=int, default=0,

@@ -123,7 +123,10 @@ QZBSC_API int qzbsc_bwt_decode(unsigned char* T, int n, int index, unsigned char
                                int features);
 
 /* Batch forms of the two calls above: nblocks blocks of sizes[b] bytes laid
- * end to end in `in`/`out` (host memory), sorted in one device pass.
+ * end to end in `in`/`out` (host memory), up to 4 GB in all and each block
+ * below 2^31 bytes (LIBBSC_BAD_PARAMETER otherwise). The forward sorts the
+ * batch by alphabet class in chunks of whole blocks, the inverse in
+ * sub-batches of whole blocks below 2^31 bytes.
  * primary[b] receives the block's primary index; aux (nblocks x 256 ints,
  * may be NULL) receives I[k] = 1 + rank of suffix k*r, k = 0..(n-1)/r.
  * No counterpart in the reference: they replace qz's rayon fan-out of

@@ -24,19 +24,41 @@ static int sample_rate(int n)
     return m + 1;
 }
 
+// A batch of the BWT entries holds < kBatchMax bytes (32-bit offsets); the
+// forward sorts it in chunks and the inverse in sub-batches of whole blocks
+// below kSubBatchMax (its row entries carry a 31-bit LF), so blocks are
+// below kSubBatchMax too.
+constexpr uint64_t kBatchMax = 0xffff0000ull;
+constexpr uint64_t kSubBatchMax = 0x7fff0000ull;
+
 static int check_sizes(const int* sizes, int nblocks, std::vector<uint32_t>& offs)
 {
     if (nblocks <= 0 || !sizes) return QZ_BAD_PARAMETER;
     offs.assign(nblocks + 1, 0);
     uint64_t tot = 0;
     for (int b = 0; b < nblocks; ++b) {
-        if (sizes[b] < 1) return QZ_BAD_PARAMETER;
+        if (sizes[b] < 1 || (uint64_t)sizes[b] >= kSubBatchMax) return QZ_BAD_PARAMETER;
         offs[b] = (uint32_t)tot;
         tot += (uint64_t)sizes[b];
-        if (tot >= 0x7fffffffull) return QZ_BAD_PARAMETER;
+        if (tot >= kBatchMax) return QZ_BAD_PARAMETER;
     }
     offs[nblocks] = (uint32_t)tot;
     return 0;
+}
+
+// Runs of consecutive blocks of offs holding < kSubBatchMax bytes each, as
+// (first block, end block).
+static std::vector<std::pair<uint32_t, uint32_t>> sub_batches(const std::vector<uint32_t>& offs)
+{
+    std::vector<std::pair<uint32_t, uint32_t>> parts;
+    const uint32_t nblk = (uint32_t)offs.size() - 1;
+    for (uint32_t b = 0; b < nblk;) {
+        uint32_t e = b + 1;
+        while (e < nblk && (uint64_t)offs[e + 1] - offs[b] < kSubBatchMax) ++e;
+        parts.emplace_back(b, e);
+        b = e;
+    }
+    return parts;
 }
 
 // Forward BWT of a batch whose text already sits in device memory.
@@ -71,11 +93,36 @@ int inv_batch_device(Slot& sl, const uint8_t* d_in, uint8_t* d_out, const std::v
         if (primary[b] < 1 || primary[b] > nb) return QZ_BAD_PARAMETER;
     }
     hipStream_t s = sl.stream;
-    QZ_TRY(sl.inv.reserve(n, nblk));
-    QZ_HIP(hipMemcpyAsync(sl.inv.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
-    QZ_HIP(hipMemcpyAsync(sl.inv.prim_, primary, (size_t)nblk * 4, hipMemcpyHostToDevice, s));
-    QZ_HIP(hipEventRecord(sl.ev0, s));
-    QZ_TRY(sl.inv.decode_device(d_in, d_out, n, nblk, offs, s));
+    if (n < kSubBatchMax) {
+        QZ_TRY(sl.inv.reserve(n, nblk));
+        QZ_HIP(hipMemcpyAsync(sl.inv.off_, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, s));
+        QZ_HIP(hipMemcpyAsync(sl.inv.prim_, primary, (size_t)nblk * 4, hipMemcpyHostToDevice, s));
+        QZ_HIP(hipEventRecord(sl.ev0, s));
+        QZ_TRY(sl.inv.decode_device(d_in, d_out, n, nblk, offs, s));
+    } else {
+        // sub-batches of whole blocks, each with its own offsets; the
+        // workspace is reserved for the largest
+        const auto parts = sub_batches(offs);
+        uint64_t mn = 0;
+        uint32_t mb = 0;
+        for (const auto& p : parts) {
+            mn = std::max<uint64_t>(mn, offs[p.second] - offs[p.first]);
+            mb = std::max(mb, p.second - p.first);
+        }
+        QZ_TRY(sl.inv.reserve(mn, mb));
+        QZ_HIP(hipEventRecord(sl.ev0, s));
+        std::vector<uint32_t> lo;
+        for (const auto& p : parts) {
+            const uint32_t base = offs[p.first];
+            lo.assign(offs.begin() + p.first, offs.begin() + p.second + 1);
+            for (uint32_t& o : lo) o -= base;
+            QZ_HIP(hipMemcpyAsync(sl.inv.off_, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, s));
+            QZ_HIP(hipMemcpyAsync(sl.inv.prim_, primary + p.first, (size_t)(p.second - p.first) * 4,
+                                  hipMemcpyHostToDevice, s));
+            QZ_TRY(sl.inv.decode_device(d_in + base, d_out + base, lo.back(), p.second - p.first, lo, s));
+            QZ_HIP(hipStreamSynchronize(s));  // lo is rewritten for the next sub-batch
+        }
+    }
     QZ_HIP(hipEventRecord(sl.ev1, s));
     QZ_HIP(hipStreamSynchronize(s));
     if (gpu_ms) QZ_HIP(hipEventElapsedTime(gpu_ms, sl.ev0, sl.ev1));
@@ -115,12 +162,17 @@ QZBSC_API int qzbsc_bwt_decode_batch(const unsigned char* in, unsigned char* out
     SlotGuard g;
     if (!g.ok()) return QZ_GPU_ERROR;
     Slot& sl = g.slot();
-    const uint32_t n = offs.back();
-    QZ_TRY(sl.inv.reserve(n, (uint32_t)nblocks));
-    QZ_HIP(hipMemcpyAsync(sl.inv.U_, in, n, hipMemcpyHostToDevice, sl.stream));
-    QZ_TRY(inv_batch_device(sl, sl.inv.U_, sl.inv.T_, offs, primary, nullptr));
-    QZ_HIP(hipMemcpyAsync(out, sl.inv.T_, n, hipMemcpyDeviceToHost, sl.stream));
-    QZ_HIP(hipStreamSynchronize(sl.stream));
+    // through the engine's staging one sub-batch of whole blocks at a time
+    for (const auto& p : sub_batches(offs)) {
+        const uint32_t base = offs[p.first], n = offs[p.second] - base;
+        std::vector<uint32_t> lo(offs.begin() + p.first, offs.begin() + p.second + 1);
+        for (uint32_t& o : lo) o -= base;
+        QZ_TRY(sl.inv.reserve(n, p.second - p.first));
+        QZ_HIP(hipMemcpyAsync(sl.inv.U_, in + base, n, hipMemcpyHostToDevice, sl.stream));
+        QZ_TRY(inv_batch_device(sl, sl.inv.U_, sl.inv.T_, lo, primary + p.first, nullptr));
+        QZ_HIP(hipMemcpyAsync(out + base, sl.inv.T_, n, hipMemcpyDeviceToHost, sl.stream));
+        QZ_HIP(hipStreamSynchronize(sl.stream));
+    }
     return 0;
 }
 

@@ -1912,8 +1912,9 @@ constexpr uint32_t kClassWords = 10;  // per block: 8 presence words, run bounda
 __global__ __launch_bounds__(kThreads) void k_block_class(const uint8_t* __restrict__ text, uint32_t n, BlockMap bm,
                                                           uint32_t* __restrict__ cls)
 {
-    const uint32_t i0 = (blockIdx.x * kThreads + threadIdx.x) * kClassSample;
-    const bool live = i0 < n;
+    const uint64_t i64 = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) * kClassSample;
+    const bool live = i64 < n;  // batches reach 4 GB: no 32-bit wrap
+    const uint32_t i0 = live ? (uint32_t)i64 : 0u;
     const uint32_t b = live ? bm.find(i0) : 0xffffffffu;
     uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t cnt = 0, seen = 0;
@@ -2227,10 +2228,14 @@ static uint64_t chunk_bytes()
 uint64_t BwtEngine::chunk() { return chunk_bytes(); }
 
 // Workspace elements for a batch of n bytes whose largest block has maxblk.
+// A chunk stays below 2^31 bytes (positions and SA entries are u32, a
+// batch of the BWT entries may hold up to 4 GB).
+constexpr uint64_t kChunkMax = 0x7fff0000ull;
 static uint64_t work_cap(uint64_t n, uint64_t maxblk)
 {
     const uint64_t c = chunk_bytes();
-    return c == 0 ? n : std::min<uint64_t>(n, std::max<uint64_t>(c, maxblk));
+    const uint64_t w = c == 0 ? n : std::min<uint64_t>(n, std::max<uint64_t>(c, maxblk));
+    return std::min<uint64_t>(w, std::max<uint64_t>(kChunkMax, maxblk));
 }
 
 uint64_t BwtEngine::footprint(uint64_t n, uint32_t nblk, uint64_t maxblk)

@@ -1,7 +1,9 @@
 """The batched uneven launch at its full shape: 512 blocks of log-uniform
 size in 100 KB..25 MiB from a seeded mix of the sequence, header and quality
-generators (2.7 GB at seed 4: the size law gives a mean of 4.7 MB per block, so the "about 4 GB" of the configuration is its rough total), forward-BWT'd in sub-batches of at most 1e9 bytes
-(device batches hold < 2^31 bytes) and inverse-BWT'd.
+generators (2.7 GB at seed 4: the size law gives a mean of 4.7 MB per block, so the "about 4 GB" of the configuration is its rough total), forward-BWT'd in one call
+and inverse-BWT'd in one call (a batch may hold up to 4 GB: the forward sorts
+it by alphabet class in chunks of whole blocks, the inverse in sub-batches
+of whole blocks below 2^31 bytes).
 
 Every block: its BWT is a permutation of the block, its primary index is the
 first sampled index, and the inverse gives the block back. The smallest and
@@ -15,7 +17,7 @@ from qz_amd import bwt, synth
 
 pytestmark = pytest.mark.gpu
 
-CAP = 1_000_000_000
+CAP = 4_000_000_000  # one call for the whole workload
 
 
 def _sub_batches(sizes):
@@ -36,6 +38,7 @@ def test_uneven_512_blocks():
     print(f"uneven 512: {len(blocks)} blocks, {sum(sizes)} B, {min(sizes)}..{max(sizes)} B", flush=True)
     assert len(blocks) == 512 and min(sizes) >= 100_000 and max(sizes) <= synth.BLOCK
     out, prim, aux = [None] * 512, [0] * 512, [None] * 512
+    assert len(_sub_batches(sizes)) == 1
     for g in _sub_batches(sizes):
         o, p, a = bwt.encode_batch([blocks[b] for b in g])
         for i, b in enumerate(g):
