@@ -219,3 +219,31 @@ def test_coder_roundtrip_small_alphabets_and_short_segments():
         r, back = _coder_rt(u)
         if r >= 0:
             assert np.array_equal(back, u), (nsym, n)
+
+
+def test_batch_size_limits_are_checked_before_any_device_work():
+    """The BWT batch entries take blocks of 1 .. 2^31 - 65537 bytes and
+    batches below 2^32 - 65536 bytes in total; anything else is refused
+    before a buffer is read or a device touched (so the tiny buffers here
+    are never read)."""
+    L = qz_amd.lib()
+    QZ_BAD = -1  # QZ_BAD_PARAMETER
+    buf = np.zeros(64, np.uint8)
+    bp = buf.ctypes.data_as(u8p)
+    prim = np.ones(8, np.int32)
+    aux = np.zeros(8 * 256, np.int32)
+    pp, ap = prim.ctypes.data_as(i32p), aux.ctypes.data_as(i32p)
+    cases = [
+        [0],                                   # empty block
+        [-5],
+        [0x7fff0000],                          # one block at the sub-batch limit
+        [0x7ffeffff, 0x7ffeffff, 0x20000],     # total at 2^32 - 65536 + 0x1fffe
+        [0x7ffeffff, 0x7ffeffff, 0x10002],     # total exactly 2^32 - 65536
+    ]
+    for sizes in cases:
+        s = np.array(sizes, np.int32)
+        sp = s.ctypes.data_as(i32p)
+        assert L.qzbsc_bwt_encode_batch(bp, bp, sp, len(sizes), pp, ap) == QZ_BAD, sizes
+        assert L.qzbsc_bwt_decode_batch(bp, bp, sp, len(sizes), pp) == QZ_BAD, sizes
+    s = np.array([5], np.int32)
+    assert L.qzbsc_bwt_encode_batch(bp, bp, s.ctypes.data_as(i32p), 0, pp, ap) == QZ_BAD  # no blocks
