@@ -628,9 +628,10 @@ __global__ __launch_bounds__(kThreads) void k_round0_keys(const uint8_t* __restr
 // Symbol packing from the byte histogram (one workgroup of 256): present
 // symbols get codes 1..sigma in byte order, 0 means past the block end, and
 // k0 is the largest k with (sigma+1)^k <= 2^32.
-// Sampled run boundaries per block: one thread per 16-byte chunk at every
-// kRunSample-th position counts the chunk's positions whose successor in the
-// block differs (brun[b]) and the positions it looked at (brun[nblk + b]);
+// Sampled run boundaries per block: one thread per kRunSample-th position
+// counts, in the aligned 16-byte window holding it, the neighbouring pairs
+// inside the block that differ (brun[b]) and the pairs it looked at
+// (brun[nblk + b]);
 // summed per wave when the wave lies in one block. The key mode only needs
 // the mean run length, which a 1/64 sample estimates well.
 constexpr uint32_t kRunSample = 1024;
@@ -642,12 +643,27 @@ __global__ __launch_bounds__(kThreads) void k_block_runs(const uint8_t* __restri
     const uint32_t b = live ? bm.find(i0) : 0xffffffffu;
     uint32_t cnt = 0, seen = 0;
     if (live) {
-        const uint32_t end = bm.off[b + 1];
+        // one 16-byte load: the aligned window that holds position i0 (it
+        // holds a byte of the text, so it lies in the text's pages); the
+        // neighbouring pairs of the window inside i0's block
+        const uint32_t bs = bm.off[b], be = bm.off[b + 1];
+        const uint32_t mis = (uint32_t)((uintptr_t)(text + i0) & 15u);
+        const uint4 v = *reinterpret_cast<const uint4*>(text + i0 - mis);
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+        const int64_t p0 = (int64_t)i0 - mis;
+        uint32_t prev = 0;
+        bool have = false;
+#pragma unroll
         for (uint32_t q = 0; q < 16; ++q) {
-            const uint32_t i = i0 + q;
-            if (i + 1 >= end) break;  // the sample stays inside its block
-            cnt += text[i + 1] != text[i] ? 1u : 0u;
-            ++seen;
+            const int64_t p = p0 + q;
+            if (p < (int64_t)bs || p >= (int64_t)be) continue;
+            const uint32_t c = (w4[q >> 2] >> (8u * (q & 3u))) & 255u;
+            if (have) {
+                cnt += c != prev ? 1u : 0u;
+                ++seen;
+            }
+            prev = c;
+            have = true;
         }
     }
     const uint32_t b0 = __shfl(b, 0, QZ_WAVE);
